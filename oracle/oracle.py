@@ -1,0 +1,179 @@
+"""ctypes bindings of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+never by zinq_amd.  See oracle.c for the reference file:line each function restates.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+c_int, c_double, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _lib = ctypes.CDLL(path)
+        L = _lib
+        L.orc_get_fock.argtypes = [c_int, vp, vp, vp, c_int, vp]
+        L.orc_get_energy.argtypes = [c_int, vp, vp, vp]
+        L.orc_get_energy.restype = c_double
+        L.orc_get_density.argtypes = [c_int, vp, c_int, c_int, vp]
+        L.orc_get_density.restype = c_double
+        L.orc_get_error.argtypes = [c_int, vp, vp, vp, vp]
+        L.orc_diis.argtypes = [c_int, c_int, vp, vp, vp]
+        L.orc_geigh.argtypes = [c_int, vp, vp, vp, vp]
+        L.orc_eigh.argtypes = [c_int, vp, vp, vp]
+        L.orc_scf.argtypes = [c_int, c_int, c_int, vp, vp, vp, c_double, c_int, c_int, c_double, vp, vp, vp, vp, vp, vp]
+        L.orc_ao2mo_pp.argtypes = [c_int, vp, vp, vp]
+        L.orc_ao2mo_pppp.argtypes = [c_int, vp, vp, vp]
+        L.orc_ao2mo_oovv.argtypes = [c_int, c_int, vp, vp, vp]
+        L.orc_ao2so_coef.argtypes = [c_int, vp, vp]
+        L.orc_ao2so_pp.argtypes = [c_int, vp, vp]
+        L.orc_ao2so_pppp.argtypes = [c_int, vp, vp]
+        L.orc_mp.argtypes = [c_int, c_int, vp, vp, vp, c_int, c_int, vp]
+        L.orc_mp2_closed.argtypes = [c_int, c_int, vp, vp]
+        L.orc_mp2_closed.restype = c_double
+        L.orc_ci.argtypes = [c_int, vp, vp, vp, c_int, c_int, vp, c_int, c_double, c_int, vp]
+    return _lib
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.float64, np.int32)
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def get_fock(g, H, P, generalized=False):
+    n = H.shape[0]
+    F = np.zeros((n, n))
+    lib().orc_get_fock(n, _p(_c(g)), _p(_c(H)), _p(_c(P)), int(generalized), _p(F))
+    return F
+
+
+def get_energy(H, F, P):
+    return lib().orc_get_energy(H.shape[0], _p(_c(H)), _p(_c(F)), _p(_c(P)))
+
+
+def get_density(C, nocc, generalized=False, P_old=None):
+    n = C.shape[0]
+    P = np.zeros((n, n)) if P_old is None else _c(P_old).copy()
+    rms = lib().orc_get_density(n, _p(_c(C)), nocc, int(generalized), _p(P))
+    return P, rms
+
+
+def get_error(F, P, S):
+    n = F.shape[0]
+    e = np.zeros((n, n))
+    lib().orc_get_error(n, _p(_c(F)), _p(_c(P)), _p(_c(S)), _p(e))
+    return e
+
+
+def diis(fhist, ehist):
+    """Returns (F, ok); ok is False when the LU solve fails (src/hartree_fock.zig:821)."""
+    m, n = len(fhist), fhist[0].shape[0]
+    fh, eh = _c(np.stack(fhist)), _c(np.stack(ehist))
+    F = np.zeros((n, n))
+    rc = lib().orc_diis(m, n, _p(fh), _p(eh), _p(F))
+    return F, rc == 0
+
+
+def geigh(A, B):
+    n = A.shape[0]
+    e, C = np.zeros(n), np.zeros((n, n))
+    rc = lib().orc_geigh(n, _p(_c(A)), _p(_c(B)), _p(e), _p(C))
+    if rc:
+        raise RuntimeError("LapackError")
+    return e, C
+
+
+def eigh(A):
+    n = A.shape[0]
+    e, V = np.zeros(n), np.zeros((n, n))
+    lib().orc_eigh(n, _p(_c(A).copy()), _p(e), _p(V))
+    return e, V
+
+
+class ScfDidNotConverge(RuntimeError):
+    pass
+
+
+def scf(H, S, g, VN, nocc, generalized=False, diis=8, iterations=100, threshold=1e-8):
+    """src/hartree_fock.zig:738-875 with the core guess of :425-431.  Returns dict(E, C, e, P, F, iterations)."""
+    n = H.shape[0]
+    E = np.zeros(1)
+    C, e, P, F = np.zeros((n, n)), np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
+    it = np.zeros(1, dtype=np.int32)
+    rc = lib().orc_scf(n, nocc, int(generalized), _p(_c(H)), _p(_c(S)), _p(_c(g)), VN, diis or 0, iterations, threshold,
+                       _p(E), _p(C), _p(e), _p(P), _p(F), _p(it))
+    if rc == 1:
+        raise ScfDidNotConverge()
+    if rc == 2:
+        raise RuntimeError("LapackError")
+    return dict(E=float(E[0]), C=C, e=e, P=P, F=F, iterations=int(it[0]))
+
+
+def ao2mo_pp(A, C):
+    N = A.shape[0]
+    out = np.zeros((N, N))
+    lib().orc_ao2mo_pp(N, _p(_c(A)), _p(_c(C)), _p(out))
+    return out
+
+
+def ao2mo_pppp(g, C):
+    N = C.shape[0]
+    out = np.zeros((N,) * 4)
+    lib().orc_ao2mo_pppp(N, _p(_c(g)), _p(_c(C)), _p(out))
+    return out
+
+
+def ao2mo_oovv(g, C, nocc):
+    N = C.shape[0]
+    out = np.zeros((nocc, nocc, N - nocc, N - nocc))
+    lib().orc_ao2mo_oovv(N, nocc, _p(_c(g)), _p(_c(C)), _p(out))
+    return out
+
+
+def ao2so_coef(C):
+    n = C.shape[0]
+    out = np.zeros((2 * n, 2 * n))
+    lib().orc_ao2so_coef(n, _p(_c(C)), _p(out))
+    return out
+
+
+def mp(order, g, C, e, nocc, generalized=False):
+    """src/moller_plesset.zig:340-481; returns energies[0..order]."""
+    n = C.shape[0]
+    out = np.zeros(order + 1)
+    lib().orc_mp(order, n, _p(_c(g)), _p(_c(C)), _p(_c(e)), nocc, int(generalized), _p(out))
+    return out
+
+
+def mp2_closed(oovv, e):
+    o, _, v, _ = oovv.shape
+    return lib().orc_mp2_closed(o, v, _p(_c(oovv)), _p(_c(e)))
+
+
+def ci(g, H, C, nocc, excitations, VN, nstates=5, generalized=False):
+    """src/configuration_interaction.zig:597-683; lowest nstates total energies."""
+    n = C.shape[0]
+    exc = np.ascontiguousarray(excitations, dtype=np.int32)
+    out = np.zeros(nstates)
+    lib().orc_ci(n, _p(_c(g)), _p(_c(H)), _p(_c(C)), nocc, int(generalized), _p(exc), len(exc), VN, nstates, _p(out))
+    return out

@@ -1,0 +1,91 @@
+/*
+ * zinq_hip.h -- C ABI of the MI355X hot path: J/K Fock build over stored AO ERIs and the AO->MO
+ * four-index transform (plus the MP2 energy that consumes it).
+ *
+ * Plain C99 (size_t / int / double only) so that Zig can pull it in with @cImport next to the
+ * reference's other C headers (src/cimport.zig:3-8):
+ *     pub const zinqhip = @cImport(@cInclude("zinq_hip.h"));
+ *
+ * Conventions (those of the reference's libint shim, src/libint.h / src/libint.cpp):
+ *   - row-major FP64 arrays; host buffers are owned by the caller, device memory by the context;
+ *   - every call is synchronous: results are in the caller's buffer when it returns;
+ *   - functions return ZQ_OK (0) or a nonzero status that the Zig wrapper maps to an error
+ *     (ZQ_ERR_INVALID -> error.InvalidInput, ZQ_ERR_OOM -> error.GpuOutOfMemory,
+ *      ZQ_ERR_LAUNCH -> error.GpuLaunch, ZQ_ERR_NOT_CONVERGED -> error.ScfDidNotConverge,
+ *      ZQ_ERR_LAPACK -> error.LapackError); zq_last_error() gives the message;
+ *   - one context per host thread; contexts on one GPU run on separate HIP streams.
+ * Dense ERIs are in the layout src/libint.cpp:143-150 writes: g[((p*n+q)*n+r)*n+s] = <pq|rs> = (pr|qs).
+ */
+#ifndef ZINQ_HIP_H
+#define ZINQ_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ZQ_OK = 0,
+  ZQ_ERR_INVALID = 1,
+  ZQ_ERR_OOM = 2,
+  ZQ_ERR_LAUNCH = 3,
+  ZQ_ERR_NOT_CONVERGED = 4,
+  ZQ_ERR_LAPACK = 5,
+  ZQ_ERR_STATE = 6
+};
+
+/* which integrals zq_ao2mo produces */
+enum { ZQ_BLOCK_FULL = 0, ZQ_BLOCK_OVOV = 1 };
+
+/* arithmetic of the quarter transforms; J/K, DIIS, diagonalisation and MP2 sums stay FP64 */
+enum { ZQ_PREC_FP64 = 0, ZQ_PREC_FP32 = 1, ZQ_PREC_BF16X3 = 2 };
+
+typedef struct zq_ctx zq_ctx;
+
+/* Context on HIP device `device` (its own stream).  NULL on failure. */
+zq_ctx *zq_create(int device);
+void zq_destroy(zq_ctx *ctx);
+const char *zq_last_error(const zq_ctx *ctx);
+
+/* Load AO ERIs and keep them resident in HBM, 8-fold packed in 8x8 AO-block quartets.
+ * zq_eri_load takes the dense n^4 tensor exactly as libint_coulomb writes it (src/libint.cpp:161-167,
+ * consumed by src/molecular_system.zig:115-122); zq_eri_load_packed takes canonical packed chemist ERIs
+ * out[I(I+1)/2+J] = (pq|rs), I = p(p+1)/2+q >= J = r(r+1)/2+s (see zinq_ints.h). */
+int zq_eri_load(zq_ctx *ctx, const double *g_dense, size_t n);
+int zq_eri_load_packed(zq_ctx *ctx, const double *g_packed, size_t n);
+
+/* getFock of src/hartree_fock.zig:686-735 (non-DFT branch), all FP64:
+ *   F[k,l] = H[k,l] + sum_ij P[i,j] * (g[i,k,j,l] - x_exch * g[i,j,k,l]),  then F <- (F + F^T)/2,
+ * with x_exch = 0.5 for RHF and 1.0 for GHF (spin-blocked ERIs loaded).  H, P, F_out are n x n. */
+int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double x_exch, double *F_out);
+
+/* AO->MO transform of the loaded ERIs with MO coefficients C (n x n, columns = MOs).
+ *   ZQ_BLOCK_FULL: out[p,q,r,s] = <pq|rs> for all MOs, the result of ao2mo_pppp
+ *                  (src/integral_transform.zig:37-79); out has n^4 entries.
+ *   ZQ_BLOCK_OVOV: out[i,j,a,b] = <ij|ab> = (ia|jb), i,j < nocc, a,b >= nocc, the result of ao2mo_oovv
+ *                  (src/integral_transform.zig:165-219); out has nocc^2 (n-nocc)^2 entries.
+ * precision: ZQ_PREC_FP64 | ZQ_PREC_FP32 | ZQ_PREC_BF16X3 (quarter transforms only). */
+int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out);
+
+/* Closed-shell MP2 correlation energy from the OVOV transform, FP64 denominators and summation:
+ *   E2 = sum_ijab (ia|jb) [2 (ia|jb) - (ib|ja)] / (e_i + e_j - e_a - e_b),
+ * equal to the order-2 term of mp() (src/moller_plesset.zig:340-481) for an RHF reference. */
+int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr);
+
+/* ---- measurement hooks (no counterpart in the reference) ----
+ * With profiling on, every kernel launch is bracketed by HIP events on the context's stream;
+ * zq_profile_get returns the summed milliseconds and launch count of kernels whose name starts
+ * with `prefix` since the last zq_profile_reset. */
+void zq_profile_enable(zq_ctx *ctx, int on);
+void zq_profile_reset(zq_ctx *ctx);
+int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches);
+
+/* Device memory held by the context, in bytes (ERIs + scratch). */
+size_t zq_device_bytes(const zq_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZINQ_HIP_H */

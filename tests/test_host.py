@@ -1,0 +1,143 @@
+"""CPU tests of the host side: the C ABI libraries load and export what include/*.h declares, the host
+FP64 eigensolver used by the SCF driver, the J/K block-quartet algorithm (numpy emulation of the
+kernel's arithmetic against the oracle's getFock), and the synthetic-molecule generator."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, system_ints
+from oracle import oracle as O
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//.*", "", text)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text)
+    return sorted(set(n for n in names if not n.startswith("__")))
+
+
+@pytest.mark.parametrize("header,lib", [("zinq_hip.h", "libzinq_hip.so"), ("zinq_scf.h", "libzinq_hip.so"),
+                                        ("zinq_ints.h", "libzinq_ints.so")])
+def test_library_exports_every_declared_symbol(header, lib):
+    names = _declared(header)
+    assert len(names) >= 2
+    L = ctypes.CDLL(os.path.join(ROOT, "zinq_amd", lib))
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_host_sygv_matches_numpy():
+    from zinq_amd import hip
+    for kind in ("sto3g", "ccpvdz"):
+        s, H, S, g, VN = system_ints(kind)
+        w, C = hip.host_sygv(H, S)
+        np.testing.assert_allclose(H @ C, S @ C * w, atol=1e-11)
+        np.testing.assert_allclose(C.T @ S @ C, np.eye(s.nbf), atol=1e-12)
+        np.testing.assert_allclose(w, O.geigh(H, S)[0], atol=1e-11)
+    # clustered / degenerate spectrum
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    d = np.repeat(np.arange(10.0), 4)
+    A = Q @ np.diag(d) @ Q.T
+    w, V = hip.host_sygv(A, np.eye(40))
+    np.testing.assert_allclose(w, d, atol=1e-12)
+    np.testing.assert_allclose(A @ V, V * w, atol=1e-12)
+
+
+# ---------------------------------------------------------------- J/K algorithm emulation
+B = 8
+
+
+def _bpair(a, b):
+    return a * (a + 1) // 2 + b
+
+
+def _kets(bp, bq):
+    return [(br, bs) for bs in range(bp + 1) for br in range(bs, (bp if bs <= bq else bp - 1) + 1)]
+
+
+def _pack(chem, n):
+    nb = (n + B - 1) // B
+    g = np.zeros((nb * B,) * 4)
+    g[:n, :n, :n, :n] = chem
+    Nb = nb * (nb + 1) // 2
+    blocks = np.zeros((Nb * (Nb + 1) // 2, 64, 64))
+    r, s, p, q = np.meshgrid(*[np.arange(8)] * 4, indexing="ij")
+    for bp in range(nb):
+        for bq in range(bp + 1):
+            Ib = _bpair(bp, bq)
+            for k, (br, bs) in enumerate(_kets(bp, bq)):
+                Jb = _bpair(br, bs)
+                keep = ~(((bp == bq) & (p < q)) | ((br == bs) & (r < s)) | ((Ib == Jb) & (p * 8 + q < r * 8 + s)))
+                vals = g[bp * 8 + p, bq * 8 + q, br * 8 + r, bs * 8 + s] * keep
+                blocks[Ib * (Ib + 1) // 2 + k] = vals.reshape(64, 64)
+    return blocks, nb
+
+
+def _jk_emulated(blocks, nb, P, H, n, x, CH):
+    Pp = np.zeros((nb * B, nb * B))
+    Pp[:n, :n] = P
+    G = np.zeros_like(Pp)
+    lane = np.arange(64)
+    lp, lq = lane // 8, lane % 8
+    for bp in range(nb):
+        for bq in range(bp + 1):
+            Ib, kl = _bpair(bp, bq), _kets(bp, bq)
+            Pg, Qg = bp * 8 + lp, bq * 8 + lq
+            fI = np.where((bp == bq) & (lp == lq), 0.5, 1.0)
+            for k0 in range(0, len(kl), CH):
+                for k in range(k0, min(len(kl), k0 + CH)):
+                    br, bs = kl[k]
+                    Jb = _bpair(br, bs)
+                    blk = blocks[Ib * (Ib + 1) // 2 + k]
+                    for r in range(8):
+                        for s in range(8):
+                            w = fI * (0.5 if (br == bs and r == s) else 1.0) * np.where((Ib == Jb) & (lane == r * 8 + s), .5, 1)
+                            a = blk[r * 8 + s] * w
+                            R, S = br * 8 + r, bs * 8 + s
+                            np.add.at(G, (Pg, Qg), 2 * Pp[R, S] * a)
+                            G[R, S] += 2 * np.sum(Pp[Pg, Qg] * a)
+                            np.add.at(G, (Qg, S), -x * Pp[Pg, R] * a)
+                            np.add.at(G, (Pg, S), -x * Pp[Qg, R] * a)
+                            np.add.at(G, (Qg, R), -x * Pp[Pg, S] * a)
+                            np.add.at(G, (Pg, R), -x * Pp[Qg, S] * a)
+    F = H + (G + G.T)[:n, :n]
+    return F
+
+
+@pytest.mark.parametrize("kind", ["sto3g", "ccpvdz"])
+def test_block_quartet_jk_algorithm_matches_getfock(kind):
+    s, H, S, g, VN = system_ints(kind)
+    n = s.nbf
+    blocks, nb = _pack(g.transpose(0, 2, 1, 3), n)
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((n, n))
+    P = A + A.T
+    for x, gen in [(0.5, False), (1.0, True)]:
+        F = _jk_emulated(blocks, nb, P, H, n, x, CH=2)
+        np.testing.assert_allclose(F, O.get_fock(g, H, P, gen), atol=1e-12)
+    # every unique integral stored exactly once
+    npair = n * (n + 1) // 2
+    assert np.count_nonzero(blocks) == np.count_nonzero(s.coulomb_packed())
+
+
+def test_water_cluster_generator_is_deterministic():
+    from zinq_amd.molecules import water_cluster
+    Z1, x1 = water_cluster(4)
+    Z2, x2 = water_cluster(4)
+    np.testing.assert_array_equal(x1, x2)
+    assert list(Z1) == [8, 1, 1] * 4
+    # rigid monomers: O-H 0.9571 A, HOH 104.5 deg
+    for m in range(4):
+        o, h1, h2 = x1[3 * m:3 * m + 3]
+        assert np.linalg.norm(h1 - o) == pytest.approx(np.hypot(0.757, 0.587), abs=1e-12)
+        cosang = np.dot(h1 - o, h2 - o) / np.linalg.norm(h1 - o) / np.linalg.norm(h2 - o)
+        assert cosang == pytest.approx((0.587 ** 2 - 0.757 ** 2) / (0.757 ** 2 + 0.587 ** 2), abs=1e-12)
+    golden = os.path.join(ROOT, "tests", "golden", "water_k4.xyz")
+    if os.path.exists(golden):
+        ref = np.array([[float(v) for v in ln.split()[1:]] for ln in open(golden).read().splitlines()[2:]])
+        np.testing.assert_allclose(x1, ref, atol=1e-9)

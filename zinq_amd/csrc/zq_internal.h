@@ -1,0 +1,106 @@
+// Internal declarations shared by the HIP translation units of libzinq_hip.so.
+//
+// Device ERI layout ("block quartets").  AO indices are grouped in blocks of 8 (n padded to npad =
+// 8*nb with zero functions).  A block pair (bp >= bq) has index bpair(bp,bq) = bp(bp+1)/2 + bq; there
+// are Nb = nb(nb+1)/2 of them.  Every unique integral (pq|rs) lives in exactly one block quartet
+// (Ib >= Jb) of 64 x 64 doubles, element [ket local r*8+s][bra local p*8+q] (bra fastest).  Positions
+// that duplicate another element of the same quartet (p < q when bp == bq, r < s when br == bs, and bra
+// local < ket local when Ib == Jb) hold 0.  Quartets are stored by Ib, and for each Ib its Ib+1 ket
+// blocks Jb <= Ib in the order bs ascending, then br ascending, which is the order the J/K kernel walks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/zinq_hip.h"
+
+#define ZQ_BLK 8
+#define ZQ_QUARTET 4096  // 64 x 64 doubles
+
+__host__ __device__ inline int zq_bpair(int a, int b) { return a * (a + 1) / 2 + b; }
+
+// Number of ket blocks in the row of bra block pair (bp,bq) with ket column bs.
+__host__ __device__ inline int zq_run_len(int bp, int bq, int bs) { return (bs <= bq ? bp : bp - 1) - bs + 1; }
+
+// Position of ket block pair (br,bs) in the ket list of bra block pair (bp,bq); requires
+// bpair(br,bs) <= bpair(bp,bq).
+__host__ __device__ inline int zq_ket_pos(int bp, int bq, int br, int bs) {
+  const int extra = bs - bq - 1;
+  return bs * (bp + 1) - bs * (bs - 1) / 2 - (extra > 0 ? extra : 0) + (br - bs);
+}
+
+__host__ __device__ inline size_t zq_blk_index(int bp, int bq, int br, int bs) {
+  const size_t Ib = size_t(zq_bpair(bp, bq));
+  return Ib * (Ib + 1) / 2 + size_t(zq_ket_pos(bp, bq, br, bs));
+}
+
+// Inverse of zq_bpair for small indices.
+__host__ __device__ inline void zq_unpair(int I, int &a, int &b) {
+  a = 0;
+  while ((a + 1) * (a + 2) / 2 <= I) ++a;
+  b = I - a * (a + 1) / 2;
+}
+
+struct JKItem {
+  int Ib, bp, bq, k0, k1, rec0;
+};
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct zq_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ERIs
+  int n = 0, npad = 0, nb = 0, Nb = 0;
+  size_t nquartets = 0;
+  double *eri = nullptr;
+
+  // J/K work decomposition
+  JKItem *d_items = nullptr;
+  int nitems = 0;
+  size_t nrecords = 0;
+  double *d_records = nullptr;
+  int *d_csr_ptr = nullptr, *d_csr_idx = nullptr;
+  double *d_P = nullptr, *d_H = nullptr, *d_F = nullptr;
+
+  // named scratch buffers (grow-only)
+  std::map<std::string, DevBuf> scratch;
+
+  // profiling
+  bool profile = false;
+  struct Ev {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> events;
+  std::map<std::string, std::pair<double, int>> prof;
+};
+
+// ---- helpers implemented in zq_context.hip
+int zq_fail(zq_ctx *ctx, int code, const std::string &msg);
+int zq_check(zq_ctx *ctx, hipError_t e, const char *what);
+void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh = nullptr);
+void zq_free_eri(zq_ctx *ctx);
+void zq_prof_begin(zq_ctx *ctx, const char *name);
+void zq_prof_end(zq_ctx *ctx);
+
+#define ZQ_TRY(ctx, call)                                \
+  do {                                                   \
+    hipError_t _e = (call);                              \
+    if (_e != hipSuccess) return zq_check(ctx, _e, #call); \
+  } while (0)
+
+#define ZQ_LAUNCH_CHECK(ctx, name)                                                   \
+  do {                                                                               \
+    hipError_t _e = hipGetLastError();                                               \
+    if (_e != hipSuccess) return zq_fail(ctx, ZQ_ERR_LAUNCH, std::string(name) + ": " + hipGetErrorString(_e)); \
+  } while (0)

@@ -1,0 +1,282 @@
+// J/K Fock build over the resident 8-fold packed ERIs: the getFock contraction of
+// src/hartree_fock.zig:686-735 (non-DFT branch),
+//   F[k,l] = H[k,l] + sum_ij P[i,j] (g[i,k,j,l] - x g[i,j,k,l]),  F <- (F + F^T)/2,
+// with J_kl = sum_ij P_ij (ij|kl) and K_kl = sum_ij P_ij (ik|jl) in chemist notation.
+//
+// Each unique integral v = (pq|rs) of degeneracy deg (number of distinct index permutations) is read
+// once; with w = deg/8 it adds to an unsymmetrised accumulator G
+//   G[p,q] += 2w P[r,s] v,  G[r,s] += 2w P[p,q] v,
+//   G[q,s] -= x w P[p,r] v, G[p,s] -= x w P[q,r] v, G[q,r] -= x w P[p,s] v, G[p,r] -= x w P[q,s] v,
+// and F = H + G + G^T, which is exactly symmetric (so the reference's final symmetrisation is implied).
+//
+// Work decomposition: one wavefront per (bra block pair, run of <= CH ket blocks); lane = bra pair
+// (p,q) of the 8x8 bra block; the wave streams the quartets (coalesced 512-byte rows) and accumulates
+// the six updates in registers, reducing across lanes with butterfly reduce-scatters.  Its partial
+// results are written as 8x8 "records" with a fixed target block.  A second kernel sums, for each
+// output block, its records in a fixed order (a host-built CSR list): no atomics, bitwise reproducible.
+
+#include <algorithm>
+
+#include "zq_internal.h"
+
+namespace {
+
+// Reduce 8 values over the 8 lanes that differ in the bits MA|MB|MC; afterwards the lane holds the
+// total of value index (lane&MA ? 4 : 0) + (lane&MB ? 2 : 0) + (lane&MC ? 1 : 0).
+template <int MA, int MB, int MC>
+__device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
+  const bool ha = lane & MA, hb = lane & MB, hc = lane & MC;
+  double a[4], b[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double send = ha ? v[i] : v[i + 4], keep = ha ? v[i + 4] : v[i];
+    a[i] = keep + __shfl_xor(send, MA);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = hb ? a[i] : a[i + 2], keep = hb ? a[i + 2] : a[i];
+    b[i] = keep + __shfl_xor(send, MB);
+  }
+  const double send = hc ? b[0] : b[1], keep = hc ? b[1] : b[0];
+  return keep + __shfl_xor(send, MC);
+}
+
+__global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ eri, const JKItem *__restrict__ items,
+                                                     int nitems, const double *__restrict__ P, int npad, double xexch,
+                                                     double *__restrict__ rec) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int item_id = blockIdx.x * 4 + wave;
+  if (item_id >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const int Ib = __builtin_amdgcn_readfirstlane(items[item_id].Ib);
+  const int bp = __builtin_amdgcn_readfirstlane(items[item_id].bp);
+  const int bq = __builtin_amdgcn_readfirstlane(items[item_id].bq);
+  const int k0 = __builtin_amdgcn_readfirstlane(items[item_id].k0);
+  const int k1 = __builtin_amdgcn_readfirstlane(items[item_id].k1);
+  const int rec0 = __builtin_amdgcn_readfirstlane(items[item_id].rec0);
+
+  const int p = lane >> 3, q = lane & 7;
+  const int Pg = bp * 8 + p, Qg = bq * 8 + q;
+  const double fI = (bp == bq && p == q) ? 0.5 : 1.0;
+  const double Ppq = P[size_t(Pg) * npad + Qg];
+  const double *Prow = P + size_t(Pg) * npad, *Qrow = P + size_t(Qg) * npad;
+
+  // position of k0 in the (bs, br) walk
+  int bs = 0, kk = k0;
+  for (;; ++bs) {
+    const int len = zq_run_len(bp, bq, bs);
+    if (kk < len) break;
+    kk -= len;
+  }
+  int br = bs + kk;
+
+  double *out = rec + size_t(rec0) * 64;
+  const size_t blk0 = size_t(Ib) * (Ib + 1) / 2;
+  double Jbra = 0.0;
+  double Kps[8], Kqs[8], Pp_s[8], Pq_s[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    Kps[s] = Kqs[s] = 0.0;
+    Pp_s[s] = Prow[bs * 8 + s];
+    Pq_s[s] = Qrow[bs * 8 + s];
+  }
+
+  for (int k = k0; k < k1; ++k) {
+    const double *blk = eri + (blk0 + size_t(k)) * ZQ_QUARTET;
+    const int Jb = zq_bpair(br, bs);
+    const bool diagJ = (br == bs), diagQ = (Jb == Ib);
+    double Pp_r[8], Pq_r[8], Kpr[8], Kqr[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      Pp_r[r] = Prow[br * 8 + r];
+      Pq_r[r] = Qrow[br * 8 + r];
+    }
+    const double *Prs = P + size_t(br * 8) * npad + bs * 8;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      double jt[8], accpr = 0.0, accqr = 0.0;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const double v = blk[(r * 8 + s) * 64 + lane];
+        double w = fI;
+        if (diagJ && r == s) w *= 0.5;
+        if (diagQ && lane == r * 8 + s) w *= 0.5;
+        const double a = v * w;
+        Jbra += Prs[size_t(r) * npad + s] * a;
+        jt[s] = Ppq * a;
+        Kqs[s] += Pp_r[r] * a;
+        Kps[s] += Pq_r[r] * a;
+        accqr += Pp_s[s] * a;
+        accpr += Pq_s[s] * a;
+      }
+      // J of the ket pair (r, s): total over all 64 lanes, s = lane&7 after the reduce-scatter
+      double t = reduce_scatter8<4, 2, 1>(jt, lane);
+      t += __shfl_xor(t, 8);
+      t += __shfl_xor(t, 16);
+      t += __shfl_xor(t, 32);
+      if (p == 0) out[r * 8 + q] = 2.0 * t;
+      Kpr[r] = accpr;
+      Kqr[r] = accqr;
+    }
+    // K[p, br*8+r]: sum over q (lanes of one p) -> lane (p,q) holds r = q
+    out[64 + p * 8 + q] = -xexch * reduce_scatter8<4, 2, 1>(Kpr, lane);
+    // K[q, br*8+r]: sum over p -> lane (p,q) holds r = p; record row q, column r
+    out[128 + q * 8 + p] = -xexch * reduce_scatter8<32, 16, 8>(Kqr, lane);
+    out += 192;
+
+    const bool run_end = (br + 1 > (bs <= bq ? bp : bp - 1));
+    if (run_end || k == k1 - 1) {
+      out[p * 8 + q] = -xexch * reduce_scatter8<4, 2, 1>(Kps, lane);        // K[p, bs*8+s], s = q
+      out[64 + q * 8 + p] = -xexch * reduce_scatter8<32, 16, 8>(Kqs, lane);  // K[q, bs*8+s], s = p
+      out += 128;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) Kps[s] = Kqs[s] = 0.0;
+    }
+    if (run_end) {
+      ++bs;
+      while (bs <= bp && zq_run_len(bp, bq, bs) <= 0) ++bs;
+      br = bs;
+      if (k + 1 < k1) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          Pp_s[s] = Prow[bs * 8 + s];
+          Pq_s[s] = Qrow[bs * 8 + s];
+        }
+      }
+    } else {
+      ++br;
+    }
+  }
+  out[lane] = 2.0 * Jbra;  // G[bp*8+p, bq*8+q]
+}
+
+// F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).
+__global__ __launch_bounds__(64) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
+                                                   const int *__restrict__ csr_idx, const double *__restrict__ H, int n,
+                                                   double *__restrict__ F) {
+  const int o = blockIdx.x;
+  int x, y;
+  zq_unpair(o, x, y);
+  const int lane = threadIdx.x, i = lane >> 3, j = lane & 7;
+  const int gi = x * 8 + i, gj = y * 8 + j;
+  double acc = (gi < n && gj < n) ? H[size_t(gi) * n + gj] : 0.0;
+  const int e1 = csr_ptr[o + 1];
+  for (int e = csr_ptr[o]; e < e1; ++e) {
+    const int code = csr_idx[e];
+    const double *r = rec + size_t(code >> 1) * 64;
+    acc += (code & 1) ? r[j * 8 + i] : r[lane];
+  }
+  // on diagonal blocks lane (i,j) and lane (j,i) sum the same terms in different orders: let only
+  // i >= j write, so both triangles hold one value
+  if (gi < n && gj < n && (x != y || i >= j)) {
+    F[size_t(gi) * n + gj] = acc;
+    F[size_t(gj) * n + gi] = acc;
+  }
+}
+
+}  // namespace
+
+// Host-side work plan: work items, the exact record sequence each emits, and the per-output-block CSR.
+int zq_build_jk_plan(zq_ctx *ctx) {
+  const int nb = ctx->nb, Nb = ctx->Nb;
+  const size_t nq = ctx->nquartets;
+  const int CH = int(std::max<size_t>(1, std::min<size_t>(16, nq / 4096)));
+  std::vector<JKItem> items;
+  std::vector<std::pair<int, int>> tgt;
+  for (int Ib = 0; Ib < Nb; ++Ib) {
+    int bp, bq;
+    zq_unpair(Ib, bp, bq);
+    const int total = Ib + 1;
+    for (int k0 = 0; k0 < total; k0 += CH) {
+      const int k1 = std::min(total, k0 + CH);
+      items.push_back({Ib, bp, bq, k0, k1, int(tgt.size())});
+      int bs = 0, kk = k0;
+      for (;; ++bs) {
+        const int len = zq_run_len(bp, bq, bs);
+        if (kk < len) break;
+        kk -= len;
+      }
+      int br = bs + kk;
+      for (int k = k0; k < k1; ++k) {
+        tgt.push_back({br, bs});
+        tgt.push_back({bp, br});
+        tgt.push_back({bq, br});
+        const bool run_end = (br + 1 > (bs <= bq ? bp : bp - 1));
+        if (run_end || k == k1 - 1) {
+          tgt.push_back({bp, bs});
+          tgt.push_back({bq, bs});
+        }
+        if (run_end) {
+          ++bs;
+          while (bs <= bp && zq_run_len(bp, bq, bs) <= 0) ++bs;
+          br = bs;
+        } else {
+          ++br;
+        }
+      }
+      tgt.push_back({bp, bq});
+    }
+  }
+  if (tgt.size() >= (size_t(1) << 30)) return zq_fail(ctx, ZQ_ERR_INVALID, "too many J/K records");
+  // CSR over output blocks x >= y
+  std::vector<std::vector<int>> lists(Nb);
+  for (size_t r = 0; r < tgt.size(); ++r) {
+    const int tx = tgt[r].first, ty = tgt[r].second;
+    if (tx >= ty) lists[zq_bpair(tx, ty)].push_back(int(r) << 1);
+  }
+  for (size_t r = 0; r < tgt.size(); ++r) {
+    const int tx = tgt[r].first, ty = tgt[r].second;
+    if (ty >= tx) lists[zq_bpair(ty, tx)].push_back((int(r) << 1) | 1);
+  }
+  std::vector<int> ptr(Nb + 1, 0), idx;
+  for (int o = 0; o < Nb; ++o) {
+    ptr[o + 1] = ptr[o] + int(lists[o].size());
+    idx.insert(idx.end(), lists[o].begin(), lists[o].end());
+  }
+  (void)nb;
+  ctx->nitems = int(items.size());
+  ctx->nrecords = tgt.size();
+  const size_t npad2 = size_t(ctx->npad) * ctx->npad;
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_items, items.size() * sizeof(JKItem)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_records, tgt.size() * 64 * sizeof(double)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_csr_ptr, ptr.size() * sizeof(int)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_csr_idx, std::max<size_t>(1, idx.size()) * sizeof(int)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_P, npad2 * sizeof(double)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_H, size_t(ctx->n) * ctx->n * sizeof(double)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_F, size_t(ctx->n) * ctx->n * sizeof(double)));
+  ZQ_TRY(ctx, hipMemcpy(ctx->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice));
+  ZQ_TRY(ctx, hipMemcpy(ctx->d_csr_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!idx.empty()) ZQ_TRY(ctx, hipMemcpy(ctx->d_csr_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  ZQ_TRY(ctx, hipMemset(ctx->d_P, 0, npad2 * sizeof(double)));
+  return ZQ_OK;
+}
+
+// Device-resident J/K: P (npad x npad, zero padded) and H (n x n) already on the device; F (n x n) out.
+int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF) {
+  zq_prof_begin(ctx, "jk_records");
+  hipLaunchKernelGGL(zq_jk_records, dim3(unsigned((ctx->nitems + 3) / 4)), dim3(256), 0, ctx->stream, ctx->eri,
+                     ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, "zq_jk_records");
+  zq_prof_begin(ctx, "jk_reduce");
+  hipLaunchKernelGGL(zq_jk_reduce, dim3(unsigned(ctx->Nb)), dim3(64), 0, ctx->stream, ctx->d_records, ctx->d_csr_ptr,
+                     ctx->d_csr_idx, dH, ctx->n, dF);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, "zq_jk_reduce");
+  return ZQ_OK;
+}
+
+extern "C" int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double x_exch, double *F_out) {
+  if (!ctx || !H || !P || !F_out) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_fock_jk: null argument");
+  if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "zq_fock_jk: no ERIs loaded");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  const int n = ctx->n, np = ctx->npad;
+  ZQ_TRY(ctx, hipMemcpy2DAsync(ctx->d_P, np * sizeof(double), P, n * sizeof(double), n * sizeof(double), n,
+                               hipMemcpyHostToDevice, ctx->stream));
+  ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_H, H, size_t(n) * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, x_exch, ctx->d_F);
+  if (rc) return rc;
+  ZQ_TRY(ctx, hipMemcpyAsync(F_out, ctx->d_F, size_t(n) * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return ZQ_OK;
+}

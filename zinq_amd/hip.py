@@ -1,0 +1,190 @@
+"""ctypes bindings of libzinq_hip.so (include/zinq_hip.h and include/zinq_scf.h).
+
+The calls are the GPU replacements of the reference's getFock (src/hartree_fock.zig:686-735),
+ao2mo_pppp / ao2mo_oovv (src/integral_transform.zig:37-79,165-219) and the MP2 term of mp
+(src/moller_plesset.zig:340-481).  Errors raise GpuError subclasses named after the Zig errors.
+"""
+import ctypes
+
+import numpy as np
+
+from ._native import load, ptr
+
+ZQ_OK, ZQ_ERR_INVALID, ZQ_ERR_OOM, ZQ_ERR_LAUNCH, ZQ_ERR_NOT_CONVERGED, ZQ_ERR_LAPACK, ZQ_ERR_STATE = range(7)
+BLOCK = {"full": 0, "ovov": 1}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_load_packed", "zq_fock_jk", "zq_ao2mo",
+            "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
+            "zq_host_sygv"]
+
+
+class GpuError(RuntimeError):
+    code = None
+
+
+class InvalidInput(GpuError):
+    code = ZQ_ERR_INVALID
+
+
+class GpuOutOfMemory(GpuError):
+    code = ZQ_ERR_OOM
+
+
+class GpuLaunch(GpuError):
+    code = ZQ_ERR_LAUNCH
+
+
+class ScfDidNotConverge(GpuError):
+    code = ZQ_ERR_NOT_CONVERGED
+
+
+class LapackError(GpuError):
+    code = ZQ_ERR_LAPACK
+
+
+class BadState(GpuError):
+    code = ZQ_ERR_STATE
+
+
+_ERRORS = {c.code: c for c in (InvalidInput, GpuOutOfMemory, GpuLaunch, ScfDidNotConverge, LapackError, BadState)}
+
+
+class ScfOptions(ctypes.Structure):
+    _fields_ = [("diis", ctypes.c_int), ("iterations", ctypes.c_int), ("threshold", ctypes.c_double),
+                ("generalized", ctypes.c_int)]
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("iterations", ctypes.c_int)]
+
+
+def lib():
+    L = load("libzinq_hip.so")
+    if not getattr(L, "_zq_typed", False):
+        vp, sz, ci, cd = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+        L.zq_create.restype = vp
+        L.zq_create.argtypes = [ci]
+        L.zq_destroy.argtypes = [vp]
+        L.zq_last_error.restype = ctypes.c_char_p
+        L.zq_last_error.argtypes = [vp]
+        L.zq_eri_load.argtypes = [vp, vp, sz]
+        L.zq_eri_load_packed.argtypes = [vp, vp, sz]
+        L.zq_fock_jk.argtypes = [vp, vp, vp, cd, vp]
+        L.zq_ao2mo.argtypes = [vp, vp, sz, ci, ci, vp]
+        L.zq_mp2.argtypes = [vp, vp, vp, sz, ci, vp]
+        L.zq_profile_enable.argtypes = [vp, ci]
+        L.zq_profile_reset.argtypes = [vp]
+        L.zq_profile_get.argtypes = [vp, ctypes.c_char_p, vp, vp]
+        L.zq_device_bytes.restype = sz
+        L.zq_device_bytes.argtypes = [vp]
+        L.zq_scf_run.argtypes = [vp, sz, sz, vp, vp, cd, ctypes.POINTER(ScfOptions), vp, vp, vp, vp,
+                                 ctypes.POINTER(ScfResult)]
+        L.zq_host_sygv.argtypes = [sz, vp, vp, vp]
+        L._zq_typed = True
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def host_sygv(A, B):
+    """The host dsygvd stand-in used by the SCF driver (A x = w B x)."""
+    A, B = _f64(A).copy(), _f64(B).copy()
+    w = np.zeros(A.shape[0])
+    rc = lib().zq_host_sygv(A.shape[0], ptr(A), ptr(B), ptr(w))
+    if rc:
+        raise LapackError("generalized eigensolver failed")
+    return w, A
+
+
+class Context:
+    """One HIP device context (own stream) holding resident ERIs: zq_create / zq_destroy."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = self._L.zq_create(device)
+        if not h:
+            raise GpuError(f"zq_create({device}) failed: no usable HIP device")
+        self._h = ctypes.c_void_p(h)
+        self.n = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.zq_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != ZQ_OK:
+            msg = self._L.zq_last_error(self._h).decode()
+            raise _ERRORS.get(rc, GpuError)(msg)
+
+    def eri_load(self, g_dense):
+        g = _f64(g_dense)
+        n = g.shape[0]
+        self._check(self._L.zq_eri_load(self._h, ptr(g), n))
+        self.n = n
+
+    def eri_load_packed(self, packed, n):
+        pk = _f64(packed)
+        if pk.size != (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2:
+            raise InvalidInput("packed ERI array has the wrong length")
+        self._check(self._L.zq_eri_load_packed(self._h, ptr(pk), n))
+        self.n = n
+
+    def fock_jk(self, H, P, x_exch=0.5):
+        H, P = _f64(H), _f64(P)
+        F = np.zeros_like(H)
+        self._check(self._L.zq_fock_jk(self._h, ptr(H), ptr(P), float(x_exch), ptr(F)))
+        return F
+
+    def ao2mo(self, C, nocc=0, block="full", precision="fp64"):
+        C = _f64(C)
+        n = C.shape[0]
+        if block == "full":
+            out = np.zeros((n, n, n, n))
+        else:
+            out = np.zeros((nocc, nocc, n - nocc, n - nocc))
+        self._check(self._L.zq_ao2mo(self._h, ptr(C), nocc, BLOCK[block], PRECISION[precision], ptr(out)))
+        return out
+
+    def mp2(self, C, e, nocc, precision="fp64"):
+        C, e = _f64(C), _f64(e)
+        out = np.zeros(1)
+        self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
+        return float(out[0])
+
+    def scf(self, H, S, VN, nocc, diis=8, iterations=100, threshold=1e-8, generalized=False):
+        H, S = _f64(H), _f64(S)
+        n = H.shape[0]
+        C, e, P, F = np.zeros((n, n)), np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
+        opt = ScfOptions(int(diis or 0), int(iterations), float(threshold), int(bool(generalized)))
+        res = ScfResult()
+        rc = self._L.zq_scf_run(self._h, n, nocc, ptr(H), ptr(S), float(VN), ctypes.byref(opt), ptr(C), ptr(e), ptr(P),
+                                ptr(F), ctypes.byref(res))
+        self._check(rc)
+        return dict(E=res.energy, iterations=res.iterations, C=C, e=e, P=P, F=F)
+
+    def profile(self, on=True):
+        self._L.zq_profile_enable(self._h, int(on))
+
+    def profile_reset(self):
+        self._L.zq_profile_reset(self._h)
+
+    def profile_get(self, prefix):
+        ms, cnt = ctypes.c_double(), ctypes.c_int()
+        self._L.zq_profile_get(self._h, prefix.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        return ms.value, cnt.value
+
+    def device_bytes(self):
+        return int(self._L.zq_device_bytes(self._h))

@@ -1,0 +1,182 @@
+"""Benchmark of the MI355X hot path on the water-cluster workload.
+
+Workload (default): RHF + MP2 of (H2O)_k, k = 4, 6, 8, cc-pVDZ (n = 96, 144, 192), seeded geometries
+(zinq_amd.molecules), dense FP64 ERIs generated on the host by the integral engine (untimed, reported
+separately) and resident on the GPU 8-fold packed.  One step = for every molecule: the DIIS SCF to
+1e-10 (GPU J/K each iteration, FP64 host DIIS/diagonalisation) followed by the MP2 energy (GPU AO->MO
+transform + FP64 sum).  value = molecules / second over the timed steps.
+
+Per-kernel durations come from HIP events recorded on the stream the kernels run on (zq_profile_*),
+in one extra profiled step after the timed region; the roofline object is computed from them for the
+kernel with the largest total time.  cpu_baseline times the CPU oracle (test infrastructure) on a
+bounded sample and scales it to the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3}
+METRIC = "MO-transform TFLOP/s + molecules/s, mixed vs FP64 at 1µEh; p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def jk_bytes(n):
+    p = n * (n + 1) // 2
+    return 8 * (p * (p + 1) // 2) + 24 * n * n
+
+
+def ovov_flops(n, o):
+    v = n - o
+    return 2 * n * (o * n ** 3 + o * o * n * n + o * o * v * n + o * o * v * v)
+
+
+def prepare(ks, basis):
+    from zinq_amd import hip
+    from zinq_amd.molecules import water_system
+    mols = []
+    for k in ks:
+        t = time.time()
+        s = water_system(k, basis=basis)
+        pk = s.coulomb_packed()
+        t_int = time.time() - t
+        ctx = hip.Context(0)
+        ctx.eri_load_packed(pk, s.nbf)
+        mols.append(dict(k=k, n=s.nbf, o=s.nel // 2, H=s.hcore(), S=s.overlap(), VN=s.nrep(), ctx=ctx, t_int=t_int,
+                         sys=s, packed=pk))
+        log(f"(H2O)_{k}: n={s.nbf} integrals {t_int:.1f}s, device {ctx.device_bytes() / 1e9:.2f} GB")
+    return mols
+
+
+def step(mols, precision, results=None):
+    for m in mols:
+        r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=1e-10)
+        e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], precision)
+        if results is not None:
+            results.append(dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"]))
+
+
+def cpu_baseline(mols, seconds_budget=30.0):
+    """Single-core CPU oracle (oracle.c, the reference's loops restated) on (H2O)_k of the smallest k:
+    one getFock and one ao2mo_oovv + closed-shell MP2, scaled to the workload by n^4 (Fock builds x SCF
+    iterations) and o n^4 (transform)."""
+    from oracle import oracle as O
+    m = min(mols, key=lambda x: x["n"])
+    s = m["sys"]
+    g = s.coulomb()
+    P = np.zeros((m["n"], m["n"]))
+    t = time.time()
+    O.get_fock(g, m["H"], P)
+    t_fock = time.time() - t
+    C = np.linalg.qr(np.random.default_rng(0).standard_normal((m["n"], m["n"])))[0]
+    t = time.time()
+    O.mp2_closed(O.ao2mo_oovv(g, C, m["o"]), np.arange(m["n"], dtype=float))
+    t_tr = time.time() - t
+    total = 0.0
+    for x in mols:
+        sc4 = (x["n"] / m["n"]) ** 4
+        total += x.get("iters", 15) * t_fock * sc4 + t_tr * sc4 * (x["o"] / m["o"])
+    return dict(value=len(mols) / total, unit="molecules/s", cores=1, kind="port",
+                sample=f"oracle getFock ({t_fock:.2f}s) + ao2mo_oovv/MP2 ({t_tr:.2f}s) on (H2O)_{m['k']} n={m['n']}, "
+                       f"one core, scaled by n^4 x SCF iterations and o*n^4 to k={[x['k'] for x in mols]}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ks", default="4,6,8")
+    ap.add_argument("--basis", default="builtin:cc-pvdz")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
+        sys.exit(2)
+    ks = [int(k) for k in args.ks.split(",")]
+    mols = prepare(ks, args.basis)
+
+    for _ in range(args.warmup):
+        step(mols, args.precision)
+    results = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(mols, args.precision, results if i == 0 else None)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    for r in results:
+        for m in mols:
+            if m["k"] == r["k"]:
+                m["iters"] = r["iterations"]
+
+    # profiled step: kernel durations on the context streams
+    kern = {}
+    for m in mols:
+        m["ctx"].profile(True)
+        m["ctx"].profile_reset()
+    step(mols, args.precision)
+    per_mol = []
+    for m in mols:
+        c = m["ctx"]
+        d = {name: c.profile_get(name) for name in ("jk_records", "jk_reduce", "ao2mo_q1", "ao2mo_q2", "ao2mo_q3",
+                                                     "ao2mo_q4", "mp2_energy")}
+        c.profile(False)
+        t_tr = sum(d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")) * 1e-3
+        jk_ms, jk_cnt = d["jk_records"]
+        per_mol.append(dict(k=m["k"], n=m["n"], o=m["o"], scf_iterations=m.get("iters"),
+                            jk_ms_avg=jk_ms / max(jk_cnt, 1), jk_reduce_ms_avg=d["jk_reduce"][0] / max(d["jk_reduce"][1], 1),
+                            jk_GBs=jk_bytes(m["n"]) / (jk_ms / max(jk_cnt, 1) * 1e-3) / 1e9,
+                            transform_ms=t_tr * 1e3, transform_TFLOPs=ovov_flops(m["n"], m["o"]) / t_tr / 1e12,
+                            q_ms={q: d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")}))
+        for name, (ms, cnt) in d.items():
+            kern.setdefault(name, [0.0, 0, 0.0])
+            kern[name][0] += ms
+            kern[name][1] += cnt
+
+    # roofline of the dominant kernel, for the largest molecule
+    big = max(per_mol, key=lambda x: x["n"])
+    dom = max(kern.items(), key=lambda kv: kv[1][0])[0]
+    if dom.startswith("jk"):
+        achieved = big["jk_GBs"]
+        roof = dict(kernel="zq_jk_records", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    per_launch=f"8*N_u + 24 n^2 = {jk_bytes(big['n'])} bytes at n={big['n']}")
+    else:
+        q1_ms = big["q_ms"]["ao2mo_q1"]
+        n, o = big["n"], big["o"]
+        fl = 2 * o * n ** 4
+        achieved = fl / (q1_ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS[args.precision]
+        roof = dict(kernel="zq_q1", bound="mfma", achieved=round(achieved, 2), peak=peak, unit="TFLOP/s",
+                    frac=round(achieved / peak, 4), traffic=None, per_launch=f"2 o n^4 = {fl} flops at n={n}, o={o}")
+
+    value = len(mols) * args.steps / elapsed
+    out = dict(metric=METRIC, value=round(value, 3), unit="molecules/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=round(ms_per_step, 2), higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f64" if args.precision == "fp64" else args.precision, data="synthetic",
+               config=dict(workload=f"RHF+MP2 (H2O)_k k={ks} {args.basis.split(':')[-1]}, DIIS SCF to 1e-10, "
+                                    f"ERIs resident 8-fold packed", precision=args.precision),
+               transform_TFLOPs={f"k{p['k']}": round(p["transform_TFLOPs"], 2) for p in per_mol},
+               jk_GBs={f"k{p['k']}": round(p["jk_GBs"], 1) for p in per_mol},
+               roofline=roof, results=results, per_molecule=per_mol,
+               integrals_s={f"k{m['k']}": round(m["t_int"], 2) for m in mols})
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(mols)
+    print(json.dumps(out), flush=True)
+    for m in mols:
+        m["ctx"].close()
+
+
+if __name__ == "__main__":
+    main()

@@ -22,7 +22,11 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and receives
+// D[4*j + l/16][l%16] in accumulator element j (measured on gfx950; the FP32/BF16 16x16 forms use
+// row 4*(l/16) + j instead).
 __device__ inline d4 mfma_f64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline int mfma_f64_row(int lane, int j) { return 4 * j + (lane >> 4); }
 
 // Canonical slot of AO pair (l, s): Jb*64 + local, with the larger index as the "r" of the block pair.
 __device__ inline size_t slot_of(int l, int s) {
@@ -102,7 +106,7 @@ __global__ __launch_bounds__(512) void zq_q1_f64(const double *__restrict__ eri,
     const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int i = i0 + it * 16 + 4 * (lane >> 4) + j;
+      const int i = i0 + it * 16 + mfma_f64_row(lane, j);
       if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = acc[t][j];
     }
   }
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(256) void zq_gemm_f64(GemmArgs g) {
       const int n = n0 + wn + y * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = m0 + wm + x * 16 + 4 * (lane >> 4) + j;
+        const int m = m0 + wm + x * 16 + mfma_f64_row(lane, j);
         if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = acc[x][y][j];
       }
     }

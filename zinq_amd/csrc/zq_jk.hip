@@ -9,10 +9,10 @@
 //   G[q,s] -= x w P[p,r] v, G[p,s] -= x w P[q,r] v, G[q,r] -= x w P[p,s] v, G[p,r] -= x w P[q,s] v,
 // and F = H + G + G^T, which is exactly symmetric (so the reference's final symmetrisation is implied).
 //
-// Work decomposition: one wavefront per (bra block pair, run of <= CH ket blocks); lane = bra pair
-// (p,q) of the 8x8 bra block; the wave streams the quartets (coalesced 512-byte rows) and accumulates
-// the six updates in registers, reducing across lanes with butterfly reduce-scatters.  Its partial
-// results are written as 8x8 "records" with a fixed target block.  A second kernel sums, for each
+// Work decomposition: one workgroup per (bra block pair, run of <= CH ket blocks); lane = bra pair
+// (p,q) of the 8x8 bra block; the quartets stream through registers (coalesced 512-byte rows) and the six
+// updates accumulate there, reduced across lanes with butterfly shuffles.  Partial results are written as
+// 8x8 "records" with a fixed target block.  A second kernel sums, for each
 // output block, its records in a fixed order (a host-built CSR list): no atomics, bitwise reproducible.
 
 #include <algorithm>
@@ -41,27 +41,30 @@ __device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
   return keep + __shfl_xor(send, MC);
 }
 
+// One workgroup (4 waves) per work item; wave w handles ket rows r = 2w, 2w+1 of every quartet for all
+// 64 bra pairs (lane = bra pair (p,q)), loading its two 512-byte rows straight into registers.  While a
+// quartet is being contracted, the next quartet's rows and the per-lane density values it needs are
+// already in flight (they are issued before any load the current contraction waits on, so the in-order
+// vmcnt wait never stalls on them); density values common to the wave come in through scalar loads.
+// Cross-wave sums (K[p,s], K[q,s] at the end of a bs run, J[p,q] at the end of the item) go through
+// LDS in a fixed order.
 __global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ eri, const JKItem *__restrict__ items,
                                                      int nitems, const double *__restrict__ P, int npad, double xexch,
                                                      double *__restrict__ rec) {
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int item_id = blockIdx.x * 4 + wave;
+  __shared__ double red[4][2][64];
+  const int item_id = blockIdx.x;
   if (item_id >= nitems) return;
-  const int lane = threadIdx.x & 63;
-  const int Ib = __builtin_amdgcn_readfirstlane(items[item_id].Ib);
-  const int bp = __builtin_amdgcn_readfirstlane(items[item_id].bp);
-  const int bq = __builtin_amdgcn_readfirstlane(items[item_id].bq);
-  const int k0 = __builtin_amdgcn_readfirstlane(items[item_id].k0);
-  const int k1 = __builtin_amdgcn_readfirstlane(items[item_id].k1);
-  const int rec0 = __builtin_amdgcn_readfirstlane(items[item_id].rec0);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int Ib = items[item_id].Ib, bp = items[item_id].bp, bq = items[item_id].bq;
+  const int k0 = items[item_id].k0, k1 = items[item_id].k1, rec0 = items[item_id].rec0;
 
   const int p = lane >> 3, q = lane & 7;
   const int Pg = bp * 8 + p, Qg = bq * 8 + q;
   const double fI = (bp == bq && p == q) ? 0.5 : 1.0;
-  const double Ppq = P[size_t(Pg) * npad + Qg];
   const double *Prow = P + size_t(Pg) * npad, *Qrow = P + size_t(Qg) * npad;
+  const double Ppq = Prow[Qg];
 
-  // position of k0 in the (bs, br) walk
   int bs = 0, kk = k0;
   for (;; ++bs) {
     const int len = zq_run_len(bp, bq, bs);
@@ -71,106 +74,156 @@ __global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ 
   int br = bs + kk;
 
   double *out = rec + size_t(rec0) * 64;
-  const size_t blk0 = size_t(Ib) * (Ib + 1) / 2;
+  // this wave's two rows of the item's first quartet: element (r*8+s)*64 + lane, r = 2w + rr
+  const double *src = eri + (size_t(Ib) * (Ib + 1) / 2 + size_t(k0)) * ZQ_QUARTET + (2 * wave) * 512 + lane;
   double Jbra = 0.0;
-  double Kps[8], Kqs[8], Pp_s[8], Pq_s[8];
+  double Kps[8], Kqs[8], Pp_s[8], Pq_s[8], cur[16], nxt[16];
+  double Pr_cur[4], Pr_nxt[4];  // Prow[br*8+r], Qrow[br*8+r] for the wave's two rows
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     Kps[s] = Kqs[s] = 0.0;
     Pp_s[s] = Prow[bs * 8 + s];
     Pq_s[s] = Qrow[bs * 8 + s];
   }
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    Pr_cur[2 * rr] = Prow[br * 8 + 2 * wave + rr];
+    Pr_cur[2 * rr + 1] = Qrow[br * 8 + 2 * wave + rr];
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) cur[j] = src[j * 64];
 
   for (int k = k0; k < k1; ++k) {
-    const double *blk = eri + (blk0 + size_t(k)) * ZQ_QUARTET;
+    const bool run_end = (br + 1 > (bs <= bq ? bp : bp - 1));
+    int nbs = bs, nbr = br + 1;
+    if (run_end) {
+      ++nbs;
+      while (nbs <= bp && zq_run_len(bp, bq, nbs) <= 0) ++nbs;
+      nbr = nbs;
+    }
+    if (k + 1 < k1) {
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr) {
+        Pr_nxt[2 * rr] = Prow[nbr * 8 + 2 * wave + rr];
+        Pr_nxt[2 * rr + 1] = Qrow[nbr * 8 + 2 * wave + rr];
+      }
+      const double *nx = src + size_t(k + 1 - k0) * ZQ_QUARTET;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) nxt[j] = nx[j * 64];
+    }
     const int Jb = zq_bpair(br, bs);
     const bool diagJ = (br == bs), diagQ = (Jb == Ib);
-    double Pp_r[8], Pq_r[8], Kpr[8], Kqr[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      Pp_r[r] = Prow[br * 8 + r];
-      Pq_r[r] = Qrow[br * 8 + r];
-    }
-    const double *Prs = P + size_t(br * 8) * npad + bs * 8;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int rr = 0; rr < 2; ++rr) {
+      const int r = 2 * wave + rr;
+      const double Pp_r = Pr_cur[2 * rr], Pq_r = Pr_cur[2 * rr + 1];
+      const double *Prs = P + size_t(br * 8 + r) * npad + bs * 8;  // wave-uniform
       double jt[8], accpr = 0.0, accqr = 0.0;
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
-        const double v = blk[(r * 8 + s) * 64 + lane];
+        const double v = cur[rr * 8 + s];
         double w = fI;
         if (diagJ && r == s) w *= 0.5;
         if (diagQ && lane == r * 8 + s) w *= 0.5;
         const double a = v * w;
-        Jbra += Prs[size_t(r) * npad + s] * a;
+        Jbra += Prs[s] * a;
         jt[s] = Ppq * a;
-        Kqs[s] += Pp_r[r] * a;
-        Kps[s] += Pq_r[r] * a;
+        Kqs[s] += Pp_r * a;
+        Kps[s] += Pq_r * a;
         accqr += Pp_s[s] * a;
         accpr += Pq_s[s] * a;
       }
-      // J of the ket pair (r, s): total over all 64 lanes, s = lane&7 after the reduce-scatter
       double t = reduce_scatter8<4, 2, 1>(jt, lane);
       t += __shfl_xor(t, 8);
       t += __shfl_xor(t, 16);
       t += __shfl_xor(t, 32);
-      if (p == 0) out[r * 8 + q] = 2.0 * t;
-      Kpr[r] = accpr;
-      Kqr[r] = accqr;
+      if (p == 0) out[r * 8 + q] = 2.0 * t;  // J of ket pair (r, s = q)
+      accpr += __shfl_xor(accpr, 1);
+      accpr += __shfl_xor(accpr, 2);
+      accpr += __shfl_xor(accpr, 4);
+      if (q == 0) out[64 + p * 8 + r] = -xexch * accpr;  // K[p, br*8+r]
+      accqr += __shfl_xor(accqr, 8);
+      accqr += __shfl_xor(accqr, 16);
+      accqr += __shfl_xor(accqr, 32);
+      if (p == 0) out[128 + q * 8 + r] = -xexch * accqr;  // K[q, br*8+r]
     }
-    // K[p, br*8+r]: sum over q (lanes of one p) -> lane (p,q) holds r = q
-    out[64 + p * 8 + q] = -xexch * reduce_scatter8<4, 2, 1>(Kpr, lane);
-    // K[q, br*8+r]: sum over p -> lane (p,q) holds r = p; record row q, column r
-    out[128 + q * 8 + p] = -xexch * reduce_scatter8<32, 16, 8>(Kqr, lane);
     out += 192;
 
-    const bool run_end = (br + 1 > (bs <= bq ? bp : bp - 1));
     if (run_end || k == k1 - 1) {
-      out[p * 8 + q] = -xexch * reduce_scatter8<4, 2, 1>(Kps, lane);        // K[p, bs*8+s], s = q
-      out[64 + q * 8 + p] = -xexch * reduce_scatter8<32, 16, 8>(Kqs, lane);  // K[q, bs*8+s], s = p
+      red[wave][0][lane] = reduce_scatter8<4, 2, 1>(Kps, lane);    // K[p, bs*8+s], s = q
+      red[wave][1][lane] = reduce_scatter8<32, 16, 8>(Kqs, lane);  // K[q, bs*8+s], s = p
+      __syncthreads();
+      if (wave == 0) {
+        const double a = ((red[0][0][lane] + red[1][0][lane]) + red[2][0][lane]) + red[3][0][lane];
+        const double b = ((red[0][1][lane] + red[1][1][lane]) + red[2][1][lane]) + red[3][1][lane];
+        out[p * 8 + q] = -xexch * a;
+        out[64 + q * 8 + p] = -xexch * b;
+      }
+      __syncthreads();
       out += 128;
 #pragma unroll
       for (int s = 0; s < 8; ++s) Kps[s] = Kqs[s] = 0.0;
-    }
-    if (run_end) {
-      ++bs;
-      while (bs <= bp && zq_run_len(bp, bq, bs) <= 0) ++bs;
-      br = bs;
-      if (k + 1 < k1) {
+      if (run_end && k + 1 < k1) {
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-          Pp_s[s] = Prow[bs * 8 + s];
-          Pq_s[s] = Qrow[bs * 8 + s];
+          Pp_s[s] = Prow[nbs * 8 + s];
+          Pq_s[s] = Qrow[nbs * 8 + s];
         }
       }
-    } else {
-      ++br;
     }
+    bs = nbs, br = nbr;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) cur[j] = nxt[j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Pr_cur[j] = Pr_nxt[j];
   }
-  out[lane] = 2.0 * Jbra;  // G[bp*8+p, bq*8+q]
+  red[wave][0][lane] = Jbra;
+  __syncthreads();
+  if (wave == 0) out[lane] = 2.0 * (((red[0][0][lane] + red[1][0][lane]) + red[2][0][lane]) + red[3][0][lane]);
 }
 
-// F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).
-__global__ __launch_bounds__(64) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
-                                                   const int *__restrict__ csr_idx, const double *__restrict__ H, int n,
-                                                   double *__restrict__ F) {
+// F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).  Four waves
+// take contiguous quarters of the block's record list with 8 loads in flight; the quarters are added in
+// a fixed order, so the result is bitwise reproducible.
+__global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
+                                                    const int *__restrict__ csr_idx, const double *__restrict__ H, int n,
+                                                    double *__restrict__ F) {
+  __shared__ double part[4][64];
   const int o = blockIdx.x;
   int x, y;
   zq_unpair(o, x, y);
-  const int lane = threadIdx.x, i = lane >> 3, j = lane & 7;
-  const int gi = x * 8 + i, gj = y * 8 + j;
-  double acc = (gi < n && gj < n) ? H[size_t(gi) * n + gj] : 0.0;
-  const int e1 = csr_ptr[o + 1];
-  for (int e = csr_ptr[o]; e < e1; ++e) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i = lane >> 3, j = lane & 7;
+  const int e0 = csr_ptr[o], e1 = csr_ptr[o + 1], len = e1 - e0;
+  const int a0 = e0 + (len * wave) / 4, a1 = e0 + (len * (wave + 1)) / 4;
+  double acc = 0.0;
+  int e = a0;
+  for (; e + 8 <= a1; e += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int code = csr_idx[e + u];
+      const double *r = rec + size_t(code >> 1) * 64;
+      v[u] = (code & 1) ? r[j * 8 + i] : r[lane];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; e < a1; ++e) {
     const int code = csr_idx[e];
     const double *r = rec + size_t(code >> 1) * 64;
     acc += (code & 1) ? r[j * 8 + i] : r[lane];
   }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave != 0) return;
+  const int gi = x * 8 + i, gj = y * 8 + j;
+  const double h = (gi < n && gj < n) ? H[size_t(gi) * n + gj] : 0.0;
+  const double f = h + (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
   // on diagonal blocks lane (i,j) and lane (j,i) sum the same terms in different orders: let only
   // i >= j write, so both triangles hold one value
   if (gi < n && gj < n && (x != y || i >= j)) {
-    F[size_t(gi) * n + gj] = acc;
-    F[size_t(gj) * n + gi] = acc;
+    F[size_t(gi) * n + gj] = f;
+    F[size_t(gj) * n + gi] = f;
   }
 }
 
@@ -180,7 +233,7 @@ __global__ __launch_bounds__(64) void zq_jk_reduce(const double *__restrict__ re
 int zq_build_jk_plan(zq_ctx *ctx) {
   const int nb = ctx->nb, Nb = ctx->Nb;
   const size_t nq = ctx->nquartets;
-  const int CH = int(std::max<size_t>(1, std::min<size_t>(16, nq / 4096)));
+  const int CH = int(std::max<size_t>(1, std::min<size_t>(16, nq / 3072)));
   std::vector<JKItem> items;
   std::vector<std::pair<int, int>> tgt;
   for (int Ib = 0; Ib < Nb; ++Ib) {
@@ -254,12 +307,12 @@ int zq_build_jk_plan(zq_ctx *ctx) {
 // Device-resident J/K: P (npad x npad, zero padded) and H (n x n) already on the device; F (n x n) out.
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF) {
   zq_prof_begin(ctx, "jk_records");
-  hipLaunchKernelGGL(zq_jk_records, dim3(unsigned((ctx->nitems + 3) / 4)), dim3(256), 0, ctx->stream, ctx->eri,
+  hipLaunchKernelGGL(zq_jk_records, dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream, ctx->eri,
                      ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_records");
   zq_prof_begin(ctx, "jk_reduce");
-  hipLaunchKernelGGL(zq_jk_reduce, dim3(unsigned(ctx->Nb)), dim3(64), 0, ctx->stream, ctx->d_records, ctx->d_csr_ptr,
+  hipLaunchKernelGGL(zq_jk_reduce, dim3(unsigned(ctx->Nb)), dim3(256), 0, ctx->stream, ctx->d_records, ctx->d_csr_ptr,
                      ctx->d_csr_idx, dH, ctx->n, dF);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_reduce");

@@ -126,3 +126,35 @@ def test_invalid_inputs_raise(ctx):
     with pytest.raises(hip.BadState):
         fresh.fock_jk(H, H)
     fresh.close()
+
+
+# ---------------------------------------------------------------- reduced-precision quarter transforms
+# Bars (absolute, on (ia|jb) and on E_MP2).  AO integrals reach ~5 Eh (oxygen core), so each quarter
+# carries absolute rounding of ~5 x unit roundoff x sqrt(terms): FP32 (2^-24) 5e-5 / 1e-6 Eh; BF16X3
+# (hi*hi + hi*lo + lo*hi of a two-term bf16 split, ~2^-16 relative per product) 5e-3 / 5e-5 Eh.
+PREC_TOL = {"fp32": (5e-5, 1e-6), "bf16x3": (5e-3, 5e-5)}
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("kind", ["sto3g", "ccpvdz", "w2"])
+def test_reduced_precision_ovov_and_mp2(ctx, kind, prec):
+    s, H, S, g, VN = _load(ctx, kind)
+    o = s.nel // 2
+    r = O.scf(H, S, g, VN, o, threshold=1e-10)
+    To = O.ao2mo_oovv(g, r["C"], o)
+    T = ctx.ao2mo(r["C"], o, "ovov", prec)
+    e2 = ctx.mp2(r["C"], r["e"], o, prec)
+    e2o = O.mp2_closed(To, r["e"])
+    tol_t, tol_e = PREC_TOL[prec]
+    print(f"{kind} {prec}: max|d(ia|jb)|={np.abs(T - To).max():.3e} |dE_MP2|={abs(e2 - e2o):.3e}")
+    assert np.abs(T - To).max() < tol_t
+    assert abs(e2 - e2o) < tol_e
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_reduced_precision_full_transform(ctx, prec):
+    s, H, S, g, VN = _load(ctx, "ccpvdz")
+    C = np.linalg.qr(np.random.default_rng(9).standard_normal((s.nbf, s.nbf)))[0]
+    err = np.abs(ctx.ao2mo(C, 0, "full", prec) - O.ao2mo_pppp(g, C)).max()
+    print(f"full {prec}: {err:.3e}")
+    assert err < PREC_TOL[prec][0] * 5

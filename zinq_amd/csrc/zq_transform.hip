@@ -3,30 +3,62 @@
 // Reference: ao2mo_pppp (src/integral_transform.zig:37-79) and its occupied/virtual restriction
 // ao2mo_oovv (:165-219) compute, as four quarter transforms in physicist order,
 //   out[i,j,a,b] = sum C[mu,i] C[lam,j] C[nu,a] C[sig,b] g[mu,lam,nu,sig],  g[mu,lam,nu,sig] = (mu nu|lam sig).
-// Here the same chain runs as four GEMM stages on MFMA (FP64: v_mfma_f64_16x16x4f64):
+// Here the same chain runs as four GEMM stages on the matrix cores:
 //   Q1  X[K][i][nu]      = sum_mu  C1[mu,i]  (mu nu|K)         for every canonical ket pair K = (lam >= sig)
 //   Q2  Y[i][j][sig][nu] = sum_lam C2[lam,j] X[(lam sig)][i][nu]   (X is symmetric in lam, sig)
 //   Q3  W[i][j][sig][a]  = sum_nu  Y[i][j][sig][nu] C3[nu,a]
 //   Q4  out[i][j][a][b]  = sum_sig W[i][j][sig][a] C4[sig,b]
 // Q1 reads the 8x8 block quartets straight from HBM into LDS (no unpacked copy of the ERIs) and uses
 // the ket-pair symmetry, so its executed flops are half of the dense chain's first quarter.
+//
+// Precisions of the quarter transforms:
+//   FP64    v_mfma_f64_16x16x4f64, FP64 intermediates;
+//   FP32    v_mfma_f32_16x16x4f32, FP32 intermediates;
+//   BF16X3  every operand split x = hi + lo (two bf16), three v_mfma_f32_16x16x32_bf16 products
+//           hi*hi + (hi*lo + lo*hi) with the small products in a separate FP32 accumulator that is added
+//           at the end; FP32 intermediates.
+// The MP2 denominators and sums are FP64 in every mode.
 
 #include <algorithm>
 #include <cmath>
 
 #include "zq_internal.h"
 
-int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF);
-
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16;
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 // v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and B[l/16][l%16] and receives
-// D[4*j + l/16][l%16] in accumulator element j (measured on gfx950; the FP32/BF16 16x16 forms use
-// row 4*(l/16) + j instead).
-__device__ inline d4 mfma_f64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-__device__ inline int mfma_f64_row(int lane, int j) { return 4 * j + (lane >> 4); }
+// D[4*j + l/16][l%16] in accumulator element j (measured on gfx950).
+struct MF64 {
+  typedef double T;
+  typedef d4 acc;
+  static __device__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  static __device__ acc mma(T a, T b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ int row(int lane, int j) { return 4 * j + (lane >> 4); }
+};
+
+// v_mfma_f32_16x16x4f32: same operand layout, D[4*(l/16) + j][l%16].
+struct MF32 {
+  typedef float T;
+  typedef f4 acc;
+  static __device__ acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ acc mma(T a, T b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ int row(int lane, int j) { return 4 * (lane >> 4) + j; }
+};
+
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][8*(l/16)+0..7] and B[8*(l/16)+0..7][l%16];
+// D[4*(l/16) + j][l%16].
+__device__ inline f4 mma_bf16(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ inline int bf16_row(int lane, int j) { return 4 * (lane >> 4) + j; }
+
+__device__ inline void split_bf16(float x, bf16 &hi, bf16 &lo) {
+  hi = (bf16)x;
+  lo = (bf16)(x - (float)hi);
+}
 
 // Canonical slot of AO pair (l, s): Jb*64 + local, with the larger index as the "r" of the block pair.
 __device__ inline size_t slot_of(int l, int s) {
@@ -34,55 +66,72 @@ __device__ inline size_t slot_of(int l, int s) {
   return size_t(zq_bpair(hi >> 3, lo >> 3)) * 64 + (hi & 7) * 8 + (lo & 7);
 }
 
-// ---------------------------------------------------------------- Q1
+// Where the 64 x 64 tile (mu in block bmu, nu in block bnu | ket pairs of block pair Jb) is stored.
+struct Q1Src {
+  size_t base;
+  int Ib, Jb;
+  bool swp, diagB, diagK;
+};
+
+__device__ inline Q1Src q1_src(int bmu, int bnu, int br, int bs, int Jb) {
+  const int bmax = bmu > bnu ? bmu : bnu, bmin = bmu > bnu ? bnu : bmu;
+  Q1Src s;
+  s.Ib = zq_bpair(bmax, bmin);
+  s.Jb = Jb;
+  s.swp = bmu < bnu;
+  s.diagB = bmax == bmin;
+  s.diagK = br == bs;
+  s.base = (s.Ib >= Jb ? zq_blk_index(bmax, bmin, br, bs) : zq_blk_index(br, bs, bmax, bmin)) * size_t(ZQ_QUARTET);
+  return s;
+}
+
+// (mu = bmu*8+u, nu = bnu*8+w | K): 0 for non-canonical ket slots of a diagonal ket block.
+__device__ inline double q1_val(const double *__restrict__ eri, const Q1Src &s, int K, int u, int w) {
+  if (s.diagK && (K >> 3) < (K & 7)) return 0.0;
+  int pl = s.swp ? w : u, ql = s.swp ? u : w;
+  if (s.diagB && pl < ql) {
+    const int t = pl;
+    pl = ql, ql = t;
+  }
+  const int bl = pl * 8 + ql;
+  int idx;
+  if (s.Ib > s.Jb)
+    idx = K * 64 + bl;
+  else if (s.Ib < s.Jb)
+    idx = bl * 64 + K;
+  else
+    idx = (K < bl ? K : bl) * 64 + (K < bl ? bl : K);
+  return eri[s.base + idx];
+}
+
+// ---------------------------------------------------------------- Q1, FP64 / FP32
 // Workgroup (512 threads) owns ket block pair Jb (64 slots) x AO block bnu (8 nu) x a group of 64 i.
-// Loop over mu blocks: stage the 64 x 64 tile T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS from
-// the quartet that stores it, then X[K][i][nu] += C1[mu,i] T via MFMA.  Output tiles: MT (<= 4) tiles of
-// 16 i by 32 tiles of 16 (K,nu) columns, split evenly over the 8 waves.
-template <int MT>
-__global__ __launch_bounds__(512) void zq_q1_f64(const double *__restrict__ eri, const double *__restrict__ C1, int ldc1,
-                                                 int m1, int nb, int npad, double *__restrict__ X) {
+// Loop over mu blocks: stage T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then
+// X[K][i][nu] += C1[mu,i] T via MFMA.  Output tiles: MT (<= 4) tiles of 16 i by 32 tiles of 16 (K,nu)
+// columns, split evenly over the 8 waves.
+template <class MF, int MT>
+__global__ __launch_bounds__(512) void zq_q1(const double *__restrict__ eri, const typename MF::T *__restrict__ C1,
+                                             int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
+  typedef typename MF::T T;
   constexpr int TPW = MT * 4;  // tiles per wave
-  __shared__ double T[64][65];
+  __shared__ T Ts[64][65];
   const int Jb = blockIdx.x / nb, bnu = blockIdx.x % nb;
   const int i0 = blockIdx.y * 64;
   int br, bs;
   zq_unpair(Jb, br, bs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  d4 acc[TPW];
+  typename MF::acc acc[TPW];
 #pragma unroll
-  for (int t = 0; t < TPW; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < TPW; ++t) acc[t] = MF::zero();
 
   for (int bmu = 0; bmu < nb; ++bmu) {
-    const int bmax = bmu > bnu ? bmu : bnu, bmin = bmu > bnu ? bnu : bmu;
-    const bool swp = bmu < bnu;
-    const int Ib = zq_bpair(bmax, bmin);
-    const size_t base =
-        (Ib >= Jb ? zq_blk_index(bmax, bmin, br, bs) : zq_blk_index(br, bs, bmax, bmin)) * size_t(ZQ_QUARTET);
+    const Q1Src src = q1_src(bmu, bnu, br, bs, Jb);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int e = tid + 512 * j;
-      const int K = e >> 6, uw = e & 63, u = uw >> 3, w = uw & 7;
-      const int r = K >> 3, s = K & 7;
-      double v = 0.0;
-      if (!(br == bs && r < s)) {
-        int pl = swp ? w : u, ql = swp ? u : w;
-        if (bmax == bmin && pl < ql) {
-          const int t = pl;
-          pl = ql, ql = t;
-        }
-        const int bl = pl * 8 + ql;
-        int idx;
-        if (Ib > Jb)
-          idx = K * 64 + bl;
-        else if (Ib < Jb)
-          idx = bl * 64 + K;
-        else
-          idx = (K < bl ? K : bl) * 64 + (K < bl ? bl : K);
-        v = eri[base + idx];
-      }
-      T[K][uw] = v;
+      const int K = e >> 6, uw = e & 63;
+      Ts[K][uw] = T(q1_val(eri, src, K, uw >> 3, uw & 7));
     }
     __syncthreads();
 #pragma unroll
@@ -93,10 +142,9 @@ __global__ __launch_bounds__(512) void zq_q1_f64(const double *__restrict__ eri,
       for (int t = 0; t < TPW; ++t) {
         const int tile = wave * TPW + t, it = tile >> 5, nt = tile & 31;
         const int i = i0 + it * 16 + (lane & 15);
-        const double a = (i < m1) ? C1[size_t(mu) * ldc1 + i] : 0.0;
+        const T a = (i < m1) ? C1[size_t(mu) * ldc1 + i] : T(0);
         const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
-        const double b = T[K][uu * 8 + wl];
-        acc[t] = mfma_f64(a, b, acc[t]);
+        acc[t] = MF::mma(a, Ts[K][uu * 8 + wl], acc[t]);
       }
     }
   }
@@ -106,42 +154,134 @@ __global__ __launch_bounds__(512) void zq_q1_f64(const double *__restrict__ eri,
     const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int i = i0 + it * 16 + mfma_f64_row(lane, j);
+      const int i = i0 + it * 16 + MF::row(lane, j);
       if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = acc[t][j];
     }
   }
 }
 
-// ---------------------------------------------------------------- generic batched GEMM (FP64 MFMA)
+// ---------------------------------------------------------------- Q1, BF16X3
+// Same tiling, four mu blocks (32 mu, one bf16 MFMA k-step) staged per step as hi/lo halves
+// Th/Tl[K][w][mu_local].  C1 comes pre-split and transposed: C1h/C1l[i][mu] (ldt >= 8*ceil(nb/4)*4).
+template <int MT>
+__global__ __launch_bounds__(512) void zq_q1_bf16x3(const double *__restrict__ eri, const bf16 *__restrict__ C1h,
+                                                    const bf16 *__restrict__ C1l, int ldt, int m1, int nb, int npad,
+                                                    float *__restrict__ X) {
+  constexpr int TPW = MT * 4;
+  __shared__ __attribute__((aligned(16))) bf16 Th[64][8][40];
+  __shared__ __attribute__((aligned(16))) bf16 Tl[64][8][40];
+  const int Jb = blockIdx.x / nb, bnu = blockIdx.x % nb;
+  const int i0 = blockIdx.y * 64;
+  int br, bs;
+  zq_unpair(Jb, br, bs);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  f4 big[TPW], small[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) big[t] = small[t] = f4{0.f, 0.f, 0.f, 0.f};
+
+  for (int bm0 = 0; bm0 < nb; bm0 += 4) {
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+      const int bmu = bm0 + bi;
+      if (bmu < nb) {
+        const Q1Src src = q1_src(bmu, bnu, br, bs, Jb);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int e = tid + 512 * j;
+          const int K = e >> 6, uw = e & 63, u = uw >> 3, w = uw & 7;
+          bf16 h, l;
+          split_bf16(float(q1_val(eri, src, K, u, w)), h, l);
+          Th[K][w][bi * 8 + u] = h;
+          Tl[K][w][bi * 8 + u] = l;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int e = tid + 512 * j;
+          const int K = e >> 6, uw = e & 63;
+          Th[K][uw & 7][bi * 8 + (uw >> 3)] = (bf16)0.f;
+          Tl[K][uw & 7][bi * 8 + (uw >> 3)] = (bf16)0.f;
+        }
+      }
+    }
+    __syncthreads();
+    const int kg = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int tile = wave * TPW + t, it = tile >> 5, nt = tile & 31;
+      const int i = i0 + it * 16 + (lane & 15);
+      bf8 ah, al;
+      if (i < m1) {
+        ah = *(const bf8 *)(C1h + size_t(i) * ldt + bm0 * 8 + kg * 8);
+        al = *(const bf8 *)(C1l + size_t(i) * ldt + bm0 * 8 + kg * 8);
+      } else {
+#pragma unroll
+        for (int x = 0; x < 8; ++x) ah[x] = al[x] = (bf16)0.f;
+      }
+      const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
+      const bf8 bh = *(const bf8 *)&Th[K][wl][kg * 8];
+      const bf8 bl = *(const bf8 *)&Tl[K][wl][kg * 8];
+      big[t] = mma_bf16(ah, bh, big[t]);
+      small[t] = mma_bf16(ah, bl, small[t]);
+      small[t] = mma_bf16(al, bh, small[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int tile = wave * TPW + t, it = tile >> 5, nt = tile & 31;
+    const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = i0 + it * 16 + bf16_row(lane, j);
+      if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = big[t][j] + small[t][j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- batched GEMM
 // C[b][m][n] = sum_k A[b][m][k] B[b][k][n] with arbitrary element strides.  Mode 1 gathers B from X:
-// B[lam][col = sig*npad + nu] = X[slot(lam, sig)][b][nu] with X strides (slot: sXs, b: sXb).
+// B[lam][col = sig*npad + nu] = X[slot(lam, sig)][b][nu] (slot stride sXs; B already offset by b).
+template <class T>
 struct GemmArgs {
-  const double *A;
+  const T *A;
   long long sAb, sAm, sAk;
-  const double *B;
+  const T *B;
   long long sBb, sBk, sBn;
-  double *C;
+  T *C;
   long long sCb, sCm, sCn;
   int M, N, K;
   int mode;
   int npad;
-  long long sXs, sXb;
+  long long sXs;
 };
 
-__global__ __launch_bounds__(256) void zq_gemm_f64(GemmArgs g) {
-  __shared__ double As[16][65];
-  __shared__ double Bs[16][65];
+template <class T>
+__device__ inline T gemm_b(const GemmArgs<T> &g, const T *B, int k, int n) {
+  if (g.mode == 1) {
+    const int sig = n / g.npad, nu = n - sig * g.npad;
+    return B[(long long)slot_of(k, sig) * g.sXs + nu];
+  }
+  return B[k * g.sBk + n * g.sBn];
+}
+
+// FP64 / FP32: 64 x 64 tiles, BK = 16, 4 waves of 32 x 32 (2 x 2 MFMA tiles).
+template <class MF>
+__global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
+  typedef typename MF::T T;
+  __shared__ T As[16][65];
+  __shared__ T Bs[16][65];
   const int b = blockIdx.z;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const double *A = g.A + b * g.sAb;
-  const double *B = g.B + b * g.sBb;
-  d4 acc[2][2];
+  const T *A = g.A + b * g.sAb;
+  const T *B = g.B + b * g.sBb;
+  typename MF::acc acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) acc[x][y] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int y = 0; y < 2; ++y) acc[x][y] = MF::zero();
   const bool a_kfast = (g.sAk == 1);
   const bool b_nfast = (g.mode == 1) || (g.sBn == 1);
   for (int k0 = 0; k0 < g.K; k0 += 16) {
@@ -155,7 +295,7 @@ __global__ __launch_bounds__(256) void zq_gemm_f64(GemmArgs g) {
       else
         kk = e >> 6, mm = e & 63;
       const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : 0.0;
+      As[kk][mm] = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : T(0);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -166,22 +306,13 @@ __global__ __launch_bounds__(256) void zq_gemm_f64(GemmArgs g) {
       else
         nn = e >> 4, kk = e & 15;
       const int n = n0 + nn, k = k0 + kk;
-      double v = 0.0;
-      if (n < g.N && k < g.K) {
-        if (g.mode == 1) {
-          const int sig = n / g.npad, nu = n - sig * g.npad;
-          v = B[(long long)slot_of(k, sig) * g.sXs + nu];
-        } else {
-          v = B[k * g.sBk + n * g.sBn];
-        }
-      }
-      Bs[kk][nn] = v;
+      Bs[kk][nn] = (n < g.N && k < g.K) ? gemm_b(g, B, k, n) : T(0);
     }
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
-      double a[2], bb[2];
+      T a[2], bb[2];
 #pragma unroll
       for (int x = 0; x < 2; ++x) a[x] = As[kk][wm + x * 16 + (lane & 15)];
 #pragma unroll
@@ -189,10 +320,10 @@ __global__ __launch_bounds__(256) void zq_gemm_f64(GemmArgs g) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = mfma_f64(a[x], bb[y], acc[x][y]);
+        for (int y = 0; y < 2; ++y) acc[x][y] = MF::mma(a[x], bb[y], acc[x][y]);
     }
   }
-  double *C = g.C + b * g.sCb;
+  T *C = g.C + b * g.sCb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -200,24 +331,103 @@ __global__ __launch_bounds__(256) void zq_gemm_f64(GemmArgs g) {
       const int n = n0 + wn + y * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = m0 + wm + x * 16 + mfma_f64_row(lane, j);
+        const int m = m0 + wm + x * 16 + MF::row(lane, j);
         if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = acc[x][y][j];
       }
     }
 }
 
-// ---------------------------------------------------------------- MP2 energy
+// BF16X3: FP32 operands split into bf16 hi/lo while staged; 64 x 64 tiles, BK = 32 (one k-step).
+__global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
+  __shared__ __attribute__((aligned(16))) bf16 Ah[64][40];
+  __shared__ __attribute__((aligned(16))) bf16 Al[64][40];
+  __shared__ __attribute__((aligned(16))) bf16 Bh[64][40];
+  __shared__ __attribute__((aligned(16))) bf16 Bl[64][40];
+  const int b = blockIdx.z;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const float *A = g.A + b * g.sAb;
+  const float *B = g.B + b * g.sBb;
+  f4 big[2][2], small[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) big[x][y] = small[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+  const bool a_kfast = (g.sAk == 1);
+  const bool b_nfast = (g.mode == 1) || (g.sBn == 1);
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 256 * j;
+      int mm, kk;
+      if (a_kfast)
+        mm = e >> 5, kk = e & 31;
+      else
+        kk = e >> 6, mm = e & 63;
+      const int m = m0 + mm, k = k0 + kk;
+      const float v = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : 0.f;
+      split_bf16(v, Ah[mm][kk], Al[mm][kk]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 256 * j;
+      int nn, kk;
+      if (b_nfast)
+        kk = e >> 6, nn = e & 63;
+      else
+        nn = e >> 5, kk = e & 31;
+      const int n = n0 + nn, k = k0 + kk;
+      const float v = (n < g.N && k < g.K) ? gemm_b(g, B, k, n) : 0.f;
+      split_bf16(v, Bh[nn][kk], Bl[nn][kk]);
+    }
+    __syncthreads();
+    const int kg = (lane >> 4) * 8;
+    bf8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      ah[x] = *(const bf8 *)&Ah[wm + x * 16 + (lane & 15)][kg];
+      al[x] = *(const bf8 *)&Al[wm + x * 16 + (lane & 15)][kg];
+      bh[x] = *(const bf8 *)&Bh[wn + x * 16 + (lane & 15)][kg];
+      bl[x] = *(const bf8 *)&Bl[wn + x * 16 + (lane & 15)][kg];
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        big[x][y] = mma_bf16(ah[x], bh[y], big[x][y]);
+        small[x][y] = mma_bf16(ah[x], bl[y], small[x][y]);
+        small[x][y] = mma_bf16(al[x], bh[y], small[x][y]);
+      }
+  }
+  float *C = g.C + b * g.sCb;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int n = n0 + wn + y * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int m = m0 + wm + x * 16 + bf16_row(lane, j);
+        if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = big[x][y][j] + small[x][y][j];
+      }
+    }
+}
+
+// ---------------------------------------------------------------- MP2 energy (FP64 arithmetic)
 // One workgroup per (i,j): sum_ab T[i,j,a,b] (2 T[i,j,a,b] - T[i,j,b,a]) / (e_i + e_j - e_a - e_b)
 // in a fixed order; partial[i*o + j].
-__global__ __launch_bounds__(256) void zq_mp2_pairs(const double *__restrict__ T, const double *__restrict__ e, int o,
-                                                    int v, double *__restrict__ partial) {
+template <class T>
+__global__ __launch_bounds__(256) void zq_mp2_pairs(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
+                                                    double *__restrict__ partial) {
   const int ij = blockIdx.x, i = ij / o, j = ij % o;
-  const double *t = T + size_t(ij) * v * v;
+  const T *t = t4 + size_t(ij) * v * v;
   const double eij = e[i] + e[j];
   double s = 0.0;
   for (int ab = threadIdx.x; ab < v * v; ab += 256) {
     const int a = ab / v, b = ab % v;
-    const double x = t[ab], y = t[b * v + a];
+    const double x = double(t[ab]), y = double(t[b * v + a]);
     s += x * (2.0 * x - y) / (eij - e[o + a] - e[o + b]);
   }
   __shared__ double red[256];
@@ -238,49 +448,80 @@ __global__ void zq_sum_serial(const double *__restrict__ x, int n, double *__res
   }
 }
 
-int launch_gemm(zq_ctx *ctx, const GemmArgs &g, int batch, const char *name) {
+__global__ void zq_widen(const float *__restrict__ x, size_t n, double *__restrict__ y) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) y[i] = x[i];
+}
+
+template <class MF>
+int launch_gemm(zq_ctx *ctx, const GemmArgs<typename MF::T> &g, int batch, const char *name) {
   dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(batch));
   zq_prof_begin(ctx, name);
-  hipLaunchKernelGGL(zq_gemm_f64, grid, dim3(256), 0, ctx->stream, g);
+  hipLaunchKernelGGL(zq_gemm<MF>, grid, dim3(256), 0, ctx->stream, g);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, name);
   return ZQ_OK;
 }
 
-}  // namespace
+int launch_gemm_bf16x3(zq_ctx *ctx, const GemmArgs<float> &g, int batch, const char *name) {
+  dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(batch));
+  zq_prof_begin(ctx, name);
+  hipLaunchKernelGGL(zq_gemm_bf16x3, grid, dim3(256), 0, ctx->stream, g);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, name);
+  return ZQ_OK;
+}
 
-// Runs Q1..Q4 into device memory; *d_out receives out[i][j][a][b] with dims m1 x m2 x m3 x m4.
-int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, double **d_out, int dims[4]) {
-  if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "no ERIs loaded");
-  const int n = ctx->n, np = ctx->npad, nb = ctx->nb, Nb = ctx->Nb;
-  if (block != ZQ_BLOCK_FULL && block != ZQ_BLOCK_OVOV) return zq_fail(ctx, ZQ_ERR_INVALID, "bad block");
-  if (precision != ZQ_PREC_FP64) return zq_fail(ctx, ZQ_ERR_INVALID, "precision not available");
-  if (block == ZQ_BLOCK_OVOV && (nocc <= 0 || nocc >= n)) return zq_fail(ctx, ZQ_ERR_INVALID, "bad nocc");
-  const int m1 = block == ZQ_BLOCK_OVOV ? nocc : n, m2 = m1;
-  const int off3 = block == ZQ_BLOCK_OVOV ? nocc : 0, m3 = n - off3, m4 = m3;
-  dims[0] = m1, dims[1] = m2, dims[2] = m3, dims[3] = m4;
+template <class T>
+int upload(zq_ctx *ctx, const char *name, const std::vector<T> &h, T **d) {
+  *d = (T *)zq_scratch(ctx, name, h.size() * sizeof(T));
+  if (!*d) return zq_fail(ctx, ZQ_ERR_OOM, std::string("transform scratch ") + name);
+  ZQ_TRY(ctx, hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return ZQ_OK;
+}
 
-  // coefficient slices, zero rows for padded AOs: C1 (np x m1), C2T (m2 x np), C3 (np x m3), C4 = C3
-  std::vector<double> h1(size_t(np) * m1, 0.0), h2t(size_t(m2) * np, 0.0), h3(size_t(np) * m3, 0.0);
+struct Dims {
+  int n, np, nb, Nb, m1, m2, m3, m4, off3;
+};
+
+// Q1..Q4 with element type T (double: FP64, float: FP32 or BF16X3).
+template <class T, class MF>
+int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_out) {
+  const int n = d.n, np = d.np, nb = d.nb, Nb = d.Nb, m1 = d.m1, m2 = d.m2, m3 = d.m3, m4 = d.m4;
+  // coefficient slices, zero rows for padded AOs: C1 (np x m1), C2T (m2 x np), C3 = C4 (np x m3)
+  std::vector<T> h1(size_t(np) * m1, T(0)), h2t(size_t(m2) * np, T(0)), h3(size_t(np) * m3, T(0));
   for (int mu = 0; mu < n; ++mu) {
-    for (int i = 0; i < m1; ++i) h1[size_t(mu) * m1 + i] = C[size_t(mu) * n + i];
-    for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = C[size_t(mu) * n + j];
-    for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = C[size_t(mu) * n + off3 + a];
+    for (int i = 0; i < m1; ++i) h1[size_t(mu) * m1 + i] = T(C[size_t(mu) * n + i]);
+    for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = T(C[size_t(mu) * n + j]);
+    for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = T(C[size_t(mu) * n + d.off3 + a]);
   }
-  double *d1 = (double *)zq_scratch(ctx, "C1", h1.size() * 8);
-  double *d2t = (double *)zq_scratch(ctx, "C2T", h2t.size() * 8);
-  double *d3 = (double *)zq_scratch(ctx, "C3", h3.size() * 8);
+  T *d1, *d2t, *d3;
+  if (int rc = upload(ctx, "C1", h1, &d1)) return rc;
+  if (int rc = upload(ctx, "C2T", h2t, &d2t)) return rc;
+  if (int rc = upload(ctx, "C3", h3, &d3)) return rc;
+  // BF16X3 Q1 takes C1 split and transposed: C1h/C1l[i][mu], mu padded to a multiple of 32
+  const int ldt = ((nb + 3) / 4) * 32;
+  bf16 *d1h = nullptr, *d1l = nullptr;
+  if (precision == ZQ_PREC_BF16X3) {
+    std::vector<bf16> hh(size_t(m1) * ldt, (bf16)0.f), hl(size_t(m1) * ldt, (bf16)0.f);
+    for (int i = 0; i < m1; ++i)
+      for (int mu = 0; mu < n; ++mu) {
+        const float x = float(C[size_t(mu) * n + i]);
+        const bf16 h = (bf16)x;
+        hh[size_t(i) * ldt + mu] = h;
+        hl[size_t(i) * ldt + mu] = (bf16)(x - (float)h);
+      }
+    if (int rc = upload(ctx, "C1h", hh, &d1h)) return rc;
+    if (int rc = upload(ctx, "C1l", hl, &d1l)) return rc;
+  }
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the host staging vectors go out of scope below
+
   const size_t nX = size_t(Nb) * 64 * m1 * np, nY = size_t(m1) * m2 * np * np, nW = size_t(m1) * m2 * np * m3,
                nO = size_t(m1) * m2 * m3 * m4;
-  double *X = (double *)zq_scratch(ctx, "X", nX * 8);
-  double *Y = (double *)zq_scratch(ctx, "Y", nY * 8);
-  double *W = (double *)zq_scratch(ctx, "W", nW * 8);
-  double *O = (double *)zq_scratch(ctx, "OUT", nO * 8);
-  if (!d1 || !d2t || !d3 || !X || !Y || !W || !O) return zq_fail(ctx, ZQ_ERR_OOM, "transform scratch");
-  ZQ_TRY(ctx, hipMemcpyAsync(d1, h1.data(), h1.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  ZQ_TRY(ctx, hipMemcpyAsync(d2t, h2t.data(), h2t.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  ZQ_TRY(ctx, hipMemcpyAsync(d3, h3.data(), h3.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the host staging vectors go out of scope below
+  T *X = (T *)zq_scratch(ctx, "X", nX * sizeof(T));
+  T *Y = (T *)zq_scratch(ctx, "Y", nY * sizeof(T));
+  T *W = (T *)zq_scratch(ctx, "W", nW * sizeof(T));
+  T *O = (T *)zq_scratch(ctx, "OUT", nO * sizeof(T));
+  if (!X || !Y || !W || !O) return zq_fail(ctx, ZQ_ERR_OOM, "transform scratch");
 
   // Q1
   {
@@ -288,69 +529,124 @@ int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int p
     const int mt = std::min(4, (std::min(m1, 64) + 15) / 16);
     dim3 grid(unsigned(Nb * nb), unsigned(groups));
     zq_prof_begin(ctx, "ao2mo_q1");
-    switch (mt) {
-      case 1: hipLaunchKernelGGL(zq_q1_f64<1>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
-      case 2: hipLaunchKernelGGL(zq_q1_f64<2>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
-      case 3: hipLaunchKernelGGL(zq_q1_f64<3>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
-      default: hipLaunchKernelGGL(zq_q1_f64<4>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
+    if (precision == ZQ_PREC_BF16X3) {
+      float *Xf = (float *)X;
+      switch (mt) {
+        case 1: hipLaunchKernelGGL(zq_q1_bf16x3<1>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1h, d1l, ldt, m1, nb, np, Xf); break;
+        case 2: hipLaunchKernelGGL(zq_q1_bf16x3<2>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1h, d1l, ldt, m1, nb, np, Xf); break;
+        case 3: hipLaunchKernelGGL(zq_q1_bf16x3<3>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1h, d1l, ldt, m1, nb, np, Xf); break;
+        default: hipLaunchKernelGGL(zq_q1_bf16x3<4>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1h, d1l, ldt, m1, nb, np, Xf); break;
+      }
+    } else {
+      switch (mt) {
+        case 1: hipLaunchKernelGGL((zq_q1<MF, 1>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
+        case 2: hipLaunchKernelGGL((zq_q1<MF, 2>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
+        case 3: hipLaunchKernelGGL((zq_q1<MF, 3>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
+        default: hipLaunchKernelGGL((zq_q1<MF, 4>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
+      }
     }
     zq_prof_end(ctx);
-    ZQ_LAUNCH_CHECK(ctx, "zq_q1_f64");
+    ZQ_LAUNCH_CHECK(ctx, "ao2mo_q1");
   }
+  GemmArgs<T> g2{}, g3{}, g4{};
   // Q2: batch over i; A = C2T (m2 x np), B gathered from X, out Y[i][j][sig][nu]
-  {
-    GemmArgs g{};
-    g.A = d2t, g.sAb = 0, g.sAm = np, g.sAk = 1;
-    g.B = X, g.sBb = np, g.mode = 1, g.npad = np, g.sXs = (long long)m1 * np;
-    g.C = Y, g.sCb = (long long)m2 * np * np, g.sCm = (long long)np * np, g.sCn = 1;
-    g.M = m2, g.N = np * np, g.K = np;
-    if (int rc = launch_gemm(ctx, g, m1, "ao2mo_q2")) return rc;
-  }
+  g2.A = d2t, g2.sAb = 0, g2.sAm = np, g2.sAk = 1;
+  g2.B = X, g2.sBb = np, g2.mode = 1, g2.npad = np, g2.sXs = (long long)m1 * np;
+  g2.C = Y, g2.sCb = (long long)m2 * np * np, g2.sCm = (long long)np * np, g2.sCn = 1;
+  g2.M = m2, g2.N = np * np, g2.K = np;
   // Q3: batch over (i,j); A = Y_ij [sig][nu], B = C3 [nu][a], out W[i][j][sig][a]
-  {
-    GemmArgs g{};
-    g.A = Y, g.sAb = (long long)np * np, g.sAm = np, g.sAk = 1;
-    g.B = d3, g.sBb = 0, g.sBk = m3, g.sBn = 1;
-    g.C = W, g.sCb = (long long)np * m3, g.sCm = m3, g.sCn = 1;
-    g.M = np, g.N = m3, g.K = np;
-    if (int rc = launch_gemm(ctx, g, m1 * m2, "ao2mo_q3")) return rc;
-  }
+  g3.A = Y, g3.sAb = (long long)np * np, g3.sAm = np, g3.sAk = 1;
+  g3.B = d3, g3.sBb = 0, g3.sBk = m3, g3.sBn = 1;
+  g3.C = W, g3.sCb = (long long)np * m3, g3.sCm = m3, g3.sCn = 1;
+  g3.M = np, g3.N = m3, g3.K = np;
   // Q4: batch over (i,j); A[a][sig] = W[sig][a], B = C4 [sig][b], out[i][j][a][b]
-  {
-    GemmArgs g{};
-    g.A = W, g.sAb = (long long)np * m3, g.sAm = 1, g.sAk = m3;
-    g.B = d3, g.sBb = 0, g.sBk = m4, g.sBn = 1;
-    g.C = O, g.sCb = (long long)m3 * m4, g.sCm = m4, g.sCn = 1;
-    g.M = m3, g.N = m4, g.K = np;
-    if (int rc = launch_gemm(ctx, g, m1 * m2, "ao2mo_q4")) return rc;
+  g4.A = W, g4.sAb = (long long)np * m3, g4.sAm = 1, g4.sAk = m3;
+  g4.B = d3, g4.sBb = 0, g4.sBk = m4, g4.sBn = 1;
+  g4.C = O, g4.sCb = (long long)m3 * m4, g4.sCm = m4, g4.sCn = 1;
+  g4.M = m3, g4.N = m4, g4.K = np;
+  if constexpr (sizeof(T) == 4) {
+    if (precision == ZQ_PREC_BF16X3) {
+      if (int rc = launch_gemm_bf16x3(ctx, g2, m1, "ao2mo_q2")) return rc;
+      if (int rc = launch_gemm_bf16x3(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
+      if (int rc = launch_gemm_bf16x3(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
+      *d_out = O;
+      return ZQ_OK;
+    }
   }
+  if (int rc = launch_gemm<MF>(ctx, g2, m1, "ao2mo_q2")) return rc;
+  if (int rc = launch_gemm<MF>(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
+  if (int rc = launch_gemm<MF>(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
   *d_out = O;
   return ZQ_OK;
+}
+
+}  // namespace
+
+// Runs Q1..Q4 into device memory.  *d_out receives out[i][j][a][b] (dims m1 x m2 x m3 x m4) as double
+// (FP64) or float (FP32, BF16X3); *is_f32 tells which.
+int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, void **d_out, int dims[4],
+                        bool *is_f32) {
+  if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "no ERIs loaded");
+  Dims d;
+  d.n = ctx->n, d.np = ctx->npad, d.nb = ctx->nb, d.Nb = ctx->Nb;
+  if (block != ZQ_BLOCK_FULL && block != ZQ_BLOCK_OVOV) return zq_fail(ctx, ZQ_ERR_INVALID, "bad block");
+  if (precision != ZQ_PREC_FP64 && precision != ZQ_PREC_FP32 && precision != ZQ_PREC_BF16X3)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "bad precision");
+  if (block == ZQ_BLOCK_OVOV && (nocc <= 0 || nocc >= d.n)) return zq_fail(ctx, ZQ_ERR_INVALID, "bad nocc");
+  d.m1 = block == ZQ_BLOCK_OVOV ? nocc : d.n, d.m2 = d.m1;
+  d.off3 = block == ZQ_BLOCK_OVOV ? nocc : 0, d.m3 = d.n - d.off3, d.m4 = d.m3;
+  dims[0] = d.m1, dims[1] = d.m2, dims[2] = d.m3, dims[3] = d.m4;
+  if (precision == ZQ_PREC_FP64) {
+    *is_f32 = false;
+    double *o = nullptr;
+    int rc = run_chain<double, MF64>(ctx, C, d, precision, &o);
+    *d_out = o;
+    return rc;
+  }
+  *is_f32 = true;
+  float *o = nullptr;
+  int rc = run_chain<float, MF32>(ctx, C, d, precision, &o);
+  *d_out = o;
+  return rc;
 }
 
 extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out) {
   if (!ctx || !C || !out) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_ao2mo: null argument");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
-  double *d = nullptr;
+  void *d = nullptr;
   int dims[4];
-  if (int rc = zq_transform_device(ctx, C, int(nocc), block, precision, &d, dims)) return rc;
+  bool f32 = false;
+  if (int rc = zq_transform_device(ctx, C, int(nocc), block, precision, &d, dims, &f32)) return rc;
   const size_t cnt = size_t(dims[0]) * dims[1] * dims[2] * dims[3];
-  ZQ_TRY(ctx, hipMemcpyAsync(out, d, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  const double *src = (const double *)d;
+  if (f32) {
+    double *wide = (double *)zq_scratch(ctx, "OUT64", cnt * sizeof(double));
+    if (!wide) return zq_fail(ctx, ZQ_ERR_OOM, "zq_ao2mo: widen buffer");
+    hipLaunchKernelGGL(zq_widen, dim3(1024), dim3(256), 0, ctx->stream, (const float *)d, cnt, wide);
+    ZQ_LAUNCH_CHECK(ctx, "zq_widen");
+    src = wide;
+  }
+  ZQ_TRY(ctx, hipMemcpyAsync(out, src, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return ZQ_OK;
 }
 
 int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr) {
-  double *T = nullptr;
+  void *T = nullptr;
   int dims[4];
-  if (int rc = zq_transform_device(ctx, C, nocc, ZQ_BLOCK_OVOV, precision, &T, dims)) return rc;
+  bool f32 = false;
+  if (int rc = zq_transform_device(ctx, C, nocc, ZQ_BLOCK_OVOV, precision, &T, dims, &f32)) return rc;
   const int o = dims[0], v = dims[2];
   double *de = (double *)zq_scratch(ctx, "mp2_e", size_t(ctx->n) * 8);
   double *part = (double *)zq_scratch(ctx, "mp2_part", size_t(o) * o * 8 + 8);
   if (!de || !part) return zq_fail(ctx, ZQ_ERR_OOM, "mp2 scratch");
   ZQ_TRY(ctx, hipMemcpyAsync(de, e, size_t(ctx->n) * 8, hipMemcpyHostToDevice, ctx->stream));
   zq_prof_begin(ctx, "mp2_energy");
-  hipLaunchKernelGGL(zq_mp2_pairs, dim3(unsigned(o * o)), dim3(256), 0, ctx->stream, T, de, o, v, part);
+  if (f32)
+    hipLaunchKernelGGL(zq_mp2_pairs<float>, dim3(unsigned(o * o)), dim3(256), 0, ctx->stream, (const float *)T, de, o, v, part);
+  else
+    hipLaunchKernelGGL(zq_mp2_pairs<double>, dim3(unsigned(o * o)), dim3(256), 0, ctx->stream, (const double *)T, de, o, v,
+                       part);
   hipLaunchKernelGGL(zq_sum_serial, dim3(1), dim3(64), 0, ctx->stream, part, o * o, part + size_t(o) * o);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_mp2_pairs");

@@ -58,9 +58,9 @@ def prepare(ks, basis):
     return mols
 
 
-def step(mols, precision, results=None):
+def step(mols, precision, results=None, device_scf=True):
     for m in mols:
-        r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=1e-10)
+        r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=1e-10, device=device_scf)
         e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], precision)
         if results is not None:
             results.append(dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"]))
@@ -100,6 +100,7 @@ def main():
     ap.add_argument("--basis", default="builtin:cc-pvdz")
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scf", default="device", choices=["device", "host"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
@@ -107,12 +108,13 @@ def main():
     ks = [int(k) for k in args.ks.split(",")]
     mols = prepare(ks, args.basis)
 
+    dev = args.scf == "device"
     for _ in range(args.warmup):
-        step(mols, args.precision)
+        step(mols, args.precision, device_scf=dev)
     results = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(mols, args.precision, results if i == 0 else None)
+        step(mols, args.precision, results if i == 0 else None, device_scf=dev)
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     for r in results:
@@ -125,12 +127,12 @@ def main():
     for m in mols:
         m["ctx"].profile(True)
         m["ctx"].profile_reset()
-    step(mols, args.precision)
+    step(mols, args.precision, device_scf=dev)
     per_mol = []
     for m in mols:
         c = m["ctx"]
         d = {name: c.profile_get(name) for name in ("jk_records", "jk_reduce", "ao2mo_q1", "ao2mo_q2", "ao2mo_q3",
-                                                     "ao2mo_q4", "mp2_energy")}
+                                                     "ao2mo_q4", "mp2_energy", "scf_syevj", "scf_gemm")}
         c.profile(False)
         t_tr = sum(d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")) * 1e-3
         jk_ms, jk_cnt = d["jk_records"]
@@ -138,7 +140,8 @@ def main():
                             jk_ms_avg=jk_ms / max(jk_cnt, 1), jk_reduce_ms_avg=d["jk_reduce"][0] / max(d["jk_reduce"][1], 1),
                             jk_GBs=jk_bytes(m["n"]) / (jk_ms / max(jk_cnt, 1) * 1e-3) / 1e9,
                             transform_ms=t_tr * 1e3, transform_TFLOPs=ovov_flops(m["n"], m["o"]) / t_tr / 1e12,
-                            q_ms={q: d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")}))
+                            q_ms={q: d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")},
+                            syevj_ms=d["scf_syevj"][0], scf_gemm_ms=d["scf_gemm"][0]))
         for name, (ms, cnt) in d.items():
             kern.setdefault(name, [0.0, 0, 0.0])
             kern[name][0] += ms
@@ -166,7 +169,7 @@ def main():
                ms_per_step=round(ms_per_step, 2), higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f64" if args.precision == "fp64" else args.precision, data="synthetic",
                config=dict(workload=f"RHF+MP2 (H2O)_k k={ks} {args.basis.split(':')[-1]}, DIIS SCF to 1e-10, "
-                                    f"ERIs resident 8-fold packed", precision=args.precision),
+                                    f"ERIs resident 8-fold packed", precision=args.precision, scf=args.scf),
                transform_TFLOPs={f"k{p['k']}": round(p["transform_TFLOPs"], 2) for p in per_mol},
                jk_GBs={f"k{p['k']}": round(p["jk_GBs"], 1) for p in per_mol},
                roofline=roof, results=results, per_molecule=per_mol,

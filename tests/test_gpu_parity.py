@@ -158,3 +158,44 @@ def test_reduced_precision_full_transform(ctx, prec):
     err = np.abs(ctx.ao2mo(C, 0, "full", prec) - O.ao2mo_pppp(g, C)).max()
     print(f"full {prec}: {err:.3e}")
     assert err < PREC_TOL[prec][0] * 5
+
+
+# ---------------------------------------------------------------- device-resident SCF
+@pytest.mark.parametrize("kind", ["sto3g", "ccpvdz", "w2"])
+def test_device_scf_matches_oracle(ctx, kind):
+    s, H, S, g, VN = _load(ctx, kind)
+    o = s.nel // 2
+    r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    ro = O.scf(H, S, g, VN, o, threshold=1e-10)
+    assert r["E"] == pytest.approx(ro["E"], abs=1e-10)
+    # geigh contract: F C = S C e, C^T S C = I, e ascending; MP2 from these orbitals agrees
+    C, e = r["C"], r["e"]
+    np.testing.assert_allclose(C.T @ S @ C, np.eye(s.nbf), atol=1e-10)
+    np.testing.assert_allclose(r["F"] @ C, S @ C * e, atol=1e-9)
+    assert np.all(np.diff(e) >= -1e-12)
+    e2 = ctx.mp2(C, e, o)
+    e2o = O.mp2_closed(O.ao2mo_oovv(g, ro["C"], o), ro["e"])
+    assert e2 == pytest.approx(e2o, abs=1e-10)
+
+
+def test_device_scf_ghf_and_reference_anchor(ctx):
+    from zinq_amd.system import spin_block_1e, spin_block_2e
+    s, H, S, g, VN = system_ints("sto3g")
+    Hs, Ss, gs = spin_block_1e(H), spin_block_1e(S), spin_block_2e(g)
+    ctx.eri_load(gs)
+    r = ctx.scf(Hs, Ss, VN, s.nel, generalized=True, device=True)
+    assert r["E"] == pytest.approx(-74.9659012172972300, abs=1e-8)  # test/hartree_fock.zig:56
+    ctx.eri_load_packed(s.coulomb_packed(), s.nbf)
+    r = ctx.scf(H, S, VN, s.nel // 2, device=True)
+    assert r["E"] == pytest.approx(-74.9659012172971900, abs=1e-8)  # test/hartree_fock.zig:24
+
+
+def test_device_scf_no_diis_and_not_converged(ctx):
+    from zinq_amd import hip
+    s, H, S, g, VN = _load(ctx, "sto3g")
+    o = s.nel // 2
+    r = ctx.scf(H, S, VN, o, diis=None, device=True)
+    ro = O.scf(H, S, g, VN, o, diis=None)
+    assert r["E"] == pytest.approx(ro["E"], abs=1e-9)
+    with pytest.raises(hip.ScfDidNotConverge):
+        ctx.scf(H, S, VN, o, iterations=1, device=True)

@@ -169,6 +169,7 @@ void zq_destroy(zq_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   prof_collect(ctx);
+  zq_release_solver(ctx);
   zq_free_eri(ctx);
   for (auto &kv : ctx->scratch)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);

@@ -72,6 +72,9 @@ struct zq_ctx {
   int *d_csr_ptr = nullptr, *d_csr_idx = nullptr;
   double *d_P = nullptr, *d_H = nullptr, *d_F = nullptr;
 
+  // rocBLAS handle used by the device SCF's eigensolver (created on first use)
+  void *solver = nullptr;
+
   // named scratch buffers (grow-only)
   std::map<std::string, DevBuf> scratch;
 
@@ -92,6 +95,10 @@ void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh
 void zq_free_eri(zq_ctx *ctx);
 void zq_prof_begin(zq_ctx *ctx, const char *name);
 void zq_prof_end(zq_ctx *ctx);
+void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip
+int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF);  // zq_jk.hip
+int zq_dgemm(zq_ctx *ctx, const double *A, long long sAm, long long sAk, const double *B, long long sBk, long long sBn,
+             double *C, long long sCm, long long sCn, int M, int N, int K, const char *name);  // zq_transform.hip
 
 #define ZQ_TRY(ctx, call)                                \
   do {                                                   \

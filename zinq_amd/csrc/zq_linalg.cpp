@@ -208,6 +208,31 @@ int sygv(int n, double *A, double *B, double *w) {
   return 0;
 }
 
+int chol_inv_t(int n, const double *S, double *X) {
+  std::vector<double> L(size_t(n) * n, 0.0), Li(size_t(n) * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double s = S[size_t(j) * n + j];
+    for (int k = 0; k < j; ++k) s -= L[size_t(j) * n + k] * L[size_t(j) * n + k];
+    if (s <= 0.0) return 1;
+    L[size_t(j) * n + j] = std::sqrt(s);
+    for (int i = j + 1; i < n; ++i) {
+      double t = S[size_t(i) * n + j];
+      for (int k = 0; k < j; ++k) t -= L[size_t(i) * n + k] * L[size_t(j) * n + k];
+      L[size_t(i) * n + j] = t / L[size_t(j) * n + j];
+    }
+  }
+  // Li = L^-1 (lower), column by column forward substitution
+  for (int c = 0; c < n; ++c)
+    for (int i = c; i < n; ++i) {
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) t -= L[size_t(i) * n + k] * Li[size_t(k) * n + c];
+      Li[size_t(i) * n + c] = t / L[size_t(i) * n + i];
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) X[size_t(i) * n + j] = Li[size_t(j) * n + i];
+  return 0;
+}
+
 int getrf(int n, double *A, int *piv) {
   for (int k = 0; k < n; ++k) {
     int p = k;

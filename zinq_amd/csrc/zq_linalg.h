@@ -20,6 +20,9 @@ int syev(int n, double *A, double *w);
 // its Cholesky factor.  Returns 0, 1 (QL failure) or 2 (B not positive definite).
 int sygv(int n, double *A, double *B, double *w);
 
+// X = L^-T for the Cholesky factor S = L L^T (so X^T S X = I); 0, or 1 if S is not positive definite.
+int chol_inv_t(int n, const double *S, double *X);
+
 // LU with partial pivoting; 0 on success, 1 on an exactly singular pivot (dgetrf info > 0).
 int getrf(int n, double *A, int *piv);
 void getrs(int n, const double *LU, const int *piv, double *b);

@@ -582,6 +582,17 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
 
 }  // namespace
 
+// C[m][n] = sum_k A[m][k] B[k][n] (FP64 MFMA, arbitrary element strides), for the SCF driver.
+int zq_dgemm(zq_ctx *ctx, const double *A, long long sAm, long long sAk, const double *B, long long sBk, long long sBn,
+             double *C, long long sCm, long long sCn, int M, int N, int K, const char *name) {
+  GemmArgs<double> g{};
+  g.A = A, g.sAm = sAm, g.sAk = sAk;
+  g.B = B, g.sBk = sBk, g.sBn = sBn;
+  g.C = C, g.sCm = sCm, g.sCn = sCn;
+  g.M = M, g.N = N, g.K = K;
+  return launch_gemm<MF64>(ctx, g, 1, name);
+}
+
 // Runs Q1..Q4 into device memory.  *d_out receives out[i][j][a][b] (dims m1 x m2 x m3 x m4) as double
 // (FP64) or float (FP32, BF16X3); *is_f32 tells which.
 int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, void **d_out, int dims[4],

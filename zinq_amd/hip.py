@@ -16,7 +16,7 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_load_packed", "zq_fock_jk", "zq_ao2mo",
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
-            "zq_host_sygv"]
+            "zq_host_sygv", "zq_scf_run_device"]
 
 
 class GpuError(RuntimeError):
@@ -81,6 +81,7 @@ def lib():
         L.zq_scf_run.argtypes = [vp, sz, sz, vp, vp, cd, ctypes.POINTER(ScfOptions), vp, vp, vp, vp,
                                  ctypes.POINTER(ScfResult)]
         L.zq_host_sygv.argtypes = [sz, vp, vp, vp]
+        L.zq_scf_run_device.argtypes = L.zq_scf_run.argtypes
         L._zq_typed = True
     return L
 
@@ -164,13 +165,16 @@ class Context:
         self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
         return float(out[0])
 
-    def scf(self, H, S, VN, nocc, diis=8, iterations=100, threshold=1e-8, generalized=False):
+    def scf(self, H, S, VN, nocc, diis=8, iterations=100, threshold=1e-8, generalized=False, device=False):
+        """zq_scf_run (host DIIS/diagonalisation, reference-faithful) or, with device=True,
+        zq_scf_run_device (everything n x n on the GPU)."""
         H, S = _f64(H), _f64(S)
         n = H.shape[0]
         C, e, P, F = np.zeros((n, n)), np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
         opt = ScfOptions(int(diis or 0), int(iterations), float(threshold), int(bool(generalized)))
         res = ScfResult()
-        rc = self._L.zq_scf_run(self._h, n, nocc, ptr(H), ptr(S), float(VN), ctypes.byref(opt), ptr(C), ptr(e), ptr(P),
+        fn = self._L.zq_scf_run_device if device else self._L.zq_scf_run
+        rc = fn(self._h, n, nocc, ptr(H), ptr(S), float(VN), ctypes.byref(opt), ptr(C), ptr(e), ptr(P),
                                 ptr(F), ctypes.byref(res))
         self._check(rc)
         return dict(E=res.energy, iterations=res.iterations, C=C, e=e, P=P, F=F)

@@ -58,12 +58,23 @@ def prepare(ks, basis):
     return mols
 
 
-def step(mols, precision, results=None, device_scf=True):
-    for m in mols:
-        r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=1e-10, device=device_scf)
-        e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], precision)
-        if results is not None:
-            results.append(dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"]))
+def run_one(m, precision, device_scf):
+    r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=1e-10, device=device_scf)
+    e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], precision)
+    return dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"])
+
+
+def step(mols, precision, results=None, device_scf=True, concurrent=True):
+    """All molecules of the step; with concurrent=True each runs in its own host thread on its own
+    context/stream (the C calls release the GIL), so their GPU work overlaps."""
+    if concurrent:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(len(mols)) as ex:
+            out = list(ex.map(lambda m: run_one(m, precision, device_scf), mols))
+    else:
+        out = [run_one(m, precision, device_scf) for m in mols]
+    if results is not None:
+        results.extend(out)
 
 
 def cpu_baseline(mols, seconds_budget=30.0):
@@ -101,20 +112,22 @@ def main():
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scf", default="device", choices=["device", "host"])
+    ap.add_argument("--serial", action="store_true", help="run the molecules of a step one after another")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).", file=sys.stderr)
+        print("bench.py measures one GPU: only the one-GPU path is built; a multi-GPU run is not supported.", file=sys.stderr)
         sys.exit(2)
     ks = [int(k) for k in args.ks.split(",")]
     mols = prepare(ks, args.basis)
 
     dev = args.scf == "device"
+    conc = not args.serial
     for _ in range(args.warmup):
-        step(mols, args.precision, device_scf=dev)
+        step(mols, args.precision, device_scf=dev, concurrent=conc)
     results = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(mols, args.precision, results if i == 0 else None, device_scf=dev)
+        step(mols, args.precision, results if i == 0 else None, device_scf=dev, concurrent=conc)
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     for r in results:
@@ -127,12 +140,12 @@ def main():
     for m in mols:
         m["ctx"].profile(True)
         m["ctx"].profile_reset()
-    step(mols, args.precision, device_scf=dev)
+    step(mols, args.precision, device_scf=dev, concurrent=False)
     per_mol = []
     for m in mols:
         c = m["ctx"]
         d = {name: c.profile_get(name) for name in ("jk_records", "jk_reduce", "ao2mo_q1", "ao2mo_q2", "ao2mo_q3",
-                                                     "ao2mo_q4", "mp2_energy", "scf_syevj", "scf_gemm")}
+                                                     "ao2mo_q4", "mp2_energy", "scf_eig", "scf_gemm")}
         c.profile(False)
         t_tr = sum(d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")) * 1e-3
         jk_ms, jk_cnt = d["jk_records"]
@@ -141,7 +154,7 @@ def main():
                             jk_GBs=jk_bytes(m["n"]) / (jk_ms / max(jk_cnt, 1) * 1e-3) / 1e9,
                             transform_ms=t_tr * 1e3, transform_TFLOPs=ovov_flops(m["n"], m["o"]) / t_tr / 1e12,
                             q_ms={q: d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")},
-                            syevj_ms=d["scf_syevj"][0], scf_gemm_ms=d["scf_gemm"][0]))
+                            eig_ms=d["scf_eig"][0], scf_gemm_ms=d["scf_gemm"][0]))
         for name, (ms, cnt) in d.items():
             kern.setdefault(name, [0.0, 0, 0.0])
             kern[name][0] += ms
@@ -149,7 +162,7 @@ def main():
 
     # roofline of the dominant kernel, for the largest molecule
     big = max(per_mol, key=lambda x: x["n"])
-    dom = max(kern.items(), key=lambda kv: kv[1][0])[0]
+    dom = max(("jk_records", "ao2mo_q1"), key=lambda k: kern[k][0])  # hot-path kernels only
     if dom.startswith("jk"):
         achieved = big["jk_GBs"]
         roof = dict(kernel="zq_jk_records", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -169,7 +182,8 @@ def main():
                ms_per_step=round(ms_per_step, 2), higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f64" if args.precision == "fp64" else args.precision, data="synthetic",
                config=dict(workload=f"RHF+MP2 (H2O)_k k={ks} {args.basis.split(':')[-1]}, DIIS SCF to 1e-10, "
-                                    f"ERIs resident 8-fold packed", precision=args.precision, scf=args.scf),
+                                    f"ERIs resident 8-fold packed", precision=args.precision, scf=args.scf,
+                           molecules_concurrent=conc),
                transform_TFLOPs={f"k{p['k']}": round(p["transform_TFLOPs"], 2) for p in per_mol},
                jk_GBs={f"k{p['k']}": round(p["jk_GBs"], 1) for p in per_mol},
                roofline=roof, results=results, per_molecule=per_mol,

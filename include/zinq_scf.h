@@ -38,8 +38,8 @@ int zq_scf_run(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double
                const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);
 
 /* Same contract as zq_scf_run with every n x n step on the GPU (J/K, energy, DIIS error and
- * extrapolation, density); the diagonalisation is a rocSOLVER Jacobi solve of F in the previous MO
- * basis (C_prev^T F C_prev, nearly diagonal as the SCF converges), C = C_prev C'.  Only the <= 9 x 9
+ * extrapolation, density); the diagonalisation is a rocSOLVER dsyevd of F in the previous MO basis
+ * (C_prev^T F C_prev, with C_prev^T S C_prev = I), C = C_prev C'.  Only the <= 9 x 9
  * DIIS system and the convergence test run on the host. */
 int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double *S, double VN,
                       const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);

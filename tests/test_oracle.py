@@ -76,7 +76,7 @@ def test_rcisd_energies_match_reference():
 # ---------------------------------------------------------------- mathematics the oracle must satisfy
 
 def test_closed_shell_mp2_equals_spin_orbital_rspt(rhf):
-    """The determinant-space order-2 term equals the closed-shell sum over (ia|jb)."""
+    """The determinant-space order-2 term equals the closed-shell sum over (ia|jb): E2 = sum (ia|jb)[2(ia|jb) - (ib|ja)] / (e_i + e_j - e_a - e_b)."""
     s, H, S, g, VN, r = rhf
     o = s.nel // 2
     e2_det = O.mp(2, g, r["C"], r["e"], o)[2]

@@ -7,8 +7,8 @@
 //     B row  = <err_new, err_i>                   deterministic reductions; the <= 9 x 9 DIIS system
 //                                                 is solved on the host (diis, :176-227) and the
 //     F_ext  = sum c_i F_i, symmetrised           combination runs on the GPU
-//     F' C'  = C' e with F' = C_prev^T F_ext C_prev   rocSOLVER Jacobi (syevj): in the previous MO basis
-//                                                 F' is nearly diagonal, so few sweeps are needed;
+//     F' C'  = C' e with F' = C_prev^T F_ext C_prev   rocSOLVER divide and conquer (syevd), in the
+//                                                 previous MO basis (F' nearly diagonal near convergence);
 //     C      = C_prev C'                          C_prev^T S C_prev = I, hence F C = S C e (geigh)
 //     P      = f C_occ C_occ^T, rms(dP)           (getDensity, :621-640)
 //   The core guess uses C_prev = L^-T from the Cholesky factor S = L L^T.
@@ -129,7 +129,7 @@ struct Dev {
   zq_ctx *ctx;
   int n;
   size_t nn;
-  double *S, *X, *Cprev, *C, *T, *Fp, *W, *FP, *FPS, *err, *Fh, *Eh, *part, *scal, *coef, *eig;
+  double *S, *X, *Cprev, *C, *T, *Fp, *W, *FP, *FPS, *err, *Fh, *Eh, *part, *scal, *coef, *eig, *offd;
   int *slots, *info, *nsw;
   double *resid;
 };
@@ -145,7 +145,7 @@ int alloc(zq_ctx *ctx, Dev &d, int n, int maxh) {
               {"scf_T", &d.T, nn},       {"scf_Fp", &d.Fp, nn},   {"scf_W", &d.W, nn},      {"scf_FP", &d.FP, nn},
               {"scf_FPS", &d.FPS, nn},   {"scf_err", &d.err, nn}, {"scf_Fh", &d.Fh, nn * maxh},
               {"scf_Eh", &d.Eh, nn * maxh}, {"scf_part", &d.part, size_t(kRedBlocks) * 16},
-              {"scf_scal", &d.scal, 16}, {"scf_coef", &d.coef, 16}, {"scf_eig", &d.eig, size_t(n)},
+              {"scf_scal", &d.scal, 16}, {"scf_coef", &d.coef, 16}, {"scf_eig", &d.eig, size_t(n)}, {"scf_offd", &d.offd, size_t(n)},
               {"scf_resid", &d.resid, 4}};
   for (auto &b : bufs) {
     *b.p = (double *)zq_scratch(ctx, b.name, b.cnt * sizeof(double));
@@ -177,11 +177,11 @@ int diag_in_basis(Dev &d, const double *A, const double *Cb, double *Cout) {
   if (int rc = zq_dgemm(ctx, d.T, n, 1, Cb, n, 1, d.Fp, n, 1, n, n, n, "scf_gemm")) return rc;  // Fp = T Cb
   rocblas_handle h = solver(ctx);
   if (!h) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocblas handle");
-  zq_prof_begin(ctx, "scf_syevj");
-  const rocblas_status st = rocsolver_dsyevj(h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n, d.Fp,
-                                             n, 0.0, d.resid, 100, d.nsw, d.eig, d.info);
+  zq_prof_begin(ctx, "scf_eig");
+  const rocblas_status st =
+      rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, d.Fp, n, d.eig, d.offd, d.info);
   zq_prof_end(ctx);
-  if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dsyevj");
+  if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dsyevd");
   // eigenvectors are the columns of the column-major Fp: V[k][p] = Fp[p*n + k]
   return zq_dgemm(ctx, Cb, n, 1, d.Fp, 1, n, Cout, n, 1, n, n, n, "scf_gemm");
 }

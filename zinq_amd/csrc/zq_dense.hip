@@ -1,0 +1,279 @@
+// Small dense FP64 kernels of the device SCF: an n x n GEMM sized for latency and the SP2 density
+// purification that replaces the per-iteration diagonalisation.
+//
+// zq_gemm_nt: C = alpha A B^T (A: M x K, B: N x K, both read along rows).  One 64-lane wave owns a
+//   16 x 16 tile of C and walks K in chunks of 16 with v_mfma_f64_16x16x4f64.  Inside a chunk lane l
+//   (r = l % 16, q = l / 16) holds A[i0+r][k0+4q .. k0+4q+3] and B[j0+r][same k]: the 4 MFMAs of the
+//   chunk take element t of every lane, i.e. the k order inside a chunk is permuted, which leaves the
+//   sum unchanged and turns every operand fetch into one 32-byte row segment.  The next 64 k are
+//   loaded before the current 64 are multiplied.  No LDS, no barriers: the kernel is latency-bound at
+//   the SCF's sizes (n <= a few hundred), so the grid has one small workgroup per tile.
+//   Every operand of the SCF is symmetric or stored transposed, so A B^T covers all of its products.
+//
+// SP2 (second-order spectral projection, A. M. N. Niklasson, Phys. Rev. B 66, 155115 (2002)):
+//   X_0 = (e_max I - F') / (e_max - e_min)    Gershgorin bounds of the orthonormal-basis Fock matrix
+//   X_{k+1} = X_k^2        if |tr(X_k^2) - N| <= |2 tr(X_k) - tr(X_k^2) - N|
+//             2 X_k - X_k^2 otherwise
+//   converges to the projector onto the N lowest eigenvectors of F' (the aufbau density) when F' has
+//   a gap between eigenvalues N and N+1.  idem_k = tr(X_k) - tr(X_k^2) = sum lambda (1 - lambda) >= 0
+//   bounds every eigenvalue's distance to {0, 1}; once idem_k < tol two more steps square every
+//   eigenvalue's distance at least once, and the iteration stops.
+//   One kernel per step: X symmetric, so only tiles ti <= tj are computed and mirrored; the epilogue
+//   applies the branch and writes per-tile partial traces (tr X, ||X||_F^2 = tr X^2), which every
+//   workgroup of the next step sums in the same fixed order, so all of them take the same branch.
+//   X, the partials and the stop state are ping-ponged between two buffers (input [p], output [p^1]).
+
+#include <algorithm>
+
+#include "zq_internal.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct Frag {
+  double a[4][4], b[4][4];  // [chunk][t]
+};
+
+// Loads the 64-k block starting at kb of rows ia (A) and jb (B) for lane group q.
+template <bool VEC>
+__device__ inline void load_block(Frag &f, const double *__restrict__ A, int lda, const double *__restrict__ B, int ldb,
+                                  int ia, int jb, int M, int N, int K, int kb, int q) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int k = kb + 16 * c + 4 * q;
+    if (VEC) {
+      // K % 16 == 0, even leading dimensions, 16-byte aligned bases: a chunk lies wholly inside K
+      const bool kin = kb + 16 * c < K;
+      if (kin && ia < M) {
+        const double2 *p = reinterpret_cast<const double2 *>(A + size_t(ia) * lda + k);
+        const double2 u = p[0], v = p[1];
+        f.a[c][0] = u.x, f.a[c][1] = u.y, f.a[c][2] = v.x, f.a[c][3] = v.y;
+      } else {
+        f.a[c][0] = f.a[c][1] = f.a[c][2] = f.a[c][3] = 0.0;
+      }
+      if (kin && jb < N) {
+        const double2 *p = reinterpret_cast<const double2 *>(B + size_t(jb) * ldb + k);
+        const double2 u = p[0], v = p[1];
+        f.b[c][0] = u.x, f.b[c][1] = u.y, f.b[c][2] = v.x, f.b[c][3] = v.y;
+      } else {
+        f.b[c][0] = f.b[c][1] = f.b[c][2] = f.b[c][3] = 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        f.a[c][t] = (ia < M && k + t < K) ? A[size_t(ia) * lda + k + t] : 0.0;
+        f.b[c][t] = (jb < N && k + t < K) ? B[size_t(jb) * ldb + k + t] : 0.0;
+      }
+    }
+  }
+}
+
+// acc = the 16 x 16 tile (i0, j0) of A B^T; element j of lane l is row i0 + 4j + l/16, column j0 + l%16.
+template <bool VEC>
+__device__ inline d4 tile_nt(const double *__restrict__ A, int lda, const double *__restrict__ B, int ldb, int i0, int j0,
+                             int M, int N, int K) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  const int ia = i0 + r, jb = j0 + r;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  Frag cur, nxt;
+  load_block<VEC>(cur, A, lda, B, ldb, ia, jb, M, N, K, 0, q);
+  for (int kb = 0; kb < K; kb += 64) {
+    if (kb + 64 < K) load_block<VEC>(nxt, A, lda, B, ldb, ia, jb, M, N, K, kb + 64, q);
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a[c][t], cur.b[c][t], acc, 0, 0, 0);
+    cur = nxt;
+  }
+  return acc;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(64) void k_gemm_nt(const double *__restrict__ A, int lda, const double *__restrict__ B,
+                                                int ldb, double *__restrict__ C, int ldc, int M, int N, int K,
+                                                double alpha) {
+  const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+  const d4 acc = tile_nt<VEC>(A, lda, B, ldb, i0, j0, M, N, K);
+  const int lane = threadIdx.x, col = j0 + (lane & 15);
+  if (col >= N) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int row = i0 + 4 * j + (lane >> 4);
+    if (row < M) C[size_t(row) * ldc + col] = alpha * acc[j];
+  }
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline void tile_of(int t, int T, int &ti, int &tj) {
+  // t enumerates the pairs ti <= tj row by row: row ti holds T - ti tiles
+  ti = 0;
+  while (t >= T - ti) {
+    t -= T - ti;
+    ++ti;
+  }
+  tj = ti + t;
+}
+
+// Gershgorin bounds of the symmetric n x n F (ld n): scal[0] = min_i (F_ii - R_i), scal[1] = max_i (F_ii + R_i).
+__global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal) {
+  __shared__ double lo_s[4], hi_s[4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double lo = 1e300, hi = -1e300;
+  for (int i = w; i < n; i += 4) {
+    double rsum = 0.0;
+    for (int j = lane; j < n; j += 64) rsum += (j == i) ? 0.0 : fabs(F[size_t(i) * n + j]);
+    rsum = wave_sum(rsum);
+    const double d = F[size_t(i) * n + i];
+    lo = fmin(lo, d - rsum);
+    hi = fmax(hi, d + rsum);
+  }
+  if (lane == 0) lo_s[w] = lo, hi_s[w] = hi;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    scal[0] = fmin(fmin(lo_s[0], lo_s[1]), fmin(lo_s[2], lo_s[3]));
+    scal[1] = fmax(fmax(hi_s[0], hi_s[1]), fmax(hi_s[2], hi_s[3]));
+  }
+}
+
+// Writes the partial traces of the tile this wave holds (vals[j] at rows i0+4j+q, column j0+r).
+__device__ inline void tile_partials(const double v[4], int ti, int tj, int i0, int j0, double2 *__restrict__ part, int t) {
+  const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (ti == tj && i0 + 4 * j + q == j0 + r) s1 += v[j];
+    s2 += v[j] * v[j];
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (lane == 0) part[t] = make_double2(s1, ti == tj ? s2 : 2.0 * s2);
+}
+
+// X_0 on the m x m padded grid (zero outside n x n), its partials, and the stop state.
+__global__ __launch_bounds__(64) void k_sp2_init(const double *__restrict__ F, int n, int m, const double *__restrict__ scal,
+                                                 double *__restrict__ X, double2 *__restrict__ part, int *__restrict__ state) {
+  const int T = m / 16;
+  int ti, tj;
+  tile_of(blockIdx.x, T, ti, tj);
+  const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  const double emin = scal[0], emax = scal[1];
+  const double inv = 1.0 / fmax(emax - emin, 1e-10 * fmax(1.0, fabs(emax)));
+  double v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int row = i0 + 4 * j + q, col = j0 + r;
+    double x = 0.0;
+    if (row < n && col < n) x = ((row == col ? emax : 0.0) - F[size_t(row) * n + col]) * inv;
+    v[j] = x;  // the mirror gets the same value: X_0 is exactly symmetric
+    X[size_t(row) * m + col] = x;
+    if (ti != tj) X[size_t(col) * m + row] = x;
+  }
+  tile_partials(v, ti, tj, i0, j0, part, blockIdx.x);
+  if (blockIdx.x == 0 && lane == 0) state[0] = -1, state[1] = 0;
+}
+
+// One SP2 step: X_out from X_in (m x m, symmetric), partials and state from [p] to [p^1].
+// state = {count, steps}: count < 0 before idem < tol was first seen, then the number of steps taken
+// since; count >= 2 means converged (X_out = X_in from then on).
+__global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin, double *__restrict__ Xout, int m, int nocc,
+                                                 const double2 *__restrict__ pin, double2 *__restrict__ pout,
+                                                 const int *__restrict__ sin, int *__restrict__ sout, double tol) {
+  const int T = m / 16, ntiles = T * (T + 1) / 2;
+  int ti, tj;
+  tile_of(blockIdx.x, T, ti, tj);
+  const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  // traces of X_in, summed in the same order by every workgroup
+  double t1 = 0.0, t2 = 0.0;
+  for (int u = lane; u < ntiles; u += 64) t1 += pin[u].x, t2 += pin[u].y;
+  t1 = wave_sum(t1);
+  t2 = wave_sum(t2);
+  int count = sin[0];
+  const int steps = sin[1];
+  const double N = double(nocc);
+  const double idem = t1 - t2;
+  if (count < 0 && idem < tol) count = 0;
+  const bool stop = count >= 2;
+  if (count >= 0 && !stop) ++count;
+  if (blockIdx.x == 0 && lane == 0) sout[0] = count, sout[1] = stop ? steps : steps + 1;
+  double v[4];
+  if (stop) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
+  } else {
+    const bool square = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
+    const d4 acc = tile_nt<true>(Xin, m, Xin, m, i0, j0, m, m, m);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double x = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
+      v[j] = square ? acc[j] : 2.0 * x - acc[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int row = i0 + 4 * j + q, col = j0 + r;
+    Xout[size_t(row) * m + col] = v[j];
+    if (ti != tj) Xout[size_t(col) * m + row] = v[j];
+  }
+  tile_partials(v, ti, tj, i0, j0, pout, blockIdx.x);
+}
+
+}  // namespace
+
+int zq_gemm_nt(zq_ctx *ctx, const double *A, int lda, const double *B, int ldb, double *C, int ldc, int M, int N, int K,
+               double alpha, const char *name) {
+  if (M <= 0 || N <= 0 || K <= 0) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_gemm_nt: empty product");
+  if (lda < K || ldb < K || ldc < N) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_gemm_nt: leading dimension too small");
+  const bool vec = K % 16 == 0 && lda % 2 == 0 && ldb % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+                   (reinterpret_cast<uintptr_t>(B) & 15) == 0;
+  const dim3 grid((N + 15) / 16, (M + 15) / 16);
+  zq_prof_begin(ctx, name);
+  if (vec)
+    hipLaunchKernelGGL(k_gemm_nt<true>, grid, dim3(64), 0, ctx->stream, A, lda, B, ldb, C, ldc, M, N, K, alpha);
+  else
+    hipLaunchKernelGGL(k_gemm_nt<false>, grid, dim3(64), 0, ctx->stream, A, lda, B, ldb, C, ldc, M, N, K, alpha);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, "k_gemm_nt");
+  return ZQ_OK;
+}
+
+int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int max_steps, int *steps_hint,
+                   const double **dD, int *ld, int *steps) {
+  const int m = (n + 15) / 16 * 16, T = m / 16, ntiles = T * (T + 1) / 2;
+  const size_t mm = size_t(m) * m;
+  auto *X = (double *)zq_scratch(ctx, "sp2_X", 2 * mm * sizeof(double));
+  auto *part = (double2 *)zq_scratch(ctx, "sp2_part", 2 * size_t(ntiles) * sizeof(double2));
+  auto *scal = (double *)zq_scratch(ctx, "sp2_scal", 2 * sizeof(double));
+  auto *state = (int *)zq_scratch(ctx, "sp2_state", 4 * sizeof(int));
+  if (!X || !part || !scal || !state) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  zq_prof_begin(ctx, "scf_sp2");
+  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, ctx->stream, dF, n, scal);
+  hipLaunchKernelGGL(k_sp2_init, dim3(ntiles), dim3(64), 0, ctx->stream, dF, n, m, scal, X, part, state);
+  int launched = 0, h[2] = {0, 0};
+  int next_check = std::max(4, std::min(max_steps, *steps_hint + 3));
+  while (true) {
+    while (launched < next_check) {
+      const int p = launched & 1;
+      hipLaunchKernelGGL(k_sp2_step, dim3(ntiles), dim3(64), 0, ctx->stream, X + p * mm, X + (p ^ 1) * mm, m, nocc,
+                         part + p * ntiles, part + (p ^ 1) * ntiles, state + 2 * p, state + 2 * (p ^ 1), tol);
+      ++launched;
+    }
+    ZQ_LAUNCH_CHECK(ctx, "k_sp2_step");
+    ZQ_TRY(ctx, hipMemcpyAsync(h, state + 2 * (launched & 1), 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (h[0] >= 2 || launched >= max_steps) break;
+    next_check = std::min(max_steps, launched + 4);
+  }
+  zq_prof_end(ctx);
+  *dD = X + (launched & 1) * mm;
+  *ld = m;
+  *steps = h[1];
+  if (h[0] < 2) return ZQ_ERR_NOT_CONVERGED;
+  *steps_hint = h[1];
+  return ZQ_OK;
+}

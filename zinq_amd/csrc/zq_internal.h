@@ -99,6 +99,15 @@ void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF);  // zq_jk.hip
 int zq_dgemm(zq_ctx *ctx, const double *A, long long sAm, long long sAk, const double *B, long long sBk, long long sBn,
              double *C, long long sCm, long long sCn, int M, int N, int K, const char *name);  // zq_transform.hip
+// C = alpha A B^T (A: M x K, B: N x K, row-major with leading dimensions), latency-sized FP64 MFMA GEMM.
+int zq_gemm_nt(zq_ctx *ctx, const double *A, int lda, const double *B, int ldb, double *C, int ldc, int M, int N, int K,
+               double alpha, const char *name);  // zq_dense.hip
+// SP2 purification of the symmetric n x n dF (ld n): *dD = the projector onto its nocc lowest
+// eigenvectors (m x m, m = n rounded up to 16, zero padded, *ld = m, in context scratch).  ZQ_OK, or
+// ZQ_ERR_NOT_CONVERGED when max_steps did not reach idempotency (no gap); *steps_hint is the number of
+// steps before the first convergence check and is updated on success.
+int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int max_steps, int *steps_hint,
+                   const double **dD, int *ld, int *steps);  // zq_dense.hip
 
 #define ZQ_TRY(ctx, call)                                \
   do {                                                   \

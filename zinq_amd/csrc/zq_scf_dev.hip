@@ -7,11 +7,16 @@
 //     B row  = <err_new, err_i>                   deterministic reductions; the <= 9 x 9 DIIS system
 //                                                 is solved on the host (diis, :176-227) and the
 //     F_ext  = sum c_i F_i, symmetrised           combination runs on the GPU
-//     F' C'  = C' e with F' = C_prev^T F_ext C_prev   rocSOLVER divide and conquer (syevd), in the
-//                                                 previous MO basis (F' nearly diagonal near convergence);
-//     C      = C_prev C'                          C_prev^T S C_prev = I, hence F C = S C e (geigh)
-//     P      = f C_occ C_occ^T, rms(dP)           (getDensity, :621-640)
-//   The core guess uses C_prev = L^-T from the Cholesky factor S = L L^T.
+//     P      = f X D X^T, D = theta(mu - F')      F' = X^T F_ext X in the orthonormal basis X = L^-T
+//                                                 (S = L L^T); D, the projector onto the nocc lowest
+//                                                 eigenvectors of F', by SP2 purification (GEMMs only).
+//                                                 This is the density of getDensity (:621-640) after the
+//                                                 aufbau diagonalisation of geigh; without a HOMO-LUMO
+//                                                 gap SP2 does not settle and that iteration falls back
+//                                                 to rocSOLVER dsyevd.
+//     rms(dP)
+//   After the loop one dsyevd of the last diagonalised Fock matrix gives C and e (F C = S C e), and
+//   P is rebuilt from C, as the reference returns them.
 // Convergence test and DIIS bookkeeping follow the reference exactly (history <= diis, reset on a
 // singular DIIS matrix, |dE| and rms(dP) below threshold with i > 0).
 
@@ -125,13 +130,35 @@ __global__ __launch_bounds__(256) void k_density(const double *__restrict__ C, i
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// P[i][j] (padded npad stride) = Pn[i][j]; part = sum (Pn - P_old)^2.
+__global__ __launch_bounds__(256) void k_store_density(const double *__restrict__ Pn, int n, double *__restrict__ P, int npad,
+                                                       double *__restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (size_t e = blockIdx.x * 256 + threadIdx.x; e < size_t(n) * n; e += size_t(gridDim.x) * 256) {
+    const int i = int(e / n), j = int(e % n);
+    const double v = Pn[e];
+    const double d = v - P[size_t(i) * npad + j];
+    s += d * d;
+    P[size_t(i) * npad + j] = v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 struct Dev {
   zq_ctx *ctx;
   int n;
   size_t nn;
-  double *S, *X, *Cprev, *C, *T, *Fp, *W, *FP, *FPS, *err, *Fh, *Eh, *part, *scal, *coef, *eig, *offd;
-  int *slots, *info, *nsw;
-  double *resid;
+  double *S, *X, *XT, *C, *T, *Fp, *W, *FP, *FPS, *Pn, *Fh, *Eh, *part, *scal, *coef, *eig, *offd;
+  int *slots, *info;
+  int sp2_hint = 24;  // SP2 steps of the previous density, the first convergence check comes after it
+  int sp2_steps = 0, sp2_calls = 0, sp2_fallbacks = 0;
 };
 
 int alloc(zq_ctx *ctx, Dev &d, int n, int maxh) {
@@ -141,20 +168,19 @@ int alloc(zq_ctx *ctx, Dev &d, int n, int maxh) {
     const char *name;
     double **p;
     size_t cnt;
-  } bufs[] = {{"scf_S", &d.S, nn},       {"scf_X", &d.X, nn},     {"scf_Cp", &d.Cprev, nn}, {"scf_C", &d.C, nn},
-              {"scf_T", &d.T, nn},       {"scf_Fp", &d.Fp, nn},   {"scf_W", &d.W, nn},      {"scf_FP", &d.FP, nn},
-              {"scf_FPS", &d.FPS, nn},   {"scf_err", &d.err, nn}, {"scf_Fh", &d.Fh, nn * maxh},
+  } bufs[] = {{"scf_S", &d.S, nn},       {"scf_X", &d.X, nn},     {"scf_XT", &d.XT, nn},   {"scf_C", &d.C, nn},
+              {"scf_T", &d.T, nn},       {"scf_Fp", &d.Fp, nn},   {"scf_W", &d.W, nn},     {"scf_FP", &d.FP, nn},
+              {"scf_FPS", &d.FPS, nn},   {"scf_Pn", &d.Pn, nn},   {"scf_Fh", &d.Fh, nn * maxh},
               {"scf_Eh", &d.Eh, nn * maxh}, {"scf_part", &d.part, size_t(kRedBlocks) * 16},
-              {"scf_scal", &d.scal, 16}, {"scf_coef", &d.coef, 16}, {"scf_eig", &d.eig, size_t(n)}, {"scf_offd", &d.offd, size_t(n)},
-              {"scf_resid", &d.resid, 4}};
+              {"scf_scal", &d.scal, 16}, {"scf_coef", &d.coef, 16}, {"scf_eig", &d.eig, size_t(n)},
+              {"scf_offd", &d.offd, size_t(n)}};
   for (auto &b : bufs) {
     *b.p = (double *)zq_scratch(ctx, b.name, b.cnt * sizeof(double));
     if (!*b.p) return zq_fail(ctx, ZQ_ERR_OOM, std::string("device SCF buffer ") + b.name);
   }
   d.slots = (int *)zq_scratch(ctx, "scf_slots", 64 * sizeof(int));
   d.info = (int *)zq_scratch(ctx, "scf_info", 4 * sizeof(int));
-  d.nsw = (int *)zq_scratch(ctx, "scf_nsw", 4 * sizeof(int));
-  if (!d.slots || !d.info || !d.nsw) return zq_fail(ctx, ZQ_ERR_OOM, "device SCF buffers");
+  if (!d.slots || !d.info) return zq_fail(ctx, ZQ_ERR_OOM, "device SCF buffers");
   return ZQ_OK;
 }
 
@@ -168,13 +194,18 @@ rocblas_handle solver(zq_ctx *ctx) {
   return (rocblas_handle)ctx->solver;
 }
 
-// Fp = Cb^T A Cb (n x n), then eigen-decomposition of Fp (symmetric, column-major == row-major),
-// C = Cb V.  Eigenvalues ascending in d.eig.
-int diag_in_basis(Dev &d, const double *A, const double *Cb, double *Cout) {
+// Fp = X^T A X for the symmetric n x n A (X^T stored as XT, so both products are A B^T forms).
+int to_orthonormal(Dev &d, const double *A) {
+  const int n = d.n;
+  if (int rc = zq_gemm_nt(d.ctx, d.XT, n, A, n, d.T, n, n, n, n, 1.0, "scf_gemm")) return rc;  // T = X^T A
+  return zq_gemm_nt(d.ctx, d.T, n, d.XT, n, d.Fp, n, n, n, n, 1.0, "scf_gemm");                 // Fp = T X
+}
+
+// Eigen-decomposition of Fp (symmetric, column-major == row-major) by rocSOLVER dsyevd, C = X V,
+// eigenvalues ascending in d.eig.  Fp is overwritten by the eigenvectors.
+int diag_orthonormal(Dev &d, double *Cout) {
   zq_ctx *ctx = d.ctx;
   const int n = d.n;
-  if (int rc = zq_dgemm(ctx, Cb, 1, n, A, n, 1, d.T, n, 1, n, n, n, "scf_gemm")) return rc;  // T = Cb^T A
-  if (int rc = zq_dgemm(ctx, d.T, n, 1, Cb, n, 1, d.Fp, n, 1, n, n, n, "scf_gemm")) return rc;  // Fp = T Cb
   rocblas_handle h = solver(ctx);
   if (!h) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocblas handle");
   zq_prof_begin(ctx, "scf_eig");
@@ -182,8 +213,32 @@ int diag_in_basis(Dev &d, const double *A, const double *Cb, double *Cout) {
       rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, d.Fp, n, d.eig, d.offd, d.info);
   zq_prof_end(ctx);
   if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dsyevd");
-  // eigenvectors are the columns of the column-major Fp: V[k][p] = Fp[p*n + k]
-  return zq_dgemm(ctx, Cb, n, 1, d.Fp, 1, n, Cout, n, 1, n, n, n, "scf_gemm");
+  // eigenvector k is row k of the column-major result: C[i][k] = sum_p X[i][p] V[k][p]
+  return zq_gemm_nt(ctx, d.X, n, d.Fp, n, Cout, n, n, n, n, 1.0, "scf_gemm");
+}
+
+// New density from the symmetric Fock matrix A into ctx->d_P, with part[0..kRedBlocks) = sum dP^2.
+int new_density(Dev &d, const double *A, int nocc, double factor) {
+  zq_ctx *ctx = d.ctx;
+  const int n = d.n, np = ctx->npad;
+  if (int rc = to_orthonormal(d, A)) return rc;
+  const double *D = nullptr;
+  int ld = 0, steps = 0;
+  const int rc = zq_sp2_density(ctx, d.Fp, n, nocc, 1e-9, 100, &d.sp2_hint, &D, &ld, &steps);
+  ++d.sp2_calls;
+  d.sp2_steps += steps;
+  if (rc == ZQ_OK) {
+    if (int r2 = zq_gemm_nt(ctx, d.X, n, D, ld, d.T, n, n, n, n, 1.0, "scf_gemm")) return r2;        // T = X D
+    if (int r2 = zq_gemm_nt(ctx, d.T, n, d.X, n, d.Pn, n, n, n, n, factor, "scf_gemm")) return r2;  // f T X^T
+    hipLaunchKernelGGL(k_store_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.Pn, n, ctx->d_P, np, d.part);
+    return ZQ_OK;
+  }
+  if (rc != ZQ_ERR_NOT_CONVERGED) return rc;
+  // no gap at the Fermi level within 100 SP2 steps: aufbau occupation of the diagonalised F'
+  ++d.sp2_fallbacks;
+  if (int r2 = diag_orthonormal(d, d.C)) return r2;
+  hipLaunchKernelGGL(k_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.C, n, nocc, factor, ctx->d_P, np, d.part);
+  return ZQ_OK;
 }
 
 int sync_read(zq_ctx *ctx, const double *dsrc, double *host, size_t cnt) {
@@ -218,17 +273,19 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   const int gblocks = int(std::min<size_t>(1024, (nn + 255) / 256));
 
   // X = L^-T (S = L L^T), the orthonormal starting basis of the core guess
-  std::vector<double> hX(nn);
+  std::vector<double> hX(nn), hXT(nn);
   if (zq::chol_inv_t(n, S, hX.data())) return zq_fail(ctx, ZQ_ERR_LAPACK, "overlap matrix not positive definite");
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) hXT[size_t(j) * n + i] = hX[size_t(i) * n + j];
   ZQ_TRY(ctx, hipMemcpyAsync(d.S, S, nn * 8, hipMemcpyHostToDevice, ctx->stream));
   ZQ_TRY(ctx, hipMemcpyAsync(d.X, hX.data(), nn * 8, hipMemcpyHostToDevice, ctx->stream));
+  ZQ_TRY(ctx, hipMemcpyAsync(d.XT, hXT.data(), nn * 8, hipMemcpyHostToDevice, ctx->stream));
   ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_H, H, nn * 8, hipMemcpyHostToDevice, ctx->stream));
   ZQ_TRY(ctx, hipMemsetAsync(ctx->d_P, 0, size_t(np) * np * 8, ctx->stream));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
   // core-Hamiltonian guess, src/hartree_fock.zig:425-431
-  if (int rc = diag_in_basis(d, ctx->d_H, d.X, d.C)) return rc;
-  hipLaunchKernelGGL(k_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.C, n, nocc, factor, ctx->d_P, np, d.part);
+  if (int rc = new_density(d, ctx->d_H, nocc, factor)) return rc;
 
   std::deque<int> hist;  // ring-buffer slots of the DIIS history, oldest first
   std::vector<double> Bfull(64 * 64, 0.0);  // B[slot][slot]
@@ -245,8 +302,9 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
     double hbuf[16];
     if (opt->diis > 0) {
       // err = FPS - (FPS)^T into the next history slot; F copied alongside
-      if (int rc = zq_dgemm(ctx, ctx->d_F, n, 1, ctx->d_P, np, 1, d.FP, n, 1, n, n, n, "scf_gemm")) return rc;
-      if (int rc = zq_dgemm(ctx, d.FP, n, 1, d.S, n, 1, d.FPS, n, 1, n, n, n, "scf_gemm")) return rc;
+      // F P (P symmetric) and (F P) S (S symmetric) as A B^T products
+      if (int rc = zq_gemm_nt(ctx, ctx->d_F, n, ctx->d_P, np, d.FP, n, n, n, n, 1.0, "scf_gemm")) return rc;
+      if (int rc = zq_gemm_nt(ctx, d.FP, n, d.S, n, d.FPS, n, n, n, n, 1.0, "scf_gemm")) return rc;
       int slot;
       if (int(hist.size()) >= opt->diis) {
         slot = hist.front();
@@ -292,11 +350,9 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
       if (int rc = sync_read(ctx, d.scal, hbuf, 1)) return rc;
       e_new = hbuf[0] + VN;
     }
-    // diagonalise in the previous MO basis: C = C_prev eig(C_prev^T F C_prev)
+    // new density from the (extrapolated) Fock matrix
     Fout = Fdiag;
-    ZQ_TRY(ctx, hipMemcpyAsync(d.Cprev, d.C, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    if (int rc = diag_in_basis(d, Fdiag, d.Cprev, d.C)) return rc;
-    hipLaunchKernelGGL(k_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.C, n, nocc, factor, ctx->d_P, np, d.part);
+    if (int rc = new_density(d, Fdiag, nocc, factor)) return rc;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, ctx->stream, d.part, 1, d.scal);
     if (int rc = sync_read(ctx, d.scal, hbuf, 1)) return rc;
     const double delta_energy = std::fabs(e_new - e_old);
@@ -308,7 +364,10 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
       break;
     }
   }
-  // outputs: C, e, P, and the diagonalised F
+  // outputs: C and e from the last diagonalised F, P rebuilt from C, and that F
+  if (int rc = to_orthonormal(d, Fout)) return rc;
+  if (int rc = diag_orthonormal(d, d.C)) return rc;
+  hipLaunchKernelGGL(k_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.C, n, nocc, factor, ctx->d_P, np, d.part);
   std::vector<double> hP(size_t(np) * np);
   ZQ_TRY(ctx, hipMemcpyAsync(C, d.C, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipMemcpyAsync(e, d.eig, size_t(n) * 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -58,23 +58,41 @@ def prepare(ks, basis):
     return mols
 
 
-def run_one(m, precision, device_scf):
-    r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=1e-10, device=device_scf)
-    e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], precision)
+# The plain FP64 form (the reference's arithmetic, SCF to 1e-10) and the mixed path (incremental Fock
+# builds from FP32 ERIs to 1e-6, then one full FP64 build; FP32 products with FP64 block sums in the
+# transform).  MP2 denominators and sums are FP64 in both.
+PLAIN = dict(fock="fp64", transform="fp64", threshold=1e-10)
+MIXED = dict(fock="fp32", transform="fp32c", threshold=1e-6)
+
+
+def run_one(m, mode, device_scf):
+    r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=mode["threshold"],
+                     device=device_scf, fock_precision=mode["fock"] if device_scf else "fp64")
+    e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], mode["transform"])
     return dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"])
 
 
-def step(mols, precision, results=None, device_scf=True, concurrent=True):
+def step(mols, mode, results=None, device_scf=True, concurrent=True):
     """All molecules of the step; with concurrent=True each runs in its own host thread on its own
     context/stream (the C calls release the GIL), so their GPU work overlaps."""
     if concurrent:
         from concurrent.futures import ThreadPoolExecutor
         with ThreadPoolExecutor(len(mols)) as ex:
-            out = list(ex.map(lambda m: run_one(m, precision, device_scf), mols))
+            out = list(ex.map(lambda m: run_one(m, mode, device_scf), mols))
     else:
-        out = [run_one(m, precision, device_scf) for m in mols]
+        out = [run_one(m, mode, device_scf) for m in mols]
     if results is not None:
         results.extend(out)
+
+
+def timed(mols, mode, args, dev, conc):
+    for _ in range(args.warmup):
+        step(mols, mode, device_scf=dev, concurrent=conc)
+    results = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(mols, mode, results if i == 0 else None, device_scf=dev, concurrent=conc)
+    return time.perf_counter() - t0, results
 
 
 def cpu_baseline(mols, seconds_budget=30.0):
@@ -109,7 +127,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ks", default="4,6,8")
     ap.add_argument("--basis", default="builtin:cc-pvdz")
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="fp64", help="transform precision of the plain line")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-path pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scf", default="device", choices=["device", "host"])
     ap.add_argument("--serial", action="store_true", help="run the molecules of a step one after another")
@@ -122,14 +141,20 @@ def main():
 
     dev = args.scf == "device"
     conc = not args.serial
-    for _ in range(args.warmup):
-        step(mols, args.precision, device_scf=dev, concurrent=conc)
-    results = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(mols, args.precision, results if i == 0 else None, device_scf=dev, concurrent=conc)
-    elapsed = time.perf_counter() - t0
+    plain = dict(PLAIN, transform=args.precision)
+    elapsed, results = timed(mols, plain, args, dev, conc)
     ms_per_step = 1e3 * elapsed / args.steps
+    mixed = None
+    if not args.no_mixed:
+        el_m, res_m = timed(mols, MIXED, args, dev, conc)
+        de = [abs(a["E_mp2"] - b["E_mp2"]) for a, b in zip(res_m, results)]
+        mixed = dict(value=round(len(mols) * args.steps / el_m, 3), unit="molecules/s",
+                     speedup_vs_fp64=round(elapsed / el_m, 3), ms_per_step=round(1e3 * el_m / args.steps, 2),
+                     max_abs_dE_mp2_vs_fp64=max(de), bar=1e-6, within_bar=bool(max(de) <= 1e-6),
+                     config=dict(MIXED, note="Fock: incremental builds from FP32 ERIs (FP64 products/sums) to "
+                                             "the threshold, then one full FP64 build; transform: FP32 MFMA "
+                                             "products, FP64 sums per K block"),
+                     results=res_m)
     for r in results:
         for m in mols:
             if m["k"] == r["k"]:
@@ -140,12 +165,18 @@ def main():
     for m in mols:
         m["ctx"].profile(True)
         m["ctx"].profile_reset()
-    step(mols, args.precision, device_scf=dev, concurrent=False)
+    wall = {}
+    for m in mols:  # one molecule at a time, so that the wall times are the molecule's own
+        t = time.perf_counter()
+        r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=plain["threshold"], device=dev)
+        t_scf = time.perf_counter() - t
+        m["ctx"].mp2(r["C"], r["e"], m["o"], args.precision)
+        wall[m["k"]] = (t_scf * 1e3, (time.perf_counter() - t - t_scf) * 1e3)
     per_mol = []
     for m in mols:
         c = m["ctx"]
         d = {name: c.profile_get(name) for name in ("jk_records", "jk_reduce", "ao2mo_q1", "ao2mo_q2", "ao2mo_q3",
-                                                     "ao2mo_q4", "mp2_energy", "scf_eig", "scf_gemm")}
+                                                     "ao2mo_q4", "mp2_energy", "scf_eig", "scf_gemm", "scf_sp2")}
         c.profile(False)
         t_tr = sum(d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")) * 1e-3
         jk_ms, jk_cnt = d["jk_records"]
@@ -154,7 +185,8 @@ def main():
                             jk_GBs=jk_bytes(m["n"]) / (jk_ms / max(jk_cnt, 1) * 1e-3) / 1e9,
                             transform_ms=t_tr * 1e3, transform_TFLOPs=ovov_flops(m["n"], m["o"]) / t_tr / 1e12,
                             q_ms={q: d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")},
-                            eig_ms=d["scf_eig"][0], scf_gemm_ms=d["scf_gemm"][0]))
+                            eig_ms=d["scf_eig"][0], scf_gemm_ms=d["scf_gemm"][0], sp2_ms=d["scf_sp2"][0],
+                            sp2_calls=d["scf_sp2"][1], scf_wall_ms=wall[m["k"]][0], mp2_wall_ms=wall[m["k"]][1]))
         for name, (ms, cnt) in d.items():
             kern.setdefault(name, [0.0, 0, 0.0])
             kern[name][0] += ms
@@ -186,7 +218,7 @@ def main():
                            molecules_concurrent=conc),
                transform_TFLOPs={f"k{p['k']}": round(p["transform_TFLOPs"], 2) for p in per_mol},
                jk_GBs={f"k{p['k']}": round(p["jk_GBs"], 1) for p in per_mol},
-               roofline=roof, results=results, per_molecule=per_mol,
+               roofline=roof, results=results, mixed=mixed, per_molecule=per_mol,
                integrals_s={f"k{m['k']}": round(m["t_int"], 2) for m in mols})
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mols)

@@ -38,8 +38,14 @@ enum {
 /* which integrals zq_ao2mo produces */
 enum { ZQ_BLOCK_FULL = 0, ZQ_BLOCK_OVOV = 1 };
 
-/* arithmetic of the quarter transforms; J/K, DIIS, diagonalisation and MP2 sums stay FP64 */
-enum { ZQ_PREC_FP64 = 0, ZQ_PREC_FP32 = 1, ZQ_PREC_BF16X3 = 2 };
+/* arithmetic of the quarter transforms; J/K, DIIS, diagonalisation and MP2 sums stay FP64.
+ *   ZQ_PREC_FP64    FP64 MFMA, FP64 intermediates
+ *   ZQ_PREC_FP32    FP32 MFMA with FP32 accumulation, FP32 intermediates
+ *   ZQ_PREC_BF16X3  operands split into two bf16 (hi + lo), hi*hi + hi*lo + lo*hi on bf16 MFMA with FP32
+ *                   accumulation (the small products in their own accumulator), FP32 intermediates
+ *   ZQ_PREC_FP32C   FP32 MFMA products summed in FP32 over one K block, the blocks summed in FP64
+ *                   (compensated sums), FP32 intermediates: the mixed path's transform */
+enum { ZQ_PREC_FP64 = 0, ZQ_PREC_FP32 = 1, ZQ_PREC_BF16X3 = 2, ZQ_PREC_FP32C = 3 };
 
 typedef struct zq_ctx zq_ctx;
 
@@ -65,7 +71,7 @@ int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double x_exch, dou
  *                  (src/integral_transform.zig:37-79); out has n^4 entries.
  *   ZQ_BLOCK_OVOV: out[i,j,a,b] = <ij|ab> = (ia|jb), i,j < nocc, a,b >= nocc, the result of ao2mo_oovv
  *                  (src/integral_transform.zig:165-219); out has nocc^2 (n-nocc)^2 entries.
- * precision: ZQ_PREC_FP64 | ZQ_PREC_FP32 | ZQ_PREC_BF16X3 (quarter transforms only). */
+ * precision: ZQ_PREC_FP64 | ZQ_PREC_FP32 | ZQ_PREC_BF16X3 | ZQ_PREC_FP32C (quarter transforms only). */
 int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out);
 
 /* Closed-shell MP2 correlation energy from the OVOV transform, FP64 denominators and summation:

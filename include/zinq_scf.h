@@ -23,6 +23,11 @@ typedef struct {
   int iterations;    /* default 100 */
   double threshold;  /* on |dE| and rms(dP), default 1e-8 */
   int generalized;   /* 0: RHF (x = 1/2, P = 2 C C^T); 1: GHF on spin-blocked integrals (x = 1) */
+  int fock_precision; /* zq_scf_run_device only.  ZQ_PREC_FP64: every Fock build in FP64 (the reference's
+                         arithmetic).  ZQ_PREC_FP32: the mixed path -- incremental builds
+                         F_k = F_(k-1) + G(P_k - P_(k-1)) from an FP32 copy of the ERIs (FP64 products
+                         and sums) until convergence, then one full FP64 build from the last density for
+                         the energy and the orbitals.  zq_scf_run always uses FP64. */
 } zq_scf_options;
 
 typedef struct {
@@ -37,12 +42,20 @@ typedef struct {
 int zq_scf_run(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double *S, double VN,
                const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);
 
-/* Same contract as zq_scf_run with every n x n step on the GPU (J/K, energy, DIIS error and
- * extrapolation, density); the diagonalisation is a rocSOLVER dsyevd of F in the previous MO basis
- * (C_prev^T F C_prev, with C_prev^T S C_prev = I), C = C_prev C'.  Only the <= 9 x 9
- * DIIS system and the convergence test run on the host. */
+/* Same contract as zq_scf_run with every n x n step on the GPU.  Each iteration is decided on the
+ * device: J/K, energy, DIIS error, the DIIS history and its bordered LU solve, the extrapolation, and
+ * the density P = f X D X^T (X = L^-T from S = L L^T), where D, the projector onto the nocc lowest
+ * eigenvectors of X^T F X, comes from SP2 purification (GEMMs on MFMA).  The iteration is captured once
+ * as a hipGraph and replayed; the host reads the convergence state one iteration late, so the GPU is
+ * never idle between iterations.  After convergence one rocSOLVER dsyevd of the last extrapolated F
+ * gives C and e.  A density whose SP2 does not settle within the step budget (no HOMO-LUMO gap) makes
+ * the run restart on a host-synchronised driver that falls back to dsyevd for that density. */
 int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double *S, double VN,
                       const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);
+
+/* SP2 steps per density in zq_scf_run_device's captured iteration (1..100, default 48).  A density
+ * that needs more restarts the run on the host-synchronised driver and raises the budget by 16. */
+int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
 
 /* Host generalized symmetric-definite eigensolver used by zq_scf_run (the dsygvd stand-in):
  * A x = w B x; on return A holds the eigenvectors (columns), w ascending; B is overwritten.

@@ -132,10 +132,13 @@ def test_invalid_inputs_raise(ctx):
 # Bars (absolute, on (ia|jb) and on E_MP2).  AO integrals reach ~5 Eh (oxygen core), so each quarter
 # carries absolute rounding of ~5 x unit roundoff x sqrt(terms): FP32 (2^-24) 5e-5 / 1e-6 Eh; BF16X3
 # (hi*hi + hi*lo + lo*hi of a two-term bf16 split, ~2^-16 relative per product) 5e-3 / 5e-5 Eh.
-PREC_TOL = {"fp32": (5e-5, 1e-6), "bf16x3": (5e-3, 5e-5)}
+# FP32C keeps FP32 operands but sums whole K blocks in FP64, so only one block's FP32 rounding (and the
+# FP32 rounding of the operands and intermediates, ~5 x 2^-24 per element) remains: 1e-5 / 2e-7 Eh,
+# a factor 5 inside the mixed path's 1e-6 Eh budget at these sizes.
+PREC_TOL = {"fp32": (5e-5, 1e-6), "bf16x3": (5e-3, 5e-5), "fp32c": (1e-5, 2e-7)}
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp32c"])
 @pytest.mark.parametrize("kind", ["sto3g", "ccpvdz", "w2"])
 def test_reduced_precision_ovov_and_mp2(ctx, kind, prec):
     s, H, S, g, VN = _load(ctx, kind)
@@ -151,7 +154,7 @@ def test_reduced_precision_ovov_and_mp2(ctx, kind, prec):
     assert abs(e2 - e2o) < tol_e
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp32c"])
 def test_reduced_precision_full_transform(ctx, prec):
     s, H, S, g, VN = _load(ctx, "ccpvdz")
     C = np.linalg.qr(np.random.default_rng(9).standard_normal((s.nbf, s.nbf)))[0]
@@ -199,3 +202,55 @@ def test_device_scf_no_diis_and_not_converged(ctx):
     assert r["E"] == pytest.approx(ro["E"], abs=1e-9)
     with pytest.raises(hip.ScfDidNotConverge):
         ctx.scf(H, S, VN, o, iterations=1, device=True)
+
+
+def test_device_scf_graph_replays_match_eager(ctx):
+    """The captured SCF iteration (one hipGraph per context, replayed every iteration and reused by later
+    calls) gives bitwise the same SCF as the eagerly launched kernels (profiling on disables capture),
+    call after call, with eager work in between."""
+    s, H, S, g, VN = _load(ctx, "w2")
+    o = s.nel // 2
+    ctx.profile(True)
+    eager = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    ctx.profile(False)
+    for _ in range(3):
+        r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+        assert r["E"] == eager["E"] and r["iterations"] == eager["iterations"]
+        np.testing.assert_array_equal(r["P"], eager["P"])
+        np.testing.assert_array_equal(r["F"], eager["F"])
+        ctx.fock_jk(H, r["P"])  # eager kernels on the same stream between replays
+
+
+def test_device_scf_sp2_budget_fallback(ctx):
+    """A too small SP2 step budget ends the asynchronous run (state 'SP2 did not settle'); the driver
+    then reruns the host-synchronised SCF and the result is unchanged."""
+    from zinq_amd import hip
+    s, H, S, g, VN = _load(ctx, "ccpvdz")
+    o = s.nel // 2
+    ref = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    ctx.set_sp2_budget(4)
+    r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    ctx.set_sp2_budget(48)
+    assert r["E"] == pytest.approx(ref["E"], abs=1e-11)
+    assert r["iterations"] == ref["iterations"]
+    with pytest.raises(hip.InvalidInput):
+        ctx.set_sp2_budget(0)
+
+
+@pytest.mark.parametrize("kind", ["ccpvdz", "w2"])
+def test_mixed_path_within_bar(ctx, kind):
+    """Mixed path: incremental Fock builds from FP32 ERIs to 1e-6, one full FP64 build, FP32C transform.
+    Bar: |dE_MP2| <= 1e-6 Eh against the FP64 oracle (the total MP2 energy and its correlation part)."""
+    s, H, S, g, VN = _load(ctx, kind)
+    o = s.nel // 2
+    ro = O.scf(H, S, g, VN, o, threshold=1e-10)
+    e2o = O.mp2_closed(O.ao2mo_oovv(g, ro["C"], o), ro["e"])
+    r = ctx.scf(H, S, VN, o, threshold=1e-6, device=True, fock_precision="fp32")
+    e2 = ctx.mp2(r["C"], r["e"], o, "fp32c")
+    print(f"{kind} mixed: dE_rhf={r['E'] - ro['E']:+.2e} dE_corr={e2 - e2o:+.2e} it={r['iterations']}")
+    assert abs(r["E"] - ro["E"]) < 1e-7
+    assert abs(e2 - e2o) < 1e-6
+    assert abs(r["E"] + e2 - ro["E"] - e2o) < 1e-6
+    # the FP32 ERI copy stays resident; a plain FP64 SCF afterwards is unaffected
+    r64 = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    assert r64["E"] == pytest.approx(ro["E"], abs=1e-10)

@@ -30,6 +30,7 @@ void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh
   if (b.ptr) (void)hipFree(b.ptr);
   b.ptr = nullptr;
   b.bytes = 0;
+  ++ctx->scratch_gen;
   if (hipMalloc(&b.ptr, std::max<size_t>(bytes, 256)) != hipSuccess) {
     b.ptr = nullptr;
     (void)hipGetLastError();
@@ -37,6 +38,21 @@ void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh
   }
   b.bytes = bytes;
   if (fresh) *fresh = true;
+  return b.ptr;
+}
+
+void *zq_pinned(zq_ctx *ctx, const std::string &name, size_t bytes) {
+  DevBuf &b = ctx->pinned[name];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) (void)hipHostFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+  if (hipHostMalloc(&b.ptr, std::max<size_t>(bytes, 256), hipHostMallocDefault) != hipSuccess) {
+    b.ptr = nullptr;
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  b.bytes = bytes;
   return b.ptr;
 }
 
@@ -72,10 +88,10 @@ static void prof_collect(zq_ctx *ctx) {
 }
 
 void zq_free_eri(zq_ctx *ctx) {
-  for (void *p : {(void *)ctx->eri, (void *)ctx->d_items, (void *)ctx->d_records, (void *)ctx->d_csr_ptr,
-                  (void *)ctx->d_csr_idx, (void *)ctx->d_P, (void *)ctx->d_H, (void *)ctx->d_F})
+  for (void *p : {(void *)ctx->eri, (void *)ctx->eri32, (void *)ctx->d_items, (void *)ctx->d_records,
+                  (void *)ctx->d_csr_ptr, (void *)ctx->d_csr_idx, (void *)ctx->d_P, (void *)ctx->d_H, (void *)ctx->d_F})
     if (p) (void)hipFree(p);
-  ctx->eri = nullptr, ctx->d_items = nullptr, ctx->d_records = nullptr;
+  ctx->eri = nullptr, ctx->eri32 = nullptr, ctx->d_items = nullptr, ctx->d_records = nullptr;
   ctx->d_csr_ptr = nullptr, ctx->d_csr_idx = nullptr;
   ctx->d_P = ctx->d_H = ctx->d_F = nullptr;
   ctx->n = ctx->npad = ctx->nb = ctx->Nb = 0;
@@ -124,6 +140,7 @@ static int load_canonical(zq_ctx *ctx, const double *canon, size_t n) {
   if (!ctx || !canon || n == 0 || n > 4096) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_eri_load: bad arguments");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   zq_free_eri(ctx);
+  ++ctx->eri_gen;
   ctx->n = int(n);
   ctx->nb = int((n + ZQ_BLK - 1) / ZQ_BLK);
   ctx->npad = ctx->nb * ZQ_BLK;
@@ -173,6 +190,8 @@ void zq_destroy(zq_ctx *ctx) {
   zq_free_eri(ctx);
   for (auto &kv : ctx->scratch)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto &kv : ctx->pinned)
+    if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -236,6 +255,7 @@ int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches) {
 size_t zq_device_bytes(const zq_ctx *ctx) {
   if (!ctx) return 0;
   size_t b = ctx->nquartets * ZQ_QUARTET * sizeof(double) + ctx->nrecords * 64 * sizeof(double);
+  if (ctx->eri32) b += ctx->nquartets * ZQ_QUARTET * sizeof(float);
   for (auto &kv : ctx->scratch) b += kv.second.bytes;
   return b;
 }

@@ -92,7 +92,10 @@ __device__ inline d4 tile_nt(const double *__restrict__ A, int lda, const double
 template <bool VEC>
 __global__ __launch_bounds__(64) void k_gemm_nt(const double *__restrict__ A, int lda, const double *__restrict__ B,
                                                 int ldb, double *__restrict__ C, int ldc, int M, int N, int K,
-                                                double alpha) {
+                                                double alpha, const int *__restrict__ skip, const int *__restrict__ bsel,
+                                                long long bstride) {
+  if (skip && *skip) return;
+  if (bsel) B += *bsel * bstride;
   const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
   const d4 acc = tile_nt<VEC>(A, lda, B, ldb, i0, j0, M, N, K);
   const int lane = threadIdx.x, col = j0 + (lane & 15);
@@ -121,23 +124,36 @@ __device__ inline void tile_of(int t, int T, int &ti, int &tj) {
 }
 
 // Gershgorin bounds of the symmetric n x n F (ld n): scal[0] = min_i (F_ii - R_i), scal[1] = max_i (F_ii + R_i).
-__global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal) {
-  __shared__ double lo_s[4], hi_s[4];
+// One thread per row; F is symmetric, so thread i sums column i, and the loads of a wave are coalesced.
+__global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal,
+                                                     const int *__restrict__ skip) {
+  __shared__ double lo_s[16], hi_s[16];
+  if (skip && *skip) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double lo = 1e300, hi = -1e300;
-  for (int i = w; i < n; i += 4) {
+  for (int i = threadIdx.x; i < n; i += 1024) {
     double rsum = 0.0;
-    for (int j = lane; j < n; j += 64) rsum += (j == i) ? 0.0 : fabs(F[size_t(i) * n + j]);
-    rsum = wave_sum(rsum);
+#pragma unroll 8
+    for (int j = 0; j < n; ++j) rsum += fabs(F[size_t(j) * n + i]);
     const double d = F[size_t(i) * n + i];
+    rsum -= fabs(d);
     lo = fmin(lo, d - rsum);
     hi = fmax(hi, d + rsum);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
   }
   if (lane == 0) lo_s[w] = lo, hi_s[w] = hi;
   __syncthreads();
   if (threadIdx.x == 0) {
-    scal[0] = fmin(fmin(lo_s[0], lo_s[1]), fmin(lo_s[2], lo_s[3]));
-    scal[1] = fmax(fmax(hi_s[0], hi_s[1]), fmax(hi_s[2], hi_s[3]));
+    const int nw = int(blockDim.x) >> 6;
+    for (int k = 1; k < nw; ++k) lo_s[0] = fmin(lo_s[0], lo_s[k]), hi_s[0] = fmax(hi_s[0], hi_s[k]);
+    // widen by a relative margin: the row sums above carry rounding errors
+    const double pad = 1e-12 * fmax(fabs(lo_s[0]), fabs(hi_s[0]));
+    scal[0] = lo_s[0] - pad;
+    scal[1] = hi_s[0] + pad;
   }
 }
 
@@ -157,7 +173,9 @@ __device__ inline void tile_partials(const double v[4], int ti, int tj, int i0, 
 
 // X_0 on the m x m padded grid (zero outside n x n), its partials, and the stop state.
 __global__ __launch_bounds__(64) void k_sp2_init(const double *__restrict__ F, int n, int m, const double *__restrict__ scal,
-                                                 double *__restrict__ X, double2 *__restrict__ part, int *__restrict__ state) {
+                                                 double *__restrict__ X, double2 *__restrict__ part, int *__restrict__ state,
+                                                 const int *__restrict__ skip) {
+  if (skip && *skip) return;
   const int T = m / 16;
   int ti, tj;
   tile_of(blockIdx.x, T, ti, tj);
@@ -175,45 +193,67 @@ __global__ __launch_bounds__(64) void k_sp2_init(const double *__restrict__ F, i
     if (ti != tj) X[size_t(col) * m + row] = x;
   }
   tile_partials(v, ti, tj, i0, j0, part, blockIdx.x);
-  if (blockIdx.x == 0 && lane == 0) state[0] = -1, state[1] = 0;
+  if (blockIdx.x == 0 && lane == 0) state[0] = -1, state[1] = 0, state[4] = 0;
 }
 
 // One SP2 step: X_out from X_in (m x m, symmetric), partials and state from [p] to [p^1].
 // state = {count, steps}: count < 0 before idem < tol was first seen, then the number of steps taken
-// since; count >= 2 means converged (X_out = X_in from then on).
+// since; count >= 2 means settled.  The step that brings count to 2 stores the parity of its output
+// buffer in *final; later steps only carry the state forward and return.
+// For m <= 256 (PRE) every operand of the step is loaded before the first use, so the step costs one
+// memory round trip; larger m walk K with the double-buffered tile_nt.
+template <bool PRE>
 __global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin, double *__restrict__ Xout, int m, int nocc,
                                                  const double2 *__restrict__ pin, double2 *__restrict__ pout,
-                                                 const int *__restrict__ sin, int *__restrict__ sout, double tol) {
+                                                 const int *__restrict__ sin, int *__restrict__ sout, int *__restrict__ final,
+                                                 int pout_parity, double tol, const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  if (sin[0] >= 2) {  // settled: carry the state forward
+    if (blockIdx.x == 0 && threadIdx.x == 0) sout[0] = sin[0], sout[1] = sin[1];
+    return;
+  }
   const int T = m / 16, ntiles = T * (T + 1) / 2;
   int ti, tj;
   tile_of(blockIdx.x, T, ti, tj);
   const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x, r = lane & 15, q = lane >> 4;
-  // traces of X_in, summed in the same order by every workgroup
+  // issue every load first: partials, state, this lane's X elements, the operand rows
   double t1 = 0.0, t2 = 0.0;
   for (int u = lane; u < ntiles; u += 64) t1 += pin[u].x, t2 += pin[u].y;
-  t1 = wave_sum(t1);
-  t2 = wave_sum(t2);
   int count = sin[0];
   const int steps = sin[1];
+  double x[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) x[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
+  Frag f[PRE ? 4 : 1];
+  if (PRE) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) load_block<true>(f[b], Xin, m, Xin, m, i0 + r, j0 + r, m, m, m, 64 * b, q);
+  }
+  t1 = wave_sum(t1);
+  t2 = wave_sum(t2);
   const double N = double(nocc);
   const double idem = t1 - t2;
   if (count < 0 && idem < tol) count = 0;
-  const bool stop = count >= 2;
-  if (count >= 0 && !stop) ++count;
-  if (blockIdx.x == 0 && lane == 0) sout[0] = count, sout[1] = stop ? steps : steps + 1;
-  double v[4];
-  if (stop) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) v[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
-  } else {
-    const bool square = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
-    const d4 acc = tile_nt<true>(Xin, m, Xin, m, i0, j0, m, m, m);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double x = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
-      v[j] = square ? acc[j] : 2.0 * x - acc[j];
-    }
+  if (count >= 0) ++count;
+  if (blockIdx.x == 0 && lane == 0) {
+    sout[0] = count, sout[1] = steps + 1;
+    if (count == 2) *final = pout_parity;  // X_out is the settled projector
   }
+  const bool square = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  if (PRE) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f[b].a[c][t], f[b].b[c][t], acc, 0, 0, 0);
+  } else {
+    acc = tile_nt<true>(Xin, m, Xin, m, i0, j0, m, m, m);
+  }
+  double v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = square ? acc[j] : 2.0 * x[j] - acc[j];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int row = i0 + 4 * j + q, col = j0 + r;
@@ -226,52 +266,100 @@ __global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin,
 }  // namespace
 
 int zq_gemm_nt(zq_ctx *ctx, const double *A, int lda, const double *B, int ldb, double *C, int ldc, int M, int N, int K,
-               double alpha, const char *name) {
+               double alpha, const char *name, const int *skip, const int *bsel, long long bstride) {
   if (M <= 0 || N <= 0 || K <= 0) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_gemm_nt: empty product");
   if (lda < K || ldb < K || ldc < N) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_gemm_nt: leading dimension too small");
-  const bool vec = K % 16 == 0 && lda % 2 == 0 && ldb % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
-                   (reinterpret_cast<uintptr_t>(B) & 15) == 0;
+  const bool vec = K % 16 == 0 && lda % 2 == 0 && ldb % 2 == 0 && bstride % 2 == 0 &&
+                   (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
   const dim3 grid((N + 15) / 16, (M + 15) / 16);
   zq_prof_begin(ctx, name);
   if (vec)
-    hipLaunchKernelGGL(k_gemm_nt<true>, grid, dim3(64), 0, ctx->stream, A, lda, B, ldb, C, ldc, M, N, K, alpha);
+    hipLaunchKernelGGL(k_gemm_nt<true>, grid, dim3(64), 0, ctx->stream, A, lda, B, ldb, C, ldc, M, N, K, alpha, skip, bsel,
+                       bstride);
   else
-    hipLaunchKernelGGL(k_gemm_nt<false>, grid, dim3(64), 0, ctx->stream, A, lda, B, ldb, C, ldc, M, N, K, alpha);
+    hipLaunchKernelGGL(k_gemm_nt<false>, grid, dim3(64), 0, ctx->stream, A, lda, B, ldb, C, ldc, M, N, K, alpha, skip, bsel,
+                       bstride);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "k_gemm_nt");
   return ZQ_OK;
 }
 
+namespace {
+
+struct Sp2Bufs {
+  int m, T, ntiles;
+  size_t mm;
+  double *X, *scal;
+  double2 *part;
+};
+
+int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
+  b.m = (n + 15) / 16 * 16, b.T = b.m / 16, b.ntiles = b.T * (b.T + 1) / 2;
+  b.mm = size_t(b.m) * b.m;
+  b.X = (double *)zq_scratch(ctx, "sp2_X", 2 * b.mm * sizeof(double));
+  b.part = (double2 *)zq_scratch(ctx, "sp2_part", 2 * size_t(b.ntiles) * sizeof(double2));
+  b.scal = (double *)zq_scratch(ctx, "sp2_scal", 2 * sizeof(double));
+  if (!b.X || !b.part || !b.scal) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  return ZQ_OK;
+}
+
+void sp2_start(zq_ctx *ctx, const Sp2Bufs &b, const double *dF, int n, int *state, const int *skip) {
+  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(std::min(1024, (n + 63) / 64 * 64)), 0, ctx->stream, dF, n, b.scal, skip);
+  hipLaunchKernelGGL(k_sp2_init, dim3(b.ntiles), dim3(64), 0, ctx->stream, dF, n, b.m, b.scal, b.X, b.part, state, skip);
+}
+
+void sp2_step(zq_ctx *ctx, const Sp2Bufs &b, int k, int nocc, double tol, int *state, const int *skip) {
+  const int p = k & 1;
+  double *Xi = b.X + p * b.mm, *Xo = b.X + (p ^ 1) * b.mm;
+  const double2 *pi = b.part + p * b.ntiles;
+  double2 *po = b.part + (p ^ 1) * b.ntiles;
+  if (b.m <= 256)
+    hipLaunchKernelGGL(k_sp2_step<true>, dim3(b.ntiles), dim3(64), 0, ctx->stream, Xi, Xo, b.m, nocc, pi, po,
+                       state + 2 * p, state + 2 * (p ^ 1), state + 4, p ^ 1, tol, skip);
+  else
+    hipLaunchKernelGGL(k_sp2_step<false>, dim3(b.ntiles), dim3(64), 0, ctx->stream, Xi, Xo, b.m, nocc, pi, po,
+                       state + 2 * p, state + 2 * (p ^ 1), state + 4, p ^ 1, tol, skip);
+}
+
+}  // namespace
+
+int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state5,
+                   const double **dX, long long *stride, int *ld) {
+  Sp2Bufs b;
+  if (int rc = sp2_bufs(ctx, n, b)) return rc;
+  zq_prof_begin(ctx, "scf_sp2");
+  sp2_start(ctx, b, dF, n, state5, skip);
+  for (int k = 0; k < nsteps; ++k) sp2_step(ctx, b, k, nocc, tol, state5, skip);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, "k_sp2_step");
+  *dX = b.X;
+  *stride = (long long)b.mm;
+  *ld = b.m;
+  return ZQ_OK;
+}
+
 int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int max_steps, int *steps_hint,
                    const double **dD, int *ld, int *steps) {
-  const int m = (n + 15) / 16 * 16, T = m / 16, ntiles = T * (T + 1) / 2;
-  const size_t mm = size_t(m) * m;
-  auto *X = (double *)zq_scratch(ctx, "sp2_X", 2 * mm * sizeof(double));
-  auto *part = (double2 *)zq_scratch(ctx, "sp2_part", 2 * size_t(ntiles) * sizeof(double2));
-  auto *scal = (double *)zq_scratch(ctx, "sp2_scal", 2 * sizeof(double));
-  auto *state = (int *)zq_scratch(ctx, "sp2_state", 4 * sizeof(int));
-  if (!X || !part || !scal || !state) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  Sp2Bufs b;
+  if (int rc = sp2_bufs(ctx, n, b)) return rc;
+  auto *state = (int *)zq_scratch(ctx, "sp2_state", 5 * sizeof(int));
+  if (!state) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
   zq_prof_begin(ctx, "scf_sp2");
-  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(256), 0, ctx->stream, dF, n, scal);
-  hipLaunchKernelGGL(k_sp2_init, dim3(ntiles), dim3(64), 0, ctx->stream, dF, n, m, scal, X, part, state);
-  int launched = 0, h[2] = {0, 0};
+  sp2_start(ctx, b, dF, n, state, nullptr);
+  int launched = 0, h[2] = {0, 0}, fin = 0;
   int next_check = std::max(4, std::min(max_steps, *steps_hint + 3));
   while (true) {
-    while (launched < next_check) {
-      const int p = launched & 1;
-      hipLaunchKernelGGL(k_sp2_step, dim3(ntiles), dim3(64), 0, ctx->stream, X + p * mm, X + (p ^ 1) * mm, m, nocc,
-                         part + p * ntiles, part + (p ^ 1) * ntiles, state + 2 * p, state + 2 * (p ^ 1), tol);
-      ++launched;
-    }
+    while (launched < next_check) sp2_step(ctx, b, launched++, nocc, tol, state, nullptr);
     ZQ_LAUNCH_CHECK(ctx, "k_sp2_step");
     ZQ_TRY(ctx, hipMemcpyAsync(h, state + 2 * (launched & 1), 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (h[0] >= 2 || launched >= max_steps) break;
     next_check = std::min(max_steps, launched + 4);
   }
+  ZQ_TRY(ctx, hipMemcpy(&fin, state + 4, sizeof(int), hipMemcpyDeviceToHost));
   zq_prof_end(ctx);
-  *dD = X + (launched & 1) * mm;
-  *ld = m;
+  *dD = b.X + (fin & 1) * b.mm;
+  *ld = b.m;
   *steps = h[1];
   if (h[0] < 2) return ZQ_ERR_NOT_CONVERGED;
   *steps_hint = h[1];

@@ -63,6 +63,7 @@ struct zq_ctx {
   int n = 0, npad = 0, nb = 0, Nb = 0;
   size_t nquartets = 0;
   double *eri = nullptr;
+  float *eri32 = nullptr;  // FP32 copy for the mixed-precision Fock builds (zq_eri32), same layout
 
   // J/K work decomposition
   JKItem *d_items = nullptr;
@@ -75,8 +76,17 @@ struct zq_ctx {
   // rocBLAS handle used by the device SCF's eigensolver (created on first use)
   void *solver = nullptr;
 
+  // captured SCF iteration (hipGraphExec_t) and the key it was captured for; the generations change
+  // whenever a scratch buffer or the ERI residency is (re)allocated, which invalidates the capture
+  void *scf_graph = nullptr;
+  std::string scf_graph_key;
+  unsigned scratch_gen = 0, eri_gen = 0;
+  int sp2_budget = 48;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
+
   // named scratch buffers (grow-only)
   std::map<std::string, DevBuf> scratch;
+  // named pinned host buffers (hipHostMalloc, grow-only) for asynchronous device-to-host reads
+  std::map<std::string, DevBuf> pinned;
 
   // profiling
   bool profile = false;
@@ -92,22 +102,34 @@ struct zq_ctx {
 int zq_fail(zq_ctx *ctx, int code, const std::string &msg);
 int zq_check(zq_ctx *ctx, hipError_t e, const char *what);
 void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh = nullptr);
+void *zq_pinned(zq_ctx *ctx, const std::string &name, size_t bytes);
 void zq_free_eri(zq_ctx *ctx);
 void zq_prof_begin(zq_ctx *ctx, const char *name);
 void zq_prof_end(zq_ctx *ctx);
-void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip
-int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF);  // zq_jk.hip
+void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip, also drops the captured SCF graph
+int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF,
+                      const int *skip = nullptr, bool f32 = false);  // zq_jk.hip
+int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, made once per load
 int zq_dgemm(zq_ctx *ctx, const double *A, long long sAm, long long sAk, const double *B, long long sBk, long long sBn,
              double *C, long long sCm, long long sCn, int M, int N, int K, const char *name);  // zq_transform.hip
 // C = alpha A B^T (A: M x K, B: N x K, row-major with leading dimensions), latency-sized FP64 MFMA GEMM.
+// With bsel != nullptr, B is offset by *bsel * bstride elements on the device (choice made by a kernel).
 int zq_gemm_nt(zq_ctx *ctx, const double *A, int lda, const double *B, int ldb, double *C, int ldc, int M, int N, int K,
-               double alpha, const char *name);  // zq_dense.hip
+               double alpha, const char *name, const int *skip = nullptr, const int *bsel = nullptr,
+               long long bstride = 0);  // zq_dense.hip
 // SP2 purification of the symmetric n x n dF (ld n): *dD = the projector onto its nocc lowest
 // eigenvectors (m x m, m = n rounded up to 16, zero padded, *ld = m, in context scratch).  ZQ_OK, or
 // ZQ_ERR_NOT_CONVERGED when max_steps did not reach idempotency (no gap); *steps_hint is the number of
 // steps before the first convergence check and is updated on success.
 int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int max_steps, int *steps_hint,
                    const double **dD, int *ld, int *steps);  // zq_dense.hip
+// Enqueues SP2 with a fixed budget of nsteps steps and no host synchronisation.  state5 = two ping-pong
+// {count, steps} pairs and the parity of the buffer that holds the settled projector: the projector is
+// *dX + state5[4] * *stride (m x m, ld m), which a kernel selects at run time.  The final pair is
+// state5 + 2 * (nsteps & 1); count >= 2 there means settled.  All kernels return at once while
+// *skip != 0.
+int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state5,
+                   const double **dX, long long *stride, int *ld);  // zq_dense.hip
 
 #define ZQ_TRY(ctx, call)                                \
   do {                                                   \

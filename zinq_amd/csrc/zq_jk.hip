@@ -48,12 +48,13 @@ __device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
 // vmcnt wait never stalls on them); density values common to the wave come in through scalar loads.
 // Cross-wave sums (K[p,s], K[q,s] at the end of a bs run, J[p,q] at the end of the item) go through
 // LDS in a fixed order.
-__global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ eri, const JKItem *__restrict__ items,
+template <class E>
+__global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, const JKItem *__restrict__ items,
                                                      int nitems, const double *__restrict__ P, int npad, double xexch,
-                                                     double *__restrict__ rec) {
+                                                     double *__restrict__ rec, const int *__restrict__ skip) {
   __shared__ double red[4][2][64];
   const int item_id = blockIdx.x;
-  if (item_id >= nitems) return;
+  if (item_id >= nitems || (skip && *skip)) return;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int Ib = items[item_id].Ib, bp = items[item_id].bp, bq = items[item_id].bq;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ 
 
   double *out = rec + size_t(rec0) * 64;
   // this wave's two rows of the item's first quartet: element (r*8+s)*64 + lane, r = 2w + rr
-  const double *src = eri + (size_t(Ib) * (Ib + 1) / 2 + size_t(k0)) * ZQ_QUARTET + (2 * wave) * 512 + lane;
+  const E *src = eri + (size_t(Ib) * (Ib + 1) / 2 + size_t(k0)) * ZQ_QUARTET + (2 * wave) * 512 + lane;
   double Jbra = 0.0;
   double Kps[8], Kqs[8], Pp_s[8], Pq_s[8], cur[16], nxt[16];
   double Pr_cur[4], Pr_nxt[4];  // Prow[br*8+r], Qrow[br*8+r] for the wave's two rows
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ 
         Pr_nxt[2 * rr] = Prow[nbr * 8 + 2 * wave + rr];
         Pr_nxt[2 * rr + 1] = Qrow[nbr * 8 + 2 * wave + rr];
       }
-      const double *nx = src + size_t(k + 1 - k0) * ZQ_QUARTET;
+      const E *nx = src + size_t(k + 1 - k0) * ZQ_QUARTET;
 #pragma unroll
       for (int j = 0; j < 16; ++j) nxt[j] = nx[j * 64];
     }
@@ -186,9 +187,10 @@ __global__ __launch_bounds__(256) void zq_jk_records(const double *__restrict__ 
 // take contiguous quarters of the block's record list with 8 loads in flight; the quarters are added in
 // a fixed order, so the result is bitwise reproducible.
 __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
-                                                    const int *__restrict__ csr_idx, const double *__restrict__ H, int n,
-                                                    double *__restrict__ F) {
+                                                    const int *__restrict__ csr_idx, const double *H, int n, double *F,
+                                                    const int *__restrict__ skip) {
   __shared__ double part[4][64];
+  if (skip && *skip) return;
   const int o = blockIdx.x;
   int x, y;
   zq_unpair(o, x, y);
@@ -305,17 +307,45 @@ int zq_build_jk_plan(zq_ctx *ctx) {
 }
 
 // Device-resident J/K: P (npad x npad, zero padded) and H (n x n) already on the device; F (n x n) out.
-int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF) {
-  zq_prof_begin(ctx, "jk_records");
-  hipLaunchKernelGGL(zq_jk_records, dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream, ctx->eri,
-                     ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records);
+// With skip != nullptr both kernels return at once while *skip != 0 (device-side early exit).  With
+// f32 the records kernel streams the FP32 copy of the quartets (zq_eri32); products and sums stay FP64.
+// dH may equal dF: every output block reads its base before its own wave writes it, and the blocks
+// above the diagonal are only written (mirror), never read.
+int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF, const int *skip,
+                      bool f32) {
+  if (f32 && !ctx->eri32) return zq_fail(ctx, ZQ_ERR_STATE, "zq_fock_jk_device: no FP32 ERI copy");
+  zq_prof_begin(ctx, f32 ? "jk_records32" : "jk_records");
+  if (f32)
+    hipLaunchKernelGGL(zq_jk_records<float>, dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream, ctx->eri32,
+                       ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records, skip);
+  else
+    hipLaunchKernelGGL(zq_jk_records<double>, dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream, ctx->eri,
+                       ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records, skip);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_records");
   zq_prof_begin(ctx, "jk_reduce");
   hipLaunchKernelGGL(zq_jk_reduce, dim3(unsigned(ctx->Nb)), dim3(256), 0, ctx->stream, ctx->d_records, ctx->d_csr_ptr,
-                     ctx->d_csr_idx, dH, ctx->n, dF);
+                     ctx->d_csr_idx, dH, ctx->n, dF, skip);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_reduce");
+  return ZQ_OK;
+}
+
+namespace {
+__global__ void zq_narrow(const double *__restrict__ x, size_t n, float *__restrict__ y) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    y[i] = float(x[i]);
+}
+}  // namespace
+
+// FP32 copy of the resident quartets (round to nearest), made once per ERI load.
+int zq_eri32(zq_ctx *ctx) {
+  if (ctx->eri32) return ZQ_OK;
+  if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "zq_eri32: no ERIs loaded");
+  const size_t cnt = ctx->nquartets * ZQ_QUARTET;
+  ZQ_TRY(ctx, hipMalloc(&ctx->eri32, cnt * sizeof(float)));
+  hipLaunchKernelGGL(zq_narrow, dim3(4096), dim3(256), 0, ctx->stream, ctx->eri, cnt, ctx->eri32);
+  ZQ_LAUNCH_CHECK(ctx, "zq_narrow");
   return ZQ_OK;
 }
 

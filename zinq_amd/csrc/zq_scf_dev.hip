@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <vector>
@@ -39,8 +40,10 @@ constexpr int kRedBlocks = 64;
 
 // partial[b] = sum over a fixed strided subset of elements of x[i]*y[i] (or of terms), deterministic.
 __global__ __launch_bounds__(256) void k_energy_part(const double *__restrict__ P, int npad, const double *__restrict__ H,
-                                                     const double *__restrict__ F, int n, double *__restrict__ part) {
+                                                     const double *__restrict__ F, int n, double *__restrict__ part,
+                                                     const int *__restrict__ skip) {
   __shared__ double red[256];
+  if (skip && *skip) return;
   double s = 0.0;
   for (size_t e = blockIdx.x * 256 + threadIdx.x; e < size_t(n) * n; e += size_t(gridDim.x) * 256) {
     const int i = int(e / n), j = int(e % n);
@@ -130,10 +133,12 @@ __global__ __launch_bounds__(256) void k_density(const double *__restrict__ C, i
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// P[i][j] (padded npad stride) = Pn[i][j]; part = sum (Pn - P_old)^2.
+// P[i][j] (padded npad stride) = Pn[i][j]; part = sum (Pn - P_old)^2; dP (if given) = Pn - P_old.
 __global__ __launch_bounds__(256) void k_store_density(const double *__restrict__ Pn, int n, double *__restrict__ P, int npad,
-                                                       double *__restrict__ part) {
+                                                       double *__restrict__ part, const int *__restrict__ skip,
+                                                       double *__restrict__ dP) {
   __shared__ double red[256];
+  if (skip && *skip) return;
   double s = 0.0;
   for (size_t e = blockIdx.x * 256 + threadIdx.x; e < size_t(n) * n; e += size_t(gridDim.x) * 256) {
     const int i = int(e / n), j = int(e % n);
@@ -141,6 +146,7 @@ __global__ __launch_bounds__(256) void k_store_density(const double *__restrict_
     const double d = v - P[size_t(i) * npad + j];
     s += d * d;
     P[size_t(i) * npad + j] = v;
+    if (dP) dP[size_t(i) * npad + j] = d;
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -156,6 +162,7 @@ struct Dev {
   int n;
   size_t nn;
   double *S, *X, *XT, *C, *T, *Fp, *W, *FP, *FPS, *Pn, *Fh, *Eh, *part, *scal, *coef, *eig, *offd;
+  double *dP = nullptr;  // P_k - P_(k-1), npad x npad zero padded (mixed Fock path only)
   int *slots, *info;
   int sp2_hint = 24;  // SP2 steps of the previous density, the first convergence check comes after it
   int sp2_steps = 0, sp2_calls = 0, sp2_fallbacks = 0;
@@ -171,8 +178,8 @@ int alloc(zq_ctx *ctx, Dev &d, int n, int maxh) {
   } bufs[] = {{"scf_S", &d.S, nn},       {"scf_X", &d.X, nn},     {"scf_XT", &d.XT, nn},   {"scf_C", &d.C, nn},
               {"scf_T", &d.T, nn},       {"scf_Fp", &d.Fp, nn},   {"scf_W", &d.W, nn},     {"scf_FP", &d.FP, nn},
               {"scf_FPS", &d.FPS, nn},   {"scf_Pn", &d.Pn, nn},   {"scf_Fh", &d.Fh, nn * maxh},
-              {"scf_Eh", &d.Eh, nn * maxh}, {"scf_part", &d.part, size_t(kRedBlocks) * 16},
-              {"scf_scal", &d.scal, 16}, {"scf_coef", &d.coef, 16}, {"scf_eig", &d.eig, size_t(n)},
+              {"scf_Eh", &d.Eh, nn * maxh}, {"scf_part", &d.part, size_t(kRedBlocks) * (2 + maxh)},
+              {"scf_scal", &d.scal, size_t(2 + maxh)}, {"scf_coef", &d.coef, size_t(2 + maxh)}, {"scf_eig", &d.eig, size_t(n)},
               {"scf_offd", &d.offd, size_t(n)}};
   for (auto &b : bufs) {
     *b.p = (double *)zq_scratch(ctx, b.name, b.cnt * sizeof(double));
@@ -230,7 +237,8 @@ int new_density(Dev &d, const double *A, int nocc, double factor) {
   if (rc == ZQ_OK) {
     if (int r2 = zq_gemm_nt(ctx, d.X, n, D, ld, d.T, n, n, n, n, 1.0, "scf_gemm")) return r2;        // T = X D
     if (int r2 = zq_gemm_nt(ctx, d.T, n, d.X, n, d.Pn, n, n, n, n, factor, "scf_gemm")) return r2;  // f T X^T
-    hipLaunchKernelGGL(k_store_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.Pn, n, ctx->d_P, np, d.part);
+    hipLaunchKernelGGL(k_store_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.Pn, n, ctx->d_P, np, d.part,
+                       nullptr, nullptr);
     return ZQ_OK;
   }
   if (rc != ZQ_ERR_NOT_CONVERGED) return rc;
@@ -247,32 +255,240 @@ int sync_read(zq_ctx *ctx, const double *dsrc, double *host, size_t cnt) {
   return ZQ_OK;
 }
 
-}  // namespace
+// ---------------------------------------------------------------------------------------------------
+// Asynchronous driver: the whole iteration, DIIS bookkeeping included, is decided on the device.  Every
+// kernel of an iteration returns at once when state.done != 0, so the host can enqueue iteration k+1
+// before it has seen whether iteration k converged; it reads the state one iteration late.
 
-void zq_release_solver(zq_ctx *ctx) {
-  if (ctx && ctx->solver) {
-    rocblas_destroy_handle((rocblas_handle)ctx->solver);
-    ctx->solver = nullptr;
+constexpr int kMaxHist = 16;
+
+struct ScfState {
+  int done;       // 0 running, 1 converged, 2 iteration limit reached, 3 SP2 did not settle in its budget
+  int it;         // iterations completed
+  int sp2_steps;  // SP2 steps of the last density
+  int sp2_max;    // most SP2 steps of any density so far
+  int hist_n, cur, combine, pad;
+  int hist[kMaxHist];  // DIIS slots, oldest first
+  int sp2[6];          // SP2 ping-pong {count, steps} pairs and the settled buffer's parity
+  double e_old, e_new, rms, dE;
+  double coef[kMaxHist + 1];
+  double B[kMaxHist * kMaxHist];  // B[slot][slot] = <err_slot, err_slot'>
+};
+
+__global__ void k_state_init(ScfState *st, double VN) {
+  if (threadIdx.x != 0) return;
+  st->done = 0, st->it = 0, st->sp2_steps = 0, st->sp2_max = 0;
+  st->hist_n = 0, st->cur = 0, st->combine = 0;
+  st->e_old = VN, st->e_new = VN, st->rms = 0.0, st->dE = 0.0;
+}
+
+// Slot of this iteration's error/Fock pair: the oldest when the history is full, else the lowest free.
+__global__ void k_diis_slot(ScfState *st, int diis) {
+  if (threadIdx.x != 0 || st->done) return;
+  int slot;
+  if (st->hist_n >= diis) {
+    slot = st->hist[0];
+    for (int k = 1; k < st->hist_n; ++k) st->hist[k - 1] = st->hist[k];
+    --st->hist_n;
+  } else {
+    slot = 0;
+    for (bool used = true; used; ) {
+      used = false;
+      for (int k = 0; k < st->hist_n; ++k) used |= st->hist[k] == slot;
+      if (used) ++slot;
+    }
+  }
+  st->hist[st->hist_n++] = slot;
+  st->cur = slot;
+}
+
+// err = FPS - (FPS)^T into Eh[cur], F into Fh[cur].
+__global__ void k_antisym_store(const double *__restrict__ FPS, const double *__restrict__ F, int n,
+                                double *__restrict__ Eh, double *__restrict__ Fh, const ScfState *__restrict__ st) {
+  if (st->done) return;
+  const size_t nn = size_t(n) * n, off = size_t(st->cur) * nn;
+  for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < nn; e += size_t(gridDim.x) * blockDim.x) {
+    const int i = int(e / n), j = int(e % n);
+    Eh[off + e] = FPS[e] - FPS[size_t(j) * n + i];
+    Fh[off + e] = F[e];
   }
 }
 
-extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const double *H, const double *S, double VN,
-                                 const zq_scf_options *opt, double *C, double *e, double *P, double *F,
-                                 zq_scf_result *res) {
-  if (!ctx || !H || !S || !opt || !C || !e || !P || !F || !res || n_ == 0 || nocc_ > n_)
-    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: bad arguments");
-  if (!ctx->eri || size_t(ctx->n) != n_) return zq_fail(ctx, ZQ_ERR_STATE, "zq_scf_run_device: ERIs of this size not loaded");
-  ZQ_TRY(ctx, hipSetDevice(ctx->device));
-  const int n = int(n_), nocc = int(nocc_), np = ctx->npad;
-  const bool generalized = opt->generalized != 0;
-  const double x_exch = generalized ? 1.0 : 0.5, factor = generalized ? 1.0 : 2.0;
-  const int maxh = opt->diis > 0 ? opt->diis : 1;
-  Dev d;
-  if (int rc = alloc(ctx, d, n, maxh)) return rc;
-  const size_t nn = d.nn;
-  const int gblocks = int(std::min<size_t>(1024, (nn + 255) / 256));
+// part[slot*kRedBlocks + b]: dot(err_cur, err_slot) over a strided subset (blockIdx.y = slot).
+__global__ __launch_bounds__(256) void k_dots_state(const double *__restrict__ hist, size_t nn, double *__restrict__ part,
+                                                    const ScfState *__restrict__ st) {
+  __shared__ double red[256];
+  if (st->done) return;
+  const double *enew = hist + size_t(st->cur) * nn, *ei = hist + size_t(blockIdx.y) * nn;
+  double s = 0.0;
+  if (int(blockIdx.y) < kMaxHist) {
+    bool live = false;
+    for (int k = 0; k < st->hist_n; ++k) live |= st->hist[k] == int(blockIdx.y);
+    if (live)
+      for (size_t e = blockIdx.x * 256 + threadIdx.x; e < nn; e += size_t(gridDim.x) * 256) s += enew[e] * ei[e];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * kRedBlocks + blockIdx.x] = red[0];
+}
 
-  // X = L^-T (S = L L^T), the orthonormal starting basis of the core guess
+// LU with partial pivoting and the triangular solves, the same steps as the host zq::getrf/getrs.
+__device__ bool lu_solve(int n, double *A, double *b) {
+  int piv[kMaxHist + 1];
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + k]) > best) best = fabs(A[i * n + k]), p = i;
+    piv[k] = p;
+    if (best == 0.0) return false;
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        const double t = A[k * n + j];
+        A[k * n + j] = A[p * n + j], A[p * n + j] = t;
+      }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+    }
+  }
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k) {
+      const double t = b[k];
+      b[k] = b[piv[k]], b[piv[k]] = t;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) b[i] -= A[i * n + j] * b[j];
+  for (int i = n - 1; i >= 0; --i) {
+    for (int j = i + 1; j < n; ++j) b[i] -= A[i * n + j] * b[j];
+    b[i] /= A[i * n + i];
+  }
+  return true;
+}
+
+// E = sum of the energy partials + V_NN; DIIS: the new row of B, the bordered system, its LU solve
+// (src/hartree_fock.zig:176-227), or a reset to the current entry when it is singular (:821-845).
+__global__ void k_diis_solve(const double *__restrict__ part, int diis, double VN, ScfState *__restrict__ st) {
+  if (threadIdx.x != 0 || st->done) return;
+  double e = 0.0;
+  for (int b = 0; b < kRedBlocks; ++b) e += part[b];
+  st->e_new = e + VN;
+  st->combine = 0;
+  if (diis <= 0) return;
+  const int slot = st->cur, m = st->hist_n;
+  for (int k = 0; k < kMaxHist; ++k) {
+    bool live = false;
+    for (int h = 0; h < m; ++h) live |= st->hist[h] == k;
+    if (!live) continue;
+    double d = 0.0;
+    for (int b = 0; b < kRedBlocks; ++b) d += part[(1 + k) * kRedBlocks + b];
+    st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = d;
+  }
+  if (m < 2) return;
+  const int dd = m + 1;
+  double A[(kMaxHist + 1) * (kMaxHist + 1)], c[kMaxHist + 1];
+  for (int i = 0; i < dd * dd; ++i) A[i] = 0.0;
+  for (int i = 0; i < dd; ++i) c[i] = 0.0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) A[i * dd + j] = st->B[st->hist[i] * kMaxHist + st->hist[j]];
+  for (int i = 0; i < m; ++i) A[i * dd + m] = A[m * dd + i] = -1.0;
+  c[m] = -1.0;
+  if (lu_solve(dd, A, c)) {
+    for (int i = 0; i < m; ++i) st->coef[i] = c[i];
+    st->combine = 1;
+  } else {
+    st->hist[0] = slot;
+    st->hist_n = 1;
+  }
+}
+
+// W = sum_i c_i Fh[hist_i] symmetrised when DIIS combined, else W = F.
+__global__ void k_combine_state(const double *__restrict__ Fh, const double *__restrict__ F, int n, double *__restrict__ W,
+                                const ScfState *__restrict__ st) {
+  if (st->done) return;
+  const size_t nn = size_t(n) * n;
+  const int m = st->combine ? st->hist_n : 0;
+  for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < nn; e += size_t(gridDim.x) * blockDim.x) {
+    const int i = int(e / n), j = int(e % n);
+    if (m == 0) {
+      W[e] = F[e];
+      continue;
+    }
+    if (j < i) continue;
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < m; ++k) {
+      const double *Fk = Fh + size_t(st->hist[k]) * nn;
+      a += st->coef[k] * Fk[size_t(i) * n + j];
+      b += st->coef[k] * Fk[size_t(j) * n + i];
+    }
+    const double avg = (a + b) / 2;
+    W[size_t(i) * n + j] = avg;
+    W[size_t(j) * n + i] = avg;
+  }
+}
+
+__global__ void k_sp2_check(const int *__restrict__ fin, ScfState *__restrict__ st) {
+  if (threadIdx.x != 0 || st->done) return;
+  st->sp2_steps = fin[1];
+  st->sp2_max = max(st->sp2_max, fin[1]);
+  if (fin[0] < 2) st->done = 3;
+}
+
+// rms(dP) and the convergence test of src/hartree_fock.zig:853-870.
+__global__ void k_converge(const double *__restrict__ part, int n, double thr, int maxit, ScfState *__restrict__ st) {
+  if (threadIdx.x != 0 || st->done) return;
+  double s = 0.0;
+  for (int b = 0; b < kRedBlocks; ++b) s += part[b];
+  const double rms = sqrt(s / (double(n) * n));
+  const double dE = fabs(st->e_new - st->e_old);
+  st->e_old = st->e_new;
+  st->rms = rms, st->dE = dE;
+  const bool conv = st->it > 0 && dE < thr && rms < thr;
+  ++st->it;
+  if (conv)
+    st->done = 1;
+  else if (st->it >= maxit)
+    st->done = 2;
+}
+
+// P from the symmetric Fock matrix A by SP2 with a fixed step budget, without host synchronisation.
+int density_async(Dev &d, const double *A, int nocc, double factor, int nsteps, ScfState *st, double *part_rms,
+                  double *dP) {
+  zq_ctx *ctx = d.ctx;
+  const int n = d.n, np = ctx->npad;
+  const int *skip = &st->done;
+  if (int rc = zq_gemm_nt(ctx, d.XT, n, A, n, d.T, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
+  if (int rc = zq_gemm_nt(ctx, d.T, n, d.XT, n, d.Fp, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
+  const double *D = nullptr;
+  long long stride = 0;
+  int ld = 0;
+  if (int rc = zq_sp2_enqueue(ctx, d.Fp, n, nocc, 1e-9, nsteps, skip, st->sp2, &D, &stride, &ld)) return rc;
+  hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, ctx->stream, st->sp2 + 2 * (nsteps & 1), st);
+  if (int rc = zq_gemm_nt(ctx, d.X, n, D, ld, d.T, n, n, n, n, 1.0, "scf_gemm", skip, st->sp2 + 4, stride)) return rc;
+  if (int rc = zq_gemm_nt(ctx, d.T, n, d.X, n, d.Pn, n, n, n, n, factor, "scf_gemm", skip)) return rc;
+  hipLaunchKernelGGL(k_store_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.Pn, n, ctx->d_P, np, part_rms, skip,
+                     dP);
+  ZQ_LAUNCH_CHECK(ctx, "density_async");
+  return ZQ_OK;
+}
+
+struct Run {
+  int n, nocc, np;
+  double x_exch, factor, VN;
+  const zq_scf_options *opt;
+  bool mixed;  // incremental Fock builds from the FP32 ERI copy
+};
+
+// Uploads S, X = L^-T, X^T, H; zeroes P.
+int setup(Dev &d, const double *H, const double *S) {
+  zq_ctx *ctx = d.ctx;
+  const int n = d.n, np = ctx->npad;
+  const size_t nn = d.nn;
   std::vector<double> hX(nn), hXT(nn);
   if (zq::chol_inv_t(n, S, hX.data())) return zq_fail(ctx, ZQ_ERR_LAPACK, "overlap matrix not positive definite");
   for (int i = 0; i < n; ++i)
@@ -282,27 +498,146 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   ZQ_TRY(ctx, hipMemcpyAsync(d.XT, hXT.data(), nn * 8, hipMemcpyHostToDevice, ctx->stream));
   ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_H, H, nn * 8, hipMemcpyHostToDevice, ctx->stream));
   ZQ_TRY(ctx, hipMemsetAsync(ctx->d_P, 0, size_t(np) * np * 8, ctx->stream));
-  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
+  return ZQ_OK;
+}
 
-  // core-Hamiltonian guess, src/hartree_fock.zig:425-431
-  if (int rc = new_density(d, ctx->d_H, nocc, factor)) return rc;
+// The kernels of one SCF iteration; every one of them returns at once while state.done != 0.
+int enqueue_iteration(Dev &d, const Run &r, int budget, ScfState *st, double *part_rms) {
+  zq_ctx *ctx = d.ctx;
+  const int n = d.n, maxh = r.opt->diis > 0 ? r.opt->diis : 1;
+  const size_t nn = d.nn;
+  const int gblocks = int(std::min<size_t>(1024, (nn + 255) / 256));
+  const int *skip = &st->done;
+  if (r.mixed) {
+    // F_k = F_(k-1) + G(P_k - P_(k-1)), in place on d_F, from the FP32 ERI copy
+    if (int rc = zq_fock_jk_device(ctx, d.dP, ctx->d_F, r.x_exch, ctx->d_F, skip, true)) return rc;
+  } else if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, r.x_exch, ctx->d_F, skip)) {
+    return rc;
+  }
+  hipLaunchKernelGGL(k_energy_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, ctx->d_P, r.np, ctx->d_H, ctx->d_F, n,
+                     d.part, skip);
+  if (r.opt->diis > 0) {
+    hipLaunchKernelGGL(k_diis_slot, dim3(1), dim3(64), 0, ctx->stream, st, r.opt->diis);
+    // F P (P symmetric) and (F P) S (S symmetric) as A B^T products
+    if (int rc = zq_gemm_nt(ctx, ctx->d_F, n, ctx->d_P, r.np, d.FP, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
+    if (int rc = zq_gemm_nt(ctx, d.FP, n, d.S, n, d.FPS, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
+    hipLaunchKernelGGL(k_antisym_store, dim3(gblocks), dim3(256), 0, ctx->stream, d.FPS, ctx->d_F, n, d.Eh, d.Fh, st);
+    hipLaunchKernelGGL(k_dots_state, dim3(kRedBlocks, maxh), dim3(256), 0, ctx->stream, d.Eh, nn, d.part + kRedBlocks, st);
+  }
+  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, ctx->stream, d.part, r.opt->diis, r.VN, st);
+  hipLaunchKernelGGL(k_combine_state, dim3(gblocks), dim3(256), 0, ctx->stream, d.Fh, ctx->d_F, n, d.W, st);
+  if (int rc = density_async(d, d.W, r.nocc, r.factor, budget, st, part_rms, r.mixed ? d.dP : nullptr)) return rc;
+  hipLaunchKernelGGL(k_converge, dim3(1), dim3(64), 0, ctx->stream, part_rms, n, r.opt->threshold, r.opt->iterations, st);
+  ZQ_LAUNCH_CHECK(ctx, "scf iteration");
+  return ZQ_OK;
+}
+
+// The asynchronous SCF.  Returns ZQ_OK with *status = state.done (1 converged, 2 iteration limit,
+// 3 SP2 budget exceeded: the caller reruns the synchronised driver), the final energy and iteration
+// count, and W = the last diagonalised Fock matrix.
+int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations) {
+  zq_ctx *ctx = d.ctx;
+  const int n = d.n, nocc = r.nocc;
+  const size_t nn = d.nn;
+  auto *st = (ScfState *)zq_scratch(ctx, "scf_state", sizeof(ScfState));
+  auto *hst = (int *)zq_pinned(ctx, "scf_state_host", 2 * 4 * sizeof(int));
+  if (!st || !hst) return zq_fail(ctx, ZQ_ERR_OOM, "SCF state");
+  double *part_rms = d.part + (1 + kMaxHist) * kRedBlocks;
+  hipEvent_t ev[2];
+  ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  struct EvGuard {
+    hipEvent_t *e;
+    ~EvGuard() { (void)hipEventDestroy(e[0]), (void)hipEventDestroy(e[1]); }
+  } guard{ev};
+
+  if (r.mixed) {
+    if (int rc = zq_eri32(ctx)) return rc;
+    d.dP = (double *)zq_scratch(ctx, "scf_dP", size_t(r.np) * r.np * sizeof(double));
+    if (!d.dP) return zq_fail(ctx, ZQ_ERR_OOM, "SCF dP");
+    ZQ_TRY(ctx, hipMemsetAsync(d.dP, 0, size_t(r.np) * r.np * sizeof(double), ctx->stream));
+  }
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, ctx->stream, st, r.VN);
+  // core-Hamiltonian guess, src/hartree_fock.zig:425-431; W = H until the first iteration replaces it
+  ZQ_TRY(ctx, hipMemcpyAsync(d.W, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, 64, st, part_rms, r.mixed ? d.dP : nullptr)) return rc;
+  // the incremental builds start from F = H (dP = P_0 after the guess)
+  if (r.mixed) ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_F, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  const int budget = ctx->sp2_budget;
+
+  // one iteration as a hipGraph, captured once per context and key, replayed with one launch
+  const bool use_graph = !ctx->profile;
+  char key[256];
+  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%u/%u/%p/%d", n, nocc, budget, r.opt->diis, r.opt->iterations,
+           r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->scratch_gen, ctx->eri_gen, (void *)st, int(r.mixed));
+  if (use_graph && (!ctx->scf_graph || ctx->scf_graph_key != key)) {
+    if (ctx->scf_graph) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
+    ctx->scf_graph = nullptr;
+    ZQ_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_iteration(d, r, budget, st, part_rms);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return zq_check(ctx, e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) return zq_check(ctx, e2, "hipGraphInstantiate");
+    ctx->scf_graph = exec;
+    ctx->scf_graph_key = key;
+  }
+  int k = 0;
+  for (; k < r.opt->iterations; ++k) {
+    if (use_graph) {
+      ZQ_TRY(ctx, hipGraphLaunch((hipGraphExec_t)ctx->scf_graph, ctx->stream));
+    } else if (int rc = enqueue_iteration(d, r, budget, st, part_rms)) {
+      return rc;
+    }
+    ZQ_TRY(ctx, hipMemcpyAsync(hst + 4 * (k & 1), st, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ZQ_TRY(ctx, hipEventRecord(ev[k & 1], ctx->stream));
+    if (k >= 1) {
+      // state after iteration k-1; iteration k is already queued behind it
+      ZQ_TRY(ctx, hipEventSynchronize(ev[(k - 1) & 1]));
+      if (hst[4 * ((k - 1) & 1)] != 0) break;
+    }
+  }
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ScfState hs;
+  ZQ_TRY(ctx, hipMemcpy(&hs, st, sizeof(ScfState), hipMemcpyDeviceToHost));
+  *status = hs.done == 0 ? 2 : hs.done;
+  *energy = hs.e_new;
+  *iterations = hs.it;
+  return ZQ_OK;
+}
+
+// The host-synchronised driver: SP2 with host convergence checks and a dsyevd fallback per density,
+// DIIS solved on the host.  Used when the asynchronous driver's SP2 budget was too small.
+int scf_synced(Dev &d, const Run &r, int *status, double *energy, int *iterations) {
+  zq_ctx *ctx = d.ctx;
+  const zq_scf_options *opt = r.opt;
+  const int n = d.n, nocc = r.nocc, np = r.np, maxh = opt->diis > 0 ? opt->diis : 1;
+  const size_t nn = d.nn;
+  const int gblocks = int(std::min<size_t>(1024, (nn + 255) / 256));
+  const double VN = r.VN;
+  ZQ_TRY(ctx, hipMemcpyAsync(d.W, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  if (int rc = new_density(d, ctx->d_H, nocc, r.factor)) return rc;
 
   std::deque<int> hist;  // ring-buffer slots of the DIIS history, oldest first
   std::vector<double> Bfull(64 * 64, 0.0);  // B[slot][slot]
   double e_old = VN, e_new = VN;
-  const double *Fout = ctx->d_F;  // the Fock matrix diagonalised last (DIIS-extrapolated when DIIS ran)
   int it;
   bool converged = false;
   for (it = 0; it < opt->iterations; ++it) {
-    // F = H + G(P); E
-    if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, x_exch, ctx->d_F)) return rc;
+    if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, r.x_exch, ctx->d_F)) return rc;
     hipLaunchKernelGGL(k_energy_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, ctx->d_P, np, ctx->d_H, ctx->d_F, n,
-                       d.part);
+                       d.part, nullptr);
     const double *Fdiag = ctx->d_F;
-    double hbuf[16];
+    double hbuf[1 + 64];
     if (opt->diis > 0) {
-      // err = FPS - (FPS)^T into the next history slot; F copied alongside
-      // F P (P symmetric) and (F P) S (S symmetric) as A B^T products
       if (int rc = zq_gemm_nt(ctx, ctx->d_F, n, ctx->d_P, np, d.FP, n, n, n, n, 1.0, "scf_gemm")) return rc;
       if (int rc = zq_gemm_nt(ctx, d.FP, n, d.S, n, d.FPS, n, n, n, n, 1.0, "scf_gemm")) return rc;
       int slot;
@@ -319,7 +654,7 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
       const int m = int(hist.size());
       hipLaunchKernelGGL(k_dots, dim3(kRedBlocks, maxh), dim3(256), 0, ctx->stream, d.Eh, d.Eh + size_t(slot) * nn, nn,
                          d.part + kRedBlocks);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, ctx->stream, d.part, 1 + maxh, d.scal);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(128), 0, ctx->stream, d.part, 1 + maxh, d.scal);
       if (int rc = sync_read(ctx, d.scal, hbuf, 1 + maxh)) return rc;
       e_new = hbuf[0] + VN;
       for (int k = 0; k < maxh; ++k) Bfull[slot * 64 + k] = Bfull[k * 64 + slot] = hbuf[1 + k];
@@ -350,9 +685,8 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
       if (int rc = sync_read(ctx, d.scal, hbuf, 1)) return rc;
       e_new = hbuf[0] + VN;
     }
-    // new density from the (extrapolated) Fock matrix
-    Fout = Fdiag;
-    if (int rc = new_density(d, Fdiag, nocc, factor)) return rc;
+    if (Fdiag != d.W) ZQ_TRY(ctx, hipMemcpyAsync(d.W, Fdiag, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    if (int rc = new_density(d, d.W, nocc, r.factor)) return rc;
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, ctx->stream, d.part, 1, d.scal);
     if (int rc = sync_read(ctx, d.scal, hbuf, 1)) return rc;
     const double delta_energy = std::fabs(e_new - e_old);
@@ -364,18 +698,88 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
       break;
     }
   }
-  // outputs: C and e from the last diagonalised F, P rebuilt from C, and that F
-  if (int rc = to_orthonormal(d, Fout)) return rc;
+  *status = converged ? 1 : 2;
+  *energy = e_new;
+  *iterations = converged ? it : opt->iterations;
+  return ZQ_OK;
+}
+
+}  // namespace
+
+void zq_release_solver(zq_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->solver) {
+    rocblas_destroy_handle((rocblas_handle)ctx->solver);
+    ctx->solver = nullptr;
+  }
+  if (ctx->scf_graph) {
+    (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
+    ctx->scf_graph = nullptr;
+  }
+}
+
+extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const double *H, const double *S, double VN,
+                                 const zq_scf_options *opt, double *C, double *e, double *P, double *F,
+                                 zq_scf_result *res) {
+  if (!ctx || !H || !S || !opt || !C || !e || !P || !F || !res || n_ == 0 || nocc_ > n_)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: bad arguments");
+  if (!ctx->eri || size_t(ctx->n) != n_) return zq_fail(ctx, ZQ_ERR_STATE, "zq_scf_run_device: ERIs of this size not loaded");
+  if (opt->diis > 64) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: DIIS history above 64");
+  if (opt->fock_precision != ZQ_PREC_FP64 && opt->fock_precision != ZQ_PREC_FP32)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: fock_precision must be ZQ_PREC_FP64 or ZQ_PREC_FP32");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  const int n = int(n_), nocc = int(nocc_), np = ctx->npad;
+  const bool generalized = opt->generalized != 0;
+  Run r{n, nocc, np, generalized ? 1.0 : 0.5, generalized ? 1.0 : 2.0, VN, opt, opt->fock_precision == ZQ_PREC_FP32};
+  const int maxh = opt->diis > 0 ? opt->diis : 1;
+  Dev d;
+  if (int rc = alloc(ctx, d, n, std::max(maxh, kMaxHist))) return rc;
+  const size_t nn = d.nn;
+
+  int status = 0, iterations = 0;
+  double energy = VN;
+  if (int rc = setup(d, H, S)) return rc;
+  if (opt->diis <= kMaxHist) {
+    if (int rc = scf_async(d, r, &status, &energy, &iterations)) return rc;
+  }
+  if (status == 3) ctx->sp2_budget = std::min(100, ctx->sp2_budget + 16);
+  if (status == 0 || status == 3) {
+    // DIIS history beyond the device bookkeeping, or an SP2 budget too small: synchronised driver
+    // (FP64 Fock builds throughout)
+    if (int rc = setup(d, H, S)) return rc;
+    if (int rc = scf_synced(d, r, &status, &energy, &iterations)) return rc;
+  } else if (r.mixed) {
+    // mixed path: one full FP64 build from the last density; its energy is variational (error second
+    // order in the density error) and its eigenvectors are the returned orbitals
+    if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, r.x_exch, ctx->d_F)) return rc;
+    hipLaunchKernelGGL(k_energy_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, ctx->d_P, np, ctx->d_H, ctx->d_F, n,
+                       d.part, nullptr);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, ctx->stream, d.part, 1, d.scal);
+    double h = 0.0;
+    if (int rc = sync_read(ctx, d.scal, &h, 1)) return rc;
+    energy = h + VN;
+    ++iterations;
+    ZQ_TRY(ctx, hipMemcpyAsync(d.W, ctx->d_F, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+
+  // outputs: C and e from the last diagonalised F (W), P rebuilt from C, and that F
+  if (int rc = to_orthonormal(d, d.W)) return rc;
   if (int rc = diag_orthonormal(d, d.C)) return rc;
-  hipLaunchKernelGGL(k_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.C, n, nocc, factor, ctx->d_P, np, d.part);
+  hipLaunchKernelGGL(k_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.C, n, nocc, r.factor, ctx->d_P, np, d.part);
   std::vector<double> hP(size_t(np) * np);
   ZQ_TRY(ctx, hipMemcpyAsync(C, d.C, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipMemcpyAsync(e, d.eig, size_t(n) * 8, hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipMemcpyAsync(hP.data(), ctx->d_P, hP.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  ZQ_TRY(ctx, hipMemcpyAsync(F, Fout, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
+  ZQ_TRY(ctx, hipMemcpyAsync(F, d.W, nn * 8, hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < n; ++i) std::memcpy(P + size_t(i) * n, hP.data() + size_t(i) * np, n * sizeof(double));
-  res->energy = e_new;
-  res->iterations = converged ? it : opt->iterations;
-  return converged ? ZQ_OK : zq_fail(ctx, ZQ_ERR_NOT_CONVERGED, "SCF did not converge");
+  res->energy = energy;
+  res->iterations = iterations;
+  return status == 1 ? ZQ_OK : zq_fail(ctx, ZQ_ERR_NOT_CONVERGED, "SCF did not converge");
+}
+
+extern "C" int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps) {
+  if (!ctx || steps < 1 || steps > 100) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_set_sp2_budget: steps outside 1..100");
+  ctx->sp2_budget = steps;
+  return ZQ_OK;
 }

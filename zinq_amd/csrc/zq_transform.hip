@@ -14,6 +14,8 @@
 // Precisions of the quarter transforms:
 //   FP64    v_mfma_f64_16x16x4f64, FP64 intermediates;
 //   FP32    v_mfma_f32_16x16x4f32, FP32 intermediates;
+//   FP32C   v_mfma_f32_16x16x4f32 products summed in FP32 over one K block (16 k; 8 mu in Q1), the
+//           blocks summed in FP64 (a compensated sum of FP32 products); FP32 intermediates;
 //   BF16X3  every operand split x = hi + lo (two bf16), three v_mfma_f32_16x16x32_bf16 products
 //           hi*hi + (hi*lo + lo*hi) with the small products in a separate FP32 accumulator that is added
 //           at the end; FP32 intermediates.
@@ -39,6 +41,8 @@ struct MF64 {
   static __device__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
   static __device__ acc mma(T a, T b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
   static __device__ int row(int lane, int j) { return 4 * j + (lane >> 4); }
+  static __device__ void flush(acc &) {}
+  static __device__ double val(const acc &c, int j) { return c[j]; }
 };
 
 // v_mfma_f32_16x16x4f32: same operand layout, D[4*(l/16) + j][l%16].
@@ -48,6 +52,31 @@ struct MF32 {
   static __device__ acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
   static __device__ acc mma(T a, T b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   static __device__ int row(int lane, int j) { return 4 * (lane >> 4) + j; }
+  static __device__ void flush(acc &) {}
+  static __device__ double val(const acc &c, int j) { return c[j]; }
+};
+
+// FP32 products with FP64 sums: v_mfma_f32_16x16x4f32 accumulates one K block (16 k in the GEMMs, 8 mu
+// in Q1) in FP32; flush() adds the block's partial sums into FP64 accumulators and restarts the FP32
+// ones.  The FP32 rounding of the running sum then only spans one block instead of all of K.
+struct MF32C {
+  typedef float T;
+  struct acc {
+    f4 p;
+    d4 s;
+  };
+  static __device__ acc zero() { return acc{f4{0.f, 0.f, 0.f, 0.f}, d4{0.0, 0.0, 0.0, 0.0}}; }
+  static __device__ acc mma(T a, T b, acc c) {
+    c.p = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c.p, 0, 0, 0);
+    return c;
+  }
+  static __device__ int row(int lane, int j) { return 4 * (lane >> 4) + j; }
+  static __device__ void flush(acc &c) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c.s[j] += double(c.p[j]);
+    c.p = f4{0.f, 0.f, 0.f, 0.f};
+  }
+  static __device__ double val(const acc &c, int j) { return c.s[j] + double(c.p[j]); }
 };
 
 // v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][8*(l/16)+0..7] and B[8*(l/16)+0..7][l%16];
@@ -147,6 +176,8 @@ __global__ __launch_bounds__(512) void zq_q1(const double *__restrict__ eri, con
         acc[t] = MF::mma(a, Ts[K][uu * 8 + wl], acc[t]);
       }
     }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) MF::flush(acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
@@ -155,7 +186,7 @@ __global__ __launch_bounds__(512) void zq_q1(const double *__restrict__ eri, con
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = i0 + it * 16 + MF::row(lane, j);
-      if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = acc[t][j];
+      if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = T(MF::val(acc[t], j));
     }
   }
 }
@@ -322,6 +353,10 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
 #pragma unroll
         for (int y = 0; y < 2; ++y) acc[x][y] = MF::mma(a[x], bb[y], acc[x][y]);
     }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) MF::flush(acc[x][y]);
   }
   T *C = g.C + b * g.sCb;
 #pragma unroll
@@ -332,7 +367,7 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int m = m0 + wm + x * 16 + MF::row(lane, j);
-        if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = acc[x][y][j];
+        if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = T(MF::val(acc[x][y], j));
       }
     }
 }
@@ -601,7 +636,8 @@ int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int p
   Dims d;
   d.n = ctx->n, d.np = ctx->npad, d.nb = ctx->nb, d.Nb = ctx->Nb;
   if (block != ZQ_BLOCK_FULL && block != ZQ_BLOCK_OVOV) return zq_fail(ctx, ZQ_ERR_INVALID, "bad block");
-  if (precision != ZQ_PREC_FP64 && precision != ZQ_PREC_FP32 && precision != ZQ_PREC_BF16X3)
+  if (precision != ZQ_PREC_FP64 && precision != ZQ_PREC_FP32 && precision != ZQ_PREC_BF16X3 &&
+      precision != ZQ_PREC_FP32C)
     return zq_fail(ctx, ZQ_ERR_INVALID, "bad precision");
   if (block == ZQ_BLOCK_OVOV && (nocc <= 0 || nocc >= d.n)) return zq_fail(ctx, ZQ_ERR_INVALID, "bad nocc");
   d.m1 = block == ZQ_BLOCK_OVOV ? nocc : d.n, d.m2 = d.m1;
@@ -616,7 +652,8 @@ int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int p
   }
   *is_f32 = true;
   float *o = nullptr;
-  int rc = run_chain<float, MF32>(ctx, C, d, precision, &o);
+  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, C, d, precision, &o)
+                                      : run_chain<float, MF32>(ctx, C, d, precision, &o);
   *d_out = o;
   return rc;
 }

@@ -12,11 +12,11 @@ from ._native import load, ptr
 
 ZQ_OK, ZQ_ERR_INVALID, ZQ_ERR_OOM, ZQ_ERR_LAUNCH, ZQ_ERR_NOT_CONVERGED, ZQ_ERR_LAPACK, ZQ_ERR_STATE = range(7)
 BLOCK = {"full": 0, "ovov": 1}
-PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32c": 3}
 
 EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_load_packed", "zq_fock_jk", "zq_ao2mo",
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
-            "zq_host_sygv", "zq_scf_run_device"]
+            "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget"]
 
 
 class GpuError(RuntimeError):
@@ -52,7 +52,7 @@ _ERRORS = {c.code: c for c in (InvalidInput, GpuOutOfMemory, GpuLaunch, ScfDidNo
 
 class ScfOptions(ctypes.Structure):
     _fields_ = [("diis", ctypes.c_int), ("iterations", ctypes.c_int), ("threshold", ctypes.c_double),
-                ("generalized", ctypes.c_int)]
+                ("generalized", ctypes.c_int), ("fock_precision", ctypes.c_int)]
 
 
 class ScfResult(ctypes.Structure):
@@ -82,6 +82,7 @@ def lib():
                                  ctypes.POINTER(ScfResult)]
         L.zq_host_sygv.argtypes = [sz, vp, vp, vp]
         L.zq_scf_run_device.argtypes = L.zq_scf_run.argtypes
+        L.zq_scf_set_sp2_budget.argtypes = [vp, ci]
         L._zq_typed = True
     return L
 
@@ -165,19 +166,25 @@ class Context:
         self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
         return float(out[0])
 
-    def scf(self, H, S, VN, nocc, diis=8, iterations=100, threshold=1e-8, generalized=False, device=False):
+    def scf(self, H, S, VN, nocc, diis=8, iterations=100, threshold=1e-8, generalized=False, device=False,
+            fock_precision="fp64"):
         """zq_scf_run (host DIIS/diagonalisation, reference-faithful) or, with device=True,
-        zq_scf_run_device (everything n x n on the GPU)."""
+        zq_scf_run_device (everything n x n on the GPU; fock_precision="fp32" selects the mixed path)."""
         H, S = _f64(H), _f64(S)
         n = H.shape[0]
         C, e, P, F = np.zeros((n, n)), np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
-        opt = ScfOptions(int(diis or 0), int(iterations), float(threshold), int(bool(generalized)))
+        opt = ScfOptions(int(diis or 0), int(iterations), float(threshold), int(bool(generalized)),
+                         PRECISION[fock_precision])
         res = ScfResult()
         fn = self._L.zq_scf_run_device if device else self._L.zq_scf_run
         rc = fn(self._h, n, nocc, ptr(H), ptr(S), float(VN), ctypes.byref(opt), ptr(C), ptr(e), ptr(P),
                                 ptr(F), ctypes.byref(res))
         self._check(rc)
         return dict(E=res.energy, iterations=res.iterations, C=C, e=e, P=P, F=F)
+
+    def set_sp2_budget(self, steps):
+        """SP2 steps per density in the captured device-SCF iteration (zq_scf_set_sp2_budget)."""
+        self._check(self._L.zq_scf_set_sp2_budget(self._h, int(steps)))
 
     def profile(self, on=True):
         self._L.zq_profile_enable(self._h, int(on))

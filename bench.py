@@ -120,6 +120,42 @@ def cpu_baseline(mols, seconds_budget=30.0):
                        f"one core, scaled by n^4 x SCF iterations and o*n^4 to k={[x['k'] for x in mols]}")
 
 
+def sweep(k, basis):
+    """Precision sweep on one (H2O)_k: converged FP64 RHF orbitals, then the OVOV transform and the MP2
+    energy in every transform precision, against the FP64 GPU form (itself matched to the oracle at
+    smaller sizes).  Also the whole mixed path (mixed Fock + FP32C transform) against the plain form."""
+    from zinq_amd import hip
+    from zinq_amd.molecules import water_system
+    s = water_system(k, basis=basis)
+    n, o = s.nbf, s.nel // 2
+    ctx = hip.Context(0)
+    ctx.eri_load_packed(s.coulomb_packed(), n)
+    H, S, VN = s.hcore(), s.overlap(), s.nrep()
+    r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    rows, ref_T, ref_e2 = [], None, None
+    for prec in ("fp64", "fp32c", "fp32", "bf16x3"):
+        ctx.mp2(r["C"], r["e"], o, prec)  # warm-up (scratch allocation)
+        t = time.perf_counter()
+        e2 = ctx.mp2(r["C"], r["e"], o, prec)
+        dt = time.perf_counter() - t
+        T = ctx.ao2mo(r["C"], o, "ovov", prec)
+        if prec == "fp64":
+            ref_T, ref_e2 = T, e2
+        rows.append(dict(precision=prec, E_corr=e2, abs_dE_mp2=abs(e2 - ref_e2),
+                         max_abs_d_iajb=float(np.abs(T - ref_T).max()), mp2_ms=round(dt * 1e3, 2),
+                         transform_TFLOPs=round(ovov_flops(n, o) / dt / 1e12, 2)))
+        del T
+        log(rows[-1])
+    rm = ctx.scf(H, S, VN, o, threshold=1e-6, device=True, fock_precision="fp32")
+    e2m = ctx.mp2(rm["C"], rm["e"], o, "fp32c")
+    mixed = dict(E_rhf=rm["E"], dE_rhf=rm["E"] - r["E"], E_mp2=rm["E"] + e2m,
+                 abs_dE_mp2=abs(rm["E"] + e2m - r["E"] - ref_e2), iterations=rm["iterations"])
+    ctx.close()
+    return dict(sweep=f"(H2O)_{k} {basis.split(':')[-1]}", n=n, o=o, E_rhf=r["E"], E_mp2=r["E"] + ref_e2,
+                rows=rows, mixed_path=mixed,
+                reference="FP64 GPU form (MP2 transform + FP64 sums), matched to the CPU oracle at n <= 96")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,10 +168,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scf", default="device", choices=["device", "host"])
     ap.add_argument("--serial", action="store_true", help="run the molecules of a step one after another")
+    ap.add_argument("--sweep", type=int, default=0, metavar="K", help="precision sweep on (H2O)_K instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: only the one-GPU path is built; a multi-GPU run is not supported.", file=sys.stderr)
         sys.exit(2)
+    if args.sweep:
+        print(json.dumps(sweep(args.sweep, args.basis)), flush=True)
+        return
     ks = [int(k) for k in args.ks.split(",")]
     mols = prepare(ks, args.basis)
 

@@ -254,3 +254,68 @@ def test_mixed_path_within_bar(ctx, kind):
     # the FP32 ERI copy stays resident; a plain FP64 SCF afterwards is unaffected
     r64 = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
     assert r64["E"] == pytest.approx(ro["E"], abs=1e-10)
+
+
+# ---------------------------------------------------------------- grouped many-molecule path
+def _batch_systems(count, seed=77):
+    from zinq_amd.molecules import batch_specs, batch_system
+    return [batch_system(s) for s in batch_specs(count, seed=seed)]
+
+
+def test_batch_matches_single_molecule_path():
+    """Grouped J/K + per-molecule workgroup SCF + batched dsyevd + MP2 equals the single-molecule
+    device path (same FP64 arithmetic; reductions differ in order only)."""
+    from zinq_amd import hip
+    from zinq_amd.batch import Batch
+    systems = _batch_systems(10)
+    b = Batch(0)
+    for s in systems:
+        b.add_system(s)
+    r = b.run(threshold=1e-10)
+    assert (r["status"] == 0).all(), r["status"]
+    with hip.Context(0) as ctx:
+        for m, s in enumerate(systems):
+            ctx.eri_load_packed(s.coulomb_packed(), s.nbf)
+            o = s.nel // 2
+            ref = ctx.scf(s.hcore(), s.overlap(), s.nrep(), o, threshold=1e-10, device=True)
+            e2 = ctx.mp2(ref["C"], ref["e"], o)
+            print(f"mol {m} n={s.nbf} dE={r['E_rhf'][m] - ref['E']:+.1e} dEc={r['E_corr'][m] - e2:+.1e} "
+                  f"it={r['iterations'][m]}/{ref['iterations']}")
+            assert r["E_rhf"][m] == pytest.approx(ref["E"], abs=1e-10)
+            assert r["E_corr"][m] == pytest.approx(e2, abs=1e-10)
+            e, C = b.orbitals(m)
+            np.testing.assert_allclose(e, ref["e"], atol=1e-8)
+    b.close()
+
+
+def test_batch_result_same_alone_and_in_the_mix():
+    """Each molecule's numbers do not depend on which other molecules share the batch: bitwise."""
+    from zinq_amd.batch import Batch
+    systems = _batch_systems(6, seed=91)
+    mix = Batch(0)
+    for s in systems:
+        mix.add_system(s)
+    rm = mix.run(threshold=1e-10)
+    for m in (0, 3, 5):
+        alone = Batch(0)
+        alone.add_system(systems[m])
+        ra = alone.run(threshold=1e-10)
+        assert ra["E_rhf"][0] == rm["E_rhf"][m] and ra["E_corr"][0] == rm["E_corr"][m]
+        assert ra["iterations"][0] == rm["iterations"][m]
+        alone.close()
+    mix.close()
+
+
+def test_batch_autotune_and_rerun():
+    from zinq_amd.batch import Batch
+    systems = _batch_systems(8, seed=5)
+    b = Batch(0)
+    for s in systems:
+        b.add_system(s)
+    r0 = b.run(threshold=1e-10)
+    chosen = b.autotune()
+    assert chosen["item_quartets"] in (1, 2, 4, 8, 16) and chosen["workgroup"] in (256, 512)
+    r1 = b.run(threshold=1e-10)
+    np.testing.assert_allclose(r1["E_rhf"], r0["E_rhf"], atol=1e-11)
+    np.testing.assert_allclose(r1["E_corr"], r0["E_corr"], atol=1e-11)
+    b.close()

@@ -21,7 +21,7 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header,lib", [("zinq_hip.h", "libzinq_hip.so"), ("zinq_scf.h", "libzinq_hip.so"),
-                                        ("zinq_ints.h", "libzinq_ints.so")])
+                                        ("zinq_batch.h", "libzinq_hip.so"), ("zinq_ints.h", "libzinq_ints.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = _declared(header)
     assert len(names) >= 2
@@ -141,3 +141,15 @@ def test_water_cluster_generator_is_deterministic():
     if os.path.exists(golden):
         ref = np.array([[float(v) for v in ln.split()[1:]] for ln in open(golden).read().splitlines()[2:]])
         np.testing.assert_allclose(x1, ref, atol=1e-9)
+
+
+def test_batch_specs_cover_the_workload():
+    """512 seeded molecules, k in 1..3, STO-3G and cc-pVDZ: sizes n in {7, 14, 21, 24, 48, 72}."""
+    from zinq_amd.molecules import batch_specs, batch_system
+    specs = batch_specs()
+    assert len(specs) == 512 and specs == batch_specs()
+    assert {s["k"] for s in specs} == {1, 2, 3}
+    sizes = {(s["k"], s["basis"]): None for s in specs}
+    assert len(sizes) == 6
+    n = sorted({batch_system(s).nbf for s in specs[:40]})
+    assert set(n) <= {7, 14, 21, 24, 48, 72}

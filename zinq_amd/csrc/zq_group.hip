@@ -1,0 +1,869 @@
+// Grouped RHF + MP2 of many small molecules (include/zinq_batch.h).
+//
+// Per SCF iteration, for every molecule that has not converged, three launches captured in one graph:
+//   zq_jk_records / zq_jk_reduce over the grouped work plan     F = H + G(P)   (getFock, hartree_fock.zig:686-735)
+//   k_batch_step, one workgroup per molecule:
+//     E = 1/2 sum P (H + F) + V_NN                                (getEnergy, :643-655)
+//     err = F P S - (F P S)^T, history, bordered LU, F_ext        (getError :658-683, diis :176-227, :810-845)
+//     F' = X^T F_ext X, SP2 purification -> D, P = 2 X D X^T       (geigh + getDensity, :621-640)
+//     rms(dP), |dE|, convergence test                            (:853-870)
+//   k_batch_count: molecules done, read by the host one iteration late.
+// Every per-molecule matrix lives in one workspace with leading dimension mp = n rounded up to 16 and
+// zero padding, so the small GEMMs (zq_tile.h, A B^T on FP64 MFMA, one wave per 16 x 16 tile) never
+// need bounds checks.  After convergence: F' of the last extrapolated F for each molecule, rocSOLVER
+// dsyevd_strided_batched per distinct n, C = X V, and the MP2 energy from the OVOV transform of
+// zq_transform.hip through a context that views the molecule's resident quartets.
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/zinq_batch.h"
+#include "zq_internal.h"
+#include "zq_linalg.h"
+#include "zq_tile.h"
+
+using namespace zqt;
+
+namespace {
+
+constexpr int kMaxN = 96;
+constexpr int kHist = 8;
+constexpr int kMats = 13;
+enum { M_H, M_F, M_P, M_S, M_X, M_XT, M_W, M_T, M_FP, M_FPS, M_FPR, M_XS, M_X2 };
+enum { MODE_GUESS = 0, MODE_ITER = 1 };
+
+struct BMol {
+  long long ws;  // workspace offset (doubles): kMats + 2 kHist matrices of mp x mp
+  int n, mp, nocc, pad;
+  double VN;
+};
+
+struct BState {
+  int done, it, status, hist_n, cur, combine, sp2_steps, pad;
+  int hist[kHist];
+  double e_old, e_new, rms, dE;
+  double coef[kHist + 1];
+  double B[kHist * kHist];
+};
+
+template <int NT>
+__device__ double block_sum(double v, double *red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT / 64; ++k) s += red[k];
+  return s;
+}
+
+template <int NT>
+__device__ void block_minmax(double &lo, double &hi, double *red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[w] = lo, red[NT / 64 + w] = hi;
+  __syncthreads();
+  lo = red[0], hi = red[NT / 64];
+#pragma unroll
+  for (int k = 1; k < NT / 64; ++k) lo = fmin(lo, red[k]), hi = fmax(hi, red[NT / 64 + k]);
+}
+
+// C = alpha A B^T, all mp x mp with leading dimension mp (mp a multiple of 16); the workgroup's waves
+// take the 16 x 16 tiles in turn.  Ends with a barrier.
+template <int NT>
+__device__ void wg_gemm_nt(const double *A, const double *B, double *C, int mp, double alpha) {
+  const int T = mp >> 4, waves = NT / 64, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  for (int t = wave; t < T * T; t += waves) {
+    const int ti = t / T, tj = t - ti * T;
+    const d4 acc = tile_nt<true>(A, mp, B, mp, ti * 16, tj * 16, mp, mp, mp);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) C[size_t(ti * 16 + 4 * j + q) * mp + tj * 16 + r] = alpha * acc[j];
+  }
+  __syncthreads();
+}
+
+// Bordered DIIS system (m + 1) x (m + 1) by LU with partial pivoting, the steps of zq::getrf/getrs.
+__device__ bool lu_solve(int n, double *A, double *b) {
+  int piv[kHist + 1];
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + k]) > best) best = fabs(A[i * n + k]), p = i;
+    piv[k] = p;
+    if (best == 0.0) return false;
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        const double t = A[k * n + j];
+        A[k * n + j] = A[p * n + j], A[p * n + j] = t;
+      }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+    }
+  }
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k) {
+      const double t = b[k];
+      b[k] = b[piv[k]], b[piv[k]] = t;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) b[i] -= A[i * n + j] * b[j];
+  for (int i = n - 1; i >= 0; --i) {
+    for (int j = i + 1; j < n; ++j) b[i] -= A[i * n + j] * b[j];
+    b[i] /= A[i * n + i];
+  }
+  return true;
+}
+
+// One SCF step (MODE_ITER) or the core-Hamiltonian guess (MODE_GUESS) of molecule blockIdx.x.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, const BMol *__restrict__ mols,
+                                                   BState *__restrict__ st, int *__restrict__ done, int mode, int diis,
+                                                   double thr, int maxit) {
+  __shared__ double red[2 * (NT / 64)];
+  const int mi = blockIdx.x, tid = threadIdx.x;
+  if (mode == MODE_ITER && done[mi]) return;
+  const BMol M = mols[mi];
+  BState &S = st[mi];
+  const int n = M.n, mp = M.mp;
+  const size_t mm = size_t(mp) * mp;
+  double *base = ws + M.ws;
+  double *H = base + M_H * mm, *F = base + M_F * mm, *P = base + M_P * mm, *Sm = base + M_S * mm;
+  double *X = base + M_X * mm, *XT = base + M_XT * mm, *W = base + M_W * mm, *T = base + M_T * mm;
+  double *FP = base + M_FP * mm, *FPS = base + M_FPS * mm, *Fpr = base + M_FPR * mm, *Xs = base + M_XS * mm;
+  double *X2 = base + M_X2 * mm, *Fh = base + kMats * mm, *Eh = Fh + kHist * mm;
+  const double *A = H;
+
+  if (mode == MODE_GUESS) {
+    if (tid == 0) {
+      S.done = 0, S.it = 0, S.status = 0, S.hist_n = 0, S.cur = 0, S.combine = 0, S.sp2_steps = 0;
+      S.e_old = M.VN, S.e_new = M.VN, S.rms = 0.0, S.dE = 0.0;
+      done[mi] = 0;
+    }
+  } else {
+    // energy of P and F (F = H + G(P) from the grouped J/K)
+    double e = 0.0;
+    for (int x = tid; x < n * n; x += NT) {
+      const int i = x / n, j = x - i * n;
+      const size_t o = size_t(i) * mp + j;
+      e += 0.5 * P[o] * (H[o] + F[o]);
+    }
+    e = block_sum<NT>(e, red) + M.VN;
+    if (diis > 0) {
+      if (tid == 0) {
+        int slot;
+        if (S.hist_n >= diis) {
+          slot = S.hist[0];
+          for (int k = 1; k < S.hist_n; ++k) S.hist[k - 1] = S.hist[k];
+          --S.hist_n;
+        } else {
+          slot = 0;
+          for (bool used = true; used;) {
+            used = false;
+            for (int k = 0; k < S.hist_n; ++k) used |= S.hist[k] == slot;
+            if (used) ++slot;
+          }
+        }
+        S.hist[S.hist_n++] = slot;
+        S.cur = slot;
+      }
+      wg_gemm_nt<NT>(F, P, FP, mp, 1.0);    // F P   (P symmetric)
+      wg_gemm_nt<NT>(FP, Sm, FPS, mp, 1.0);  // F P S (S symmetric)
+      const int cur = S.cur, hn = S.hist_n;
+      double *Ec = Eh + cur * mm, *Fc = Fh + cur * mm;
+      for (size_t x = tid; x < mm; x += NT) {
+        const int i = int(x / mp), j = int(x - size_t(i) * mp);
+        Ec[x] = FPS[x] - FPS[size_t(j) * mp + i];
+        Fc[x] = F[x];
+      }
+      __syncthreads();
+      for (int k = 0; k < hn; ++k) {
+        const double *Ek = Eh + S.hist[k] * mm;
+        double d = 0.0;
+        for (size_t x = tid; x < mm; x += NT) d += Ec[x] * Ek[x];
+        d = block_sum<NT>(d, red);
+        if (tid == 0) S.B[cur * kHist + S.hist[k]] = S.B[S.hist[k] * kHist + cur] = d;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        S.combine = 0;
+        if (hn >= 2) {
+          const int dd = hn + 1;
+          double Am[(kHist + 1) * (kHist + 1)], c[kHist + 1];
+          for (int i = 0; i < dd * dd; ++i) Am[i] = 0.0;
+          for (int i = 0; i < dd; ++i) c[i] = 0.0;
+          for (int i = 0; i < hn; ++i)
+            for (int j = 0; j < hn; ++j) Am[i * dd + j] = S.B[S.hist[i] * kHist + S.hist[j]];
+          for (int i = 0; i < hn; ++i) Am[i * dd + hn] = Am[hn * dd + i] = -1.0;
+          c[hn] = -1.0;
+          if (lu_solve(dd, Am, c)) {
+            for (int i = 0; i < hn; ++i) S.coef[i] = c[i];
+            S.combine = 1;
+          } else {
+            S.hist[0] = cur;
+            S.hist_n = 1;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const int comb = diis > 0 ? S.combine : 0, hn = S.hist_n;
+    for (size_t x = tid; x < mm; x += NT) {
+      const int i = int(x / mp), j = int(x - size_t(i) * mp);
+      if (!comb) {
+        W[x] = F[x];
+      } else if (j >= i) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < hn; ++k) {
+          const double *Fk = Fh + S.hist[k] * mm;
+          a += S.coef[k] * Fk[size_t(i) * mp + j];
+          b += S.coef[k] * Fk[size_t(j) * mp + i];
+        }
+        const double avg = (a + b) / 2;
+        W[size_t(i) * mp + j] = avg;
+        W[size_t(j) * mp + i] = avg;
+      }
+    }
+    if (tid == 0) S.e_new = e;
+    __syncthreads();
+    A = W;
+  }
+
+  // F' = X^T A X
+  wg_gemm_nt<NT>(XT, A, T, mp, 1.0);
+  wg_gemm_nt<NT>(T, XT, Fpr, mp, 1.0);
+  // SP2 on F' (Gershgorin bounds, trace-correcting steps; see zq_dense.hip)
+  double lo = 1e300, hi = -1e300;
+  for (int i = tid; i < n; i += NT) {
+    double rs = 0.0;
+    for (int j = 0; j < n; ++j) rs += fabs(Fpr[size_t(j) * mp + i]);
+    const double dgl = Fpr[size_t(i) * mp + i];
+    rs -= fabs(dgl);
+    lo = fmin(lo, dgl - rs), hi = fmax(hi, dgl + rs);
+  }
+  block_minmax<NT>(lo, hi, red);
+  const double pad = 1e-12 * fmax(fabs(lo), fabs(hi));
+  lo -= pad, hi += pad;
+  const double inv = 1.0 / fmax(hi - lo, 1e-10 * fmax(1.0, fabs(hi)));
+  double t1 = 0.0, t2 = 0.0;
+  for (size_t x = tid; x < mm; x += NT) {
+    const int i = int(x / mp), j = int(x - size_t(i) * mp);
+    double v = 0.0;
+    if (i < n && j < n) v = ((i == j ? hi : 0.0) - 0.5 * (Fpr[x] + Fpr[size_t(j) * mp + i])) * inv;
+    Xs[x] = v;
+    t1 += (i == j) ? v : 0.0;
+    t2 += v * v;
+  }
+  t1 = block_sum<NT>(t1, red);
+  t2 = block_sum<NT>(t2, red);
+  const double N = double(M.nocc);
+  int count = -1, steps = 0;
+  for (; steps < 100; ++steps) {
+    if (count >= 2) break;
+    if (count < 0 && t1 - t2 < 1e-9) count = 0;
+    if (count >= 0) ++count;
+    const bool sq = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
+    wg_gemm_nt<NT>(Xs, Xs, X2, mp, 1.0);
+    double a1 = 0.0, a2 = 0.0;
+    for (size_t x = tid; x < mm; x += NT) {
+      const int i = int(x / mp), j = int(x - size_t(i) * mp);
+      const double v = sq ? X2[x] : 2.0 * Xs[x] - X2[x];
+      Xs[x] = v;
+      a1 += (i == j) ? v : 0.0;
+      a2 += v * v;
+    }
+    t1 = block_sum<NT>(a1, red);
+    t2 = block_sum<NT>(a2, red);
+  }
+  if (count < 2) {  // no gap at the Fermi level: this molecule stops here
+    if (tid == 0) S.status = 3, S.done = 1, done[mi] = 1;
+    return;
+  }
+  // P = 2 X D X^T, rms(dP)
+  wg_gemm_nt<NT>(X, Xs, T, mp, 1.0);
+  wg_gemm_nt<NT>(T, X, FP, mp, 2.0);
+  double s = 0.0;
+  for (size_t x = tid; x < mm; x += NT) {
+    const double dlt = FP[x] - P[x];
+    s += dlt * dlt;
+    P[x] = FP[x];
+  }
+  s = block_sum<NT>(s, red);
+  if (tid == 0) {
+    S.sp2_steps = steps;
+    if (mode == MODE_ITER) {
+      const double rms = sqrt(s / (double(n) * n));
+      const double dE = fabs(S.e_new - S.e_old);
+      S.e_old = S.e_new;
+      S.rms = rms, S.dE = dE;
+      const bool conv = S.it > 0 && dE < thr && rms < thr;
+      ++S.it;
+      if (conv) {
+        S.done = 1, S.status = 1;
+      } else if (S.it >= maxit) {
+        S.done = 1, S.status = 2;
+      }
+      done[mi] = S.done;
+    }
+  }
+}
+
+__global__ void k_batch_count(const int *__restrict__ done, int m, int *__restrict__ out) {
+  __shared__ int red[256];
+  int c = 0;
+  for (int i = threadIdx.x; i < m; i += 256) c += done[i] != 0;
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// F' = X^T W X of every molecule into eig (n x n, leading dimension n, at eoff[m]).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_batch_fprime(double *__restrict__ ws, const BMol *__restrict__ mols,
+                                                     double *__restrict__ eig, const long long *__restrict__ eoff) {
+  const BMol M = mols[blockIdx.x];
+  const int n = M.n, mp = M.mp;
+  const size_t mm = size_t(mp) * mp;
+  double *base = ws + M.ws;
+  double *XT = base + M_XT * mm, *W = base + M_W * mm, *T = base + M_T * mm, *Fpr = base + M_FPR * mm;
+  wg_gemm_nt<NT>(XT, W, T, mp, 1.0);
+  wg_gemm_nt<NT>(T, XT, Fpr, mp, 1.0);
+  double *out = eig + eoff[blockIdx.x];
+  for (int x = threadIdx.x; x < n * n; x += NT) {
+    const int i = x / n, j = x - i * n;
+    out[x] = 0.5 * (Fpr[size_t(i) * mp + j] + Fpr[size_t(j) * mp + i]);
+  }
+}
+
+// C = X V with the eigenvectors of eig (column-major: eigenvector k at eig[eoff + k*n ...]) into
+// cout (n x n, leading dimension n, at coff[m]).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_batch_coef(double *__restrict__ ws, const BMol *__restrict__ mols,
+                                                   const double *__restrict__ eig, const long long *__restrict__ eoff,
+                                                   double *__restrict__ cout, const long long *__restrict__ coff) {
+  const BMol M = mols[blockIdx.x];
+  const int n = M.n, mp = M.mp;
+  const size_t mm = size_t(mp) * mp;
+  double *base = ws + M.ws;
+  double *X = base + M_X * mm, *T = base + M_T * mm, *FP = base + M_FP * mm;
+  const double *V = eig + eoff[blockIdx.x];
+  for (size_t x = threadIdx.x; x < mm; x += NT) {
+    const int k = int(x / mp), p = int(x - size_t(k) * mp);
+    T[x] = (k < n && p < n) ? V[size_t(k) * n + p] : 0.0;
+  }
+  __syncthreads();
+  wg_gemm_nt<NT>(X, T, FP, mp, 1.0);
+  double *out = cout + coff[blockIdx.x];
+  for (int x = threadIdx.x; x < n * n; x += NT) {
+    const int i = x / n, j = x - i * n;
+    out[x] = FP[size_t(i) * mp + j];
+  }
+}
+
+struct HostMol {
+  int n, nocc;
+  double VN;
+  std::vector<double> H, S, eri;
+};
+
+}  // namespace
+
+struct zq_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<HostMol> mols;
+  bool uploaded = false;
+  int chunk = 0, nt = 256;
+  // device residency
+  double *eri = nullptr, *ws = nullptr, *rec = nullptr;
+  BMol *d_mols = nullptr;
+  BState *d_state = nullptr;
+  int *d_done = nullptr, *d_count = nullptr, *d_ptr = nullptr, *d_idx = nullptr;
+  JKItem *d_items = nullptr;
+  JKMol *d_jkm = nullptr;
+  int2 *d_blk = nullptr;
+  int nitems = 0, nblk = 0;
+  std::vector<long long> eri_off, ws_off;
+  int *h_count = nullptr;
+  hipGraphExec_t graph = nullptr;
+  std::string graph_key;
+  rocblas_handle solver = nullptr;
+  // results of the last run
+  std::vector<std::vector<double>> C, e;
+};
+
+namespace {
+
+int bfail(zq_batch *b, int code, const std::string &msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BTRY(b, call)                                                                     \
+  do {                                                                                    \
+    hipError_t _e = (call);                                                               \
+    if (_e != hipSuccess) return bfail(b, ZQ_ERR_LAUNCH, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+void free_plan(zq_batch *b) {
+  for (void *p : {(void *)b->rec, (void *)b->d_items, (void *)b->d_ptr, (void *)b->d_idx, (void *)b->d_blk})
+    if (p) (void)hipFree(p);
+  b->rec = nullptr, b->d_items = nullptr, b->d_ptr = nullptr, b->d_idx = nullptr, b->d_blk = nullptr;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+}
+
+void free_all(zq_batch *b) {
+  free_plan(b);
+  for (void *p : {(void *)b->eri, (void *)b->ws, (void *)b->d_mols, (void *)b->d_state, (void *)b->d_done,
+                  (void *)b->d_count, (void *)b->d_jkm})
+    if (p) (void)hipFree(p);
+  b->eri = nullptr, b->ws = nullptr, b->d_mols = nullptr, b->d_state = nullptr, b->d_done = nullptr;
+  b->d_count = nullptr, b->d_jkm = nullptr;
+  b->uploaded = false;
+}
+
+int mp_of(int n) { return (n + 15) / 16 * 16; }
+int nb_of(int n) { return (n + ZQ_BLK - 1) / ZQ_BLK; }
+
+// The grouped J/K plan for item size `chunk`: items, records and CSR rows of all molecules concatenated.
+int build_plan(zq_batch *b, int chunk) {
+  free_plan(b);
+  std::vector<JKItem> items;
+  std::vector<int> ptr(1, 0), idx;
+  std::vector<int2> blk;
+  size_t rec_base = 0;
+  for (size_t m = 0; m < b->mols.size(); ++m) {
+    const int nb = nb_of(b->mols[m].n), Nb = nb * (nb + 1) / 2;
+    const size_t nq = size_t(Nb) * (Nb + 1) / 2;
+    JKPlan pl;
+    zq_jk_plan(Nb, nq, chunk, pl);
+    for (JKItem it : pl.items) {
+      it.rec0 += int(rec_base);
+      it.mol = int(m);
+      items.push_back(it);
+    }
+    for (int o = 0; o < Nb; ++o) {
+      for (int e = pl.ptr[o]; e < pl.ptr[o + 1]; ++e) {
+        const int code = pl.idx[e];
+        idx.push_back(int((size_t(code >> 1) + rec_base) << 1) | (code & 1));
+      }
+      ptr.push_back(int(idx.size()));
+      blk.push_back(make_int2(int(m), o));
+    }
+    rec_base += pl.nrecords;
+    if (rec_base >= (size_t(1) << 30)) return bfail(b, ZQ_ERR_INVALID, "batch J/K plan: too many records");
+  }
+  b->nitems = int(items.size());
+  b->nblk = int(blk.size());
+  BTRY(b, hipMalloc(&b->rec, std::max<size_t>(1, rec_base) * 64 * sizeof(double)));
+  BTRY(b, hipMalloc(&b->d_items, items.size() * sizeof(JKItem)));
+  BTRY(b, hipMalloc(&b->d_ptr, ptr.size() * sizeof(int)));
+  BTRY(b, hipMalloc(&b->d_idx, std::max<size_t>(1, idx.size()) * sizeof(int)));
+  BTRY(b, hipMalloc(&b->d_blk, blk.size() * sizeof(int2)));
+  BTRY(b, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice));
+  BTRY(b, hipMemcpy(b->d_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!idx.empty()) BTRY(b, hipMemcpy(b->d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  BTRY(b, hipMemcpy(b->d_blk, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice));
+  b->chunk = chunk;
+  return ZQ_OK;
+}
+
+// ERIs into quartets, H/S/X/X^T into the workspace, the per-molecule tables, the plan.
+int upload(zq_batch *b) {
+  if (b->uploaded) return ZQ_OK;
+  if (b->mols.empty()) return bfail(b, ZQ_ERR_STATE, "empty batch");
+  BTRY(b, hipSetDevice(b->device));
+  const size_t M = b->mols.size();
+  b->eri_off.assign(M, 0);
+  b->ws_off.assign(M, 0);
+  size_t eri_total = 0, ws_total = 0, canon_max = 0;
+  std::vector<BMol> bm(M);
+  std::vector<JKMol> jm(M);
+  for (size_t m = 0; m < M; ++m) {
+    const HostMol &h = b->mols[m];
+    const int nb = nb_of(h.n), Nb = nb * (nb + 1) / 2, mp = mp_of(h.n);
+    const size_t nq = size_t(Nb) * (Nb + 1) / 2, mm = size_t(mp) * mp;
+    b->eri_off[m] = (long long)eri_total;
+    b->ws_off[m] = (long long)ws_total;
+    eri_total += nq * ZQ_QUARTET;
+    ws_total += (kMats + 2 * kHist) * mm;
+    canon_max = std::max(canon_max, h.eri.size());
+    bm[m] = BMol{(long long)b->ws_off[m], h.n, mp, h.nocc, 0, h.VN};
+    jm[m] = JKMol{b->eri_off[m], b->ws_off[m] + M_P * (long long)mm, b->ws_off[m] + M_H * (long long)mm,
+                  b->ws_off[m] + M_F * (long long)mm, mp, h.n};
+  }
+  BTRY(b, hipMalloc(&b->eri, eri_total * sizeof(double)));
+  BTRY(b, hipMalloc(&b->ws, ws_total * sizeof(double)));
+  BTRY(b, hipMalloc(&b->d_mols, M * sizeof(BMol)));
+  BTRY(b, hipMalloc(&b->d_state, M * sizeof(BState)));
+  BTRY(b, hipMalloc(&b->d_done, M * sizeof(int)));
+  BTRY(b, hipMalloc(&b->d_count, sizeof(int)));
+  BTRY(b, hipMalloc(&b->d_jkm, M * sizeof(JKMol)));
+  BTRY(b, hipMemset(b->ws, 0, ws_total * sizeof(double)));
+  BTRY(b, hipMemset(b->d_done, 0, M * sizeof(int)));
+  BTRY(b, hipMemcpy(b->d_mols, bm.data(), M * sizeof(BMol), hipMemcpyHostToDevice));
+  BTRY(b, hipMemcpy(b->d_jkm, jm.data(), M * sizeof(JKMol), hipMemcpyHostToDevice));
+  double *canon = nullptr;
+  BTRY(b, hipMalloc(&canon, canon_max * sizeof(double)));
+  for (size_t m = 0; m < M; ++m) {
+    const HostMol &h = b->mols[m];
+    const int nb = nb_of(h.n), Nb = nb * (nb + 1) / 2, mp = mp_of(h.n);
+    const size_t nq = size_t(Nb) * (Nb + 1) / 2, mm = size_t(mp) * mp;
+    hipError_t e = hipMemcpy(canon, h.eri.data(), h.eri.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(zq_pack_quartets, dim3(unsigned(nq)), dim3(256), 0, b->stream, canon, b->eri + b->eri_off[m], h.n,
+                         nq);
+      e = hipStreamSynchronize(b->stream);
+    }
+    // H, S, X = L^-T, X^T, zero padded to mp
+    std::vector<double> stage(4 * mm, 0.0), hX(size_t(h.n) * h.n);
+    if (e == hipSuccess && zq::chol_inv_t(h.n, h.S.data(), hX.data())) {
+      (void)hipFree(canon);
+      return bfail(b, ZQ_ERR_LAPACK, "batch molecule " + std::to_string(m) + ": overlap not positive definite");
+    }
+    for (int i = 0; i < h.n; ++i)
+      for (int j = 0; j < h.n; ++j) {
+        stage[0 * mm + size_t(i) * mp + j] = h.H[size_t(i) * h.n + j];
+        stage[1 * mm + size_t(i) * mp + j] = h.S[size_t(i) * h.n + j];
+        stage[2 * mm + size_t(i) * mp + j] = hX[size_t(i) * h.n + j];
+        stage[3 * mm + size_t(j) * mp + i] = hX[size_t(i) * h.n + j];
+      }
+    double *w = b->ws + b->ws_off[m];
+    if (e == hipSuccess) e = hipMemcpy(w + M_H * mm, stage.data(), mm * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(w + M_S * mm, stage.data() + mm, mm * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(w + M_X * mm, stage.data() + 2 * mm, mm * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(w + M_XT * mm, stage.data() + 3 * mm, mm * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(canon);
+      return bfail(b, ZQ_ERR_LAUNCH, std::string("batch upload: ") + hipGetErrorString(e));
+    }
+  }
+  (void)hipFree(canon);
+  if (int rc = build_plan(b, b->chunk)) return rc;
+  if (!b->h_count) BTRY(b, hipHostMalloc((void **)&b->h_count, 2 * sizeof(int), hipHostMallocDefault));
+  b->uploaded = true;
+  return ZQ_OK;
+}
+
+void launch_step(zq_batch *b, int mode, int diis, double thr, int maxit) {
+  const dim3 grid(unsigned(b->mols.size()));
+  if (b->nt == 512)
+    hipLaunchKernelGGL(k_batch_step<512>, grid, dim3(512), 0, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
+                       diis, thr, maxit);
+  else
+    hipLaunchKernelGGL(k_batch_step<256>, grid, dim3(256), 0, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
+                       diis, thr, maxit);
+}
+
+int enqueue_iteration(zq_batch *b, int diis, double thr, int maxit) {
+  BTRY(b, zq_jk_group(b->stream, b->eri, b->d_items, b->nitems, b->ws, 0.5, b->rec, b->d_jkm, b->d_done, b->d_ptr,
+                      b->d_idx, b->d_blk, b->nblk, b->ws, b->ws));
+  launch_step(b, MODE_ITER, diis, thr, maxit);
+  hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(256), 0, b->stream, b->d_done, int(b->mols.size()), b->d_count);
+  BTRY(b, hipGetLastError());
+  return ZQ_OK;
+}
+
+// A view context on molecule m's resident quartets, for the transform code of zq_transform.hip.
+void bind_view(zq_ctx *v, zq_batch *b, int m) {
+  const int n = b->mols[m].n;
+  v->n = n;
+  v->nb = nb_of(n);
+  v->npad = v->nb * ZQ_BLK;
+  v->Nb = v->nb * (v->nb + 1) / 2;
+  v->nquartets = size_t(v->Nb) * (v->Nb + 1) / 2;
+  v->eri = b->eri + b->eri_off[m];
+}
+
+}  // namespace
+
+extern "C" {
+
+zq_batch *zq_batch_create(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  auto *b = new zq_batch;
+  b->device = device;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+void zq_batch_destroy(zq_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  free_all(b);
+  if (b->h_count) (void)hipHostFree(b->h_count);
+  if (b->solver) rocblas_destroy_handle(b->solver);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *zq_batch_last_error(const zq_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int zq_batch_count(const zq_batch *b) { return b ? int(b->mols.size()) : 0; }
+
+int zq_batch_add(zq_batch *b, size_t n, size_t nocc, const double *H, const double *S, double VN,
+                 const double *eri_packed) {
+  if (!b || !H || !S || !eri_packed) return bfail(b, ZQ_ERR_INVALID, "zq_batch_add: null argument");
+  if (n == 0 || n > size_t(kMaxN) || nocc == 0 || nocc >= n)
+    return bfail(b, ZQ_ERR_INVALID, "zq_batch_add: need 0 < nocc < n <= 96");
+  HostMol h;
+  h.n = int(n), h.nocc = int(nocc), h.VN = VN;
+  h.H.assign(H, H + n * n);
+  h.S.assign(S, S + n * n);
+  const size_t np = n * (n + 1) / 2, nu = np * (np + 1) / 2;
+  h.eri.assign(eri_packed, eri_packed + nu);
+  if (b->uploaded) {
+    free_all(b);  // residency is rebuilt on the next run
+  }
+  b->mols.push_back(std::move(h));
+  return int(b->mols.size()) - 1;
+}
+
+int zq_batch_autotune(zq_batch *b, int chosen[2]) {
+  if (!b) return ZQ_ERR_INVALID;
+  BTRY(b, hipSetDevice(b->device));
+  if (int rc = upload(b)) return rc;
+  hipEvent_t e0, e1;
+  BTRY(b, hipEventCreate(&e0));
+  BTRY(b, hipEventCreate(&e1));
+  auto time_it = [&](auto &&fn) -> float {
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      (void)hipEventRecord(e0, b->stream);
+      fn();
+      (void)hipEventRecord(e1, b->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  int best_chunk = 1;
+  float best = 1e30f;
+  BTRY(b, hipMemset(b->d_done, 0, b->mols.size() * sizeof(int)));
+  for (int ch : {1, 2, 4, 8, 16}) {
+    if (int rc = build_plan(b, ch)) return rc;
+    const float ms = time_it([&] {
+      (void)zq_jk_group(b->stream, b->eri, b->d_items, b->nitems, b->ws, 0.5, b->rec, b->d_jkm, b->d_done, b->d_ptr,
+                        b->d_idx, b->d_blk, b->nblk, b->ws, b->ws);
+    });
+    if (ms < best) best = ms, best_chunk = ch;
+  }
+  int best_nt = 256;
+  best = 1e30f;
+  for (int nt : {256, 512}) {
+    b->nt = nt;
+    const float ms = time_it([&] { launch_step(b, MODE_GUESS, 8, 1e-8, 100); });
+    if (ms < best) best = ms, best_nt = nt;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->nt = best_nt;
+  if (int rc = build_plan(b, best_chunk)) return rc;
+  BTRY(b, hipStreamSynchronize(b->stream));
+  if (chosen) chosen[0] = best_chunk, chosen[1] = best_nt;
+  return ZQ_OK;
+}
+
+int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, double *E_rhf, double *E_corr,
+                 int *iterations, int *status) {
+  if (!b || !opt || !E_rhf || !E_corr || !iterations || !status) return bfail(b, ZQ_ERR_INVALID, "zq_batch_run: null argument");
+  if (opt->generalized || opt->fock_precision != ZQ_PREC_FP64 || opt->diis < 0 || opt->diis > kHist || opt->iterations < 1)
+    return bfail(b, ZQ_ERR_INVALID, "zq_batch_run: RHF, FP64 Fock builds, diis <= 8, iterations >= 1");
+  BTRY(b, hipSetDevice(b->device));
+  if (int rc = upload(b)) return rc;
+  const int M = int(b->mols.size());
+
+  // core-Hamiltonian guess, then the captured iteration until every molecule is done
+  launch_step(b, MODE_GUESS, opt->diis, opt->threshold, opt->iterations);
+  char key[128];
+  snprintf(key, sizeof key, "%d/%d/%d/%a/%d/%p", b->nt, b->chunk, opt->diis, opt->threshold, opt->iterations,
+           (void *)b->d_items);
+  if (!b->graph || b->graph_key != key) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    BTRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_iteration(b, opt->diis, opt->threshold, opt->iterations);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return bfail(b, ZQ_ERR_LAUNCH, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) return bfail(b, ZQ_ERR_LAUNCH, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+    b->graph_key = key;
+  }
+  hipEvent_t ev[2];
+  BTRY(b, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  BTRY(b, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  for (int k = 0; k < opt->iterations; ++k) {
+    BTRY(b, hipGraphLaunch(b->graph, b->stream));
+    BTRY(b, hipMemcpyAsync(b->h_count + (k & 1), b->d_count, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BTRY(b, hipEventRecord(ev[k & 1], b->stream));
+    if (k >= 1) {
+      BTRY(b, hipEventSynchronize(ev[(k - 1) & 1]));
+      if (b->h_count[(k - 1) & 1] >= M) break;
+    }
+  }
+  BTRY(b, hipStreamSynchronize(b->stream));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  std::vector<BState> hs(M);
+  BTRY(b, hipMemcpy(hs.data(), b->d_state, M * sizeof(BState), hipMemcpyDeviceToHost));
+
+  // eigenpairs of X^T F_ext X: molecules ordered by n so that each size is one strided batch
+  std::vector<int> order(M);
+  for (int m = 0; m < M; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->mols[a].n < b->mols[c].n; });
+  std::vector<long long> eoff(M), coff(M), woff(M);
+  long long etot = 0, wtot = 0;
+  for (int m : order) {
+    eoff[m] = etot, woff[m] = wtot;
+    etot += (long long)b->mols[m].n * b->mols[m].n;
+    wtot += b->mols[m].n;
+  }
+  for (int m = 0; m < M; ++m) coff[m] = eoff[m];
+  double *d_eig = nullptr, *d_w = nullptr, *d_e = nullptr, *d_c = nullptr;
+  long long *d_eoff = nullptr;
+  int *d_info = nullptr;
+  BTRY(b, hipMalloc(&d_eig, etot * sizeof(double)));
+  BTRY(b, hipMalloc(&d_c, etot * sizeof(double)));
+  BTRY(b, hipMalloc(&d_w, wtot * sizeof(double)));
+  BTRY(b, hipMalloc(&d_e, wtot * sizeof(double)));
+  BTRY(b, hipMalloc(&d_eoff, M * sizeof(long long)));
+  BTRY(b, hipMalloc(&d_info, M * sizeof(int)));
+  struct Free {
+    std::vector<void *> p;
+    ~Free() {
+      for (void *x : p) (void)hipFree(x);
+    }
+  } guard{{d_eig, d_c, d_w, d_e, d_eoff, d_info}};
+  BTRY(b, hipMemcpy(d_eoff, eoff.data(), M * sizeof(long long), hipMemcpyHostToDevice));
+  if (b->nt == 512)
+    hipLaunchKernelGGL(k_batch_fprime<512>, dim3(M), dim3(512), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff);
+  else
+    hipLaunchKernelGGL(k_batch_fprime<256>, dim3(M), dim3(256), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff);
+  BTRY(b, hipGetLastError());
+  if (!b->solver) {
+    if (rocblas_create_handle(&b->solver) != rocblas_status_success) return bfail(b, ZQ_ERR_LAPACK, "rocblas handle");
+  }
+  rocblas_set_stream(b->solver, b->stream);
+  for (int s = 0; s < M;) {
+    const int n = b->mols[order[s]].n;
+    int t = s;
+    while (t < M && b->mols[order[t]].n == n) ++t;
+    const int cnt = t - s, m0 = order[s];
+    const rocblas_status rs = rocsolver_dsyevd_strided_batched(
+        b->solver, rocblas_evect_original, rocblas_fill_upper, n, d_eig + eoff[m0], n, rocblas_stride(n) * n,
+        d_w + woff[m0], rocblas_stride(n), d_e + woff[m0], rocblas_stride(n), d_info + s, cnt);
+    if (rs != rocblas_status_success) return bfail(b, ZQ_ERR_LAPACK, "rocsolver_dsyevd_strided_batched");
+    s = t;
+  }
+  if (b->nt == 512)
+    hipLaunchKernelGGL(k_batch_coef<512>, dim3(M), dim3(512), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff, d_c, d_eoff);
+  else
+    hipLaunchKernelGGL(k_batch_coef<256>, dim3(M), dim3(256), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff, d_c, d_eoff);
+  BTRY(b, hipGetLastError());
+  std::vector<double> hc(etot), hw(wtot);
+  std::vector<int> info(M);
+  BTRY(b, hipMemcpyAsync(hc.data(), d_c, etot * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BTRY(b, hipMemcpyAsync(hw.data(), d_w, wtot * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BTRY(b, hipMemcpyAsync(info.data(), d_info, M * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BTRY(b, hipStreamSynchronize(b->stream));
+  b->C.assign(M, {});
+  b->e.assign(M, {});
+  for (int m = 0; m < M; ++m) {
+    const int n = b->mols[m].n;
+    b->C[m].assign(hc.begin() + coff[m], hc.begin() + coff[m] + (long long)n * n);
+    b->e[m].assign(hw.begin() + woff[m], hw.begin() + woff[m] + n);
+  }
+  {
+    std::vector<int> by_mol(M);  // rocSOLVER wrote info by batch position; reorder by molecule
+    for (int s = 0; s < M; ++s) by_mol[order[s]] = info[s] ? 1 : 0;
+    info.swap(by_mol);
+  }
+
+  // MP2 on several view contexts at once, each with its own stream
+  const int nworkers = std::min(M, 8);
+  std::vector<zq_ctx *> views(nworkers, nullptr);
+  for (int w = 0; w < nworkers; ++w) {
+    views[w] = zq_create(b->device);
+    if (!views[w]) {
+      for (zq_ctx *v : views)
+        if (v) zq_destroy(v);
+      return bfail(b, ZQ_ERR_LAUNCH, "batch MP2: context creation");
+    }
+  }
+  std::atomic<int> next{0};
+  std::vector<int> rc_m(M, ZQ_OK);
+  std::vector<std::thread> th;
+  for (int w = 0; w < nworkers; ++w)
+    th.emplace_back([&, w] {
+      (void)hipSetDevice(b->device);
+      for (int m; (m = next.fetch_add(1)) < M;) {
+        E_corr[m] = 0.0;
+        if (hs[m].status != 1 && hs[m].status != 2) continue;
+        bind_view(views[w], b, m);
+        rc_m[m] = zq_mp2_device(views[w], b->C[m].data(), b->e[m].data(), b->mols[m].nocc, mp2_precision, &E_corr[m]);
+        views[w]->eri = nullptr;
+      }
+    });
+  for (auto &t : th) t.join();
+  for (zq_ctx *v : views) {
+    v->eri = nullptr;
+    zq_destroy(v);
+  }
+  for (int m = 0; m < M; ++m) {
+    E_rhf[m] = hs[m].e_new;
+    iterations[m] = hs[m].it;
+    status[m] = hs[m].status == 1 ? ZQ_OK : hs[m].status == 2 ? ZQ_ERR_NOT_CONVERGED : ZQ_ERR_LAPACK;
+    if (info[m] && status[m] == ZQ_OK) status[m] = ZQ_ERR_LAPACK;
+    if (rc_m[m] != ZQ_OK && status[m] == ZQ_OK) status[m] = rc_m[m];
+  }
+  return ZQ_OK;
+}
+
+int zq_batch_orbitals(const zq_batch *b, int index, double *e, double *C) {
+  if (!b || index < 0 || size_t(index) >= b->C.size() || !e || !C) return ZQ_ERR_INVALID;
+  std::copy(b->e[index].begin(), b->e[index].end(), e);
+  std::copy(b->C[index].begin(), b->C[index].end(), C);
+  return ZQ_OK;
+}
+
+}  // extern "C"

@@ -47,7 +47,25 @@ __host__ __device__ inline void zq_unpair(int I, int &a, int &b) {
 
 struct JKItem {
   int Ib, bp, bq, k0, k1, rec0;
+  int mol;  // molecule of a grouped plan (0 for a single molecule)
 };
+
+// Per-molecule table of a grouped J/K build: quartets at eri + eri_off, P at P + p_off, H at
+// H + h_off, F at F + f_off, the three matrices with leading dimension ld (zero padded beyond n).
+struct JKMol {
+  long long eri_off, p_off, h_off, f_off;
+  int ld, n;
+};
+
+// Work plan of one molecule's J/K build: items, the number of 64-double records they emit, and the CSR
+// list (per output block x >= y) of record codes r << 1 | transposed.
+struct JKPlan {
+  std::vector<JKItem> items;
+  std::vector<int> ptr, idx;
+  size_t nrecords = 0;
+};
+// chunk = quartets per item (<= 0: the default for this size)
+void zq_jk_plan(int Nb, size_t nquartets, int chunk, JKPlan &plan);  // zq_jk.hip
 
 struct DevBuf {
   void *ptr = nullptr;
@@ -110,6 +128,16 @@ void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip, also drops the captured
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF,
                       const int *skip = nullptr, bool f32 = false);  // zq_jk.hip
 int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, made once per load
+// MP2 correlation energy of the resident ERIs (zq_transform.hip); C and e are host arrays.
+int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr);
+// Grouped J/K on `stream`: F = H + G(P) (x_exch) for every molecule of a grouped plan whose done flag
+// is 0.  blk[g] = (molecule, output block) for the nblk CSR rows.  Returns the launch status.
+hipError_t zq_jk_group(hipStream_t stream, const double *eri, const JKItem *items, int nitems, const double *P,
+                       double x_exch, double *rec, const JKMol *mols, const int *done, const int *csr_ptr,
+                       const int *csr_idx, const int2 *blk, int nblk, const double *H, double *F);
+// Packs canonical chemist ERIs (zq_eri_load_packed layout) into 8x8 AO-block quartets (zq_context.hip).
+__global__ __launch_bounds__(256) void zq_pack_quartets(const double *__restrict__ canon, double *__restrict__ out, int n,
+                                                        size_t nquartets);
 int zq_dgemm(zq_ctx *ctx, const double *A, long long sAm, long long sAk, const double *B, long long sBk, long long sBn,
              double *C, long long sCm, long long sCn, int M, int N, int K, const char *name);  // zq_transform.hip
 // C = alpha A B^T (A: M x K, B: N x K, row-major with leading dimensions), latency-sized FP64 MFMA GEMM.

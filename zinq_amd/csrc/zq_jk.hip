@@ -48,13 +48,23 @@ __device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
 // vmcnt wait never stalls on them); density values common to the wave come in through scalar loads.
 // Cross-wave sums (K[p,s], K[q,s] at the end of a bs run, J[p,q] at the end of the item) go through
 // LDS in a fixed order.
-template <class E>
+// GROUP: the item's molecule selects the quartets, P and its leading dimension from mols[], and the
+// item is skipped once done[molecule] is set.
+template <class E, bool GROUP>
 __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, const JKItem *__restrict__ items,
                                                      int nitems, const double *__restrict__ P, int npad, double xexch,
-                                                     double *__restrict__ rec, const int *__restrict__ skip) {
+                                                     double *__restrict__ rec, const int *__restrict__ skip,
+                                                     const JKMol *__restrict__ mols, const int *__restrict__ done) {
   __shared__ double red[4][2][64];
   const int item_id = blockIdx.x;
   if (item_id >= nitems || (skip && *skip)) return;
+  if (GROUP) {
+    const int m = items[item_id].mol;
+    if (done && done[m]) return;
+    eri += mols[m].eri_off;
+    P += mols[m].p_off;
+    npad = mols[m].ld;
+  }
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int Ib = items[item_id].Ib, bp = items[item_id].bp, bq = items[item_id].bq;
@@ -186,16 +196,30 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
 // F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).  Four waves
 // take contiguous quarters of the block's record list with 8 loads in flight; the quarters are added in
 // a fixed order, so the result is bitwise reproducible.
+// GROUP: blk[blockIdx.x] = (molecule, its output block); H, F, n and the leading dimension come from
+// mols[].  Otherwise H and F are n x n with leading dimension ldf and blockIdx.x is the output block.
+template <bool GROUP>
 __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
                                                     const int *__restrict__ csr_idx, const double *H, int n, double *F,
-                                                    const int *__restrict__ skip) {
+                                                    const int *__restrict__ skip, int ldf, const JKMol *__restrict__ mols,
+                                                    const int2 *__restrict__ blk, const int *__restrict__ done) {
   __shared__ double part[4][64];
   if (skip && *skip) return;
-  const int o = blockIdx.x;
+  int o = blockIdx.x;
+  if (GROUP) {
+    const int2 b = blk[blockIdx.x];
+    if (done && done[b.x]) return;
+    o = b.y;
+    H += mols[b.x].h_off;
+    F += mols[b.x].f_off;
+    n = mols[b.x].n;
+    ldf = mols[b.x].ld;
+  }
+  const int g = blockIdx.x;  // row of the CSR (global over a group)
   int x, y;
   zq_unpair(o, x, y);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i = lane >> 3, j = lane & 7;
-  const int e0 = csr_ptr[o], e1 = csr_ptr[o + 1], len = e1 - e0;
+  const int e0 = csr_ptr[g], e1 = csr_ptr[g + 1], len = e1 - e0;
   const int a0 = e0 + (len * wave) / 4, a1 = e0 + (len * (wave + 1)) / 4;
   double acc = 0.0;
   int e = a0;
@@ -219,24 +243,23 @@ __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ r
   __syncthreads();
   if (wave != 0) return;
   const int gi = x * 8 + i, gj = y * 8 + j;
-  const double h = (gi < n && gj < n) ? H[size_t(gi) * n + gj] : 0.0;
+  const double h = (gi < n && gj < n) ? H[size_t(gi) * ldf + gj] : 0.0;
   const double f = h + (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
   // on diagonal blocks lane (i,j) and lane (j,i) sum the same terms in different orders: let only
   // i >= j write, so both triangles hold one value
   if (gi < n && gj < n && (x != y || i >= j)) {
-    F[size_t(gi) * n + gj] = f;
-    F[size_t(gj) * n + gi] = f;
+    F[size_t(gi) * ldf + gj] = f;
+    F[size_t(gj) * ldf + gi] = f;
   }
 }
 
 }  // namespace
 
 // Host-side work plan: work items, the exact record sequence each emits, and the per-output-block CSR.
-int zq_build_jk_plan(zq_ctx *ctx) {
-  const int nb = ctx->nb, Nb = ctx->Nb;
-  const size_t nq = ctx->nquartets;
-  const int CH = int(std::max<size_t>(1, std::min<size_t>(16, nq / 3072)));
-  std::vector<JKItem> items;
+void zq_jk_plan(int Nb, size_t nq, int chunk, JKPlan &plan) {
+  const int CH = chunk > 0 ? chunk : int(std::max<size_t>(1, std::min<size_t>(16, nq / 3072)));
+  std::vector<JKItem> &items = plan.items;
+  items.clear();
   std::vector<std::pair<int, int>> tgt;
   for (int Ib = 0; Ib < Nb; ++Ib) {
     int bp, bq;
@@ -244,7 +267,7 @@ int zq_build_jk_plan(zq_ctx *ctx) {
     const int total = Ib + 1;
     for (int k0 = 0; k0 < total; k0 += CH) {
       const int k1 = std::min(total, k0 + CH);
-      items.push_back({Ib, bp, bq, k0, k1, int(tgt.size())});
+      items.push_back({Ib, bp, bq, k0, k1, int(tgt.size()), 0});
       int bs = 0, kk = k0;
       for (;; ++bs) {
         const int len = zq_run_len(bp, bq, bs);
@@ -272,7 +295,6 @@ int zq_build_jk_plan(zq_ctx *ctx) {
       tgt.push_back({bp, bq});
     }
   }
-  if (tgt.size() >= (size_t(1) << 30)) return zq_fail(ctx, ZQ_ERR_INVALID, "too many J/K records");
   // CSR over output blocks x >= y
   std::vector<std::vector<int>> lists(Nb);
   for (size_t r = 0; r < tgt.size(); ++r) {
@@ -283,17 +305,26 @@ int zq_build_jk_plan(zq_ctx *ctx) {
     const int tx = tgt[r].first, ty = tgt[r].second;
     if (ty >= tx) lists[zq_bpair(ty, tx)].push_back((int(r) << 1) | 1);
   }
-  std::vector<int> ptr(Nb + 1, 0), idx;
+  plan.ptr.assign(Nb + 1, 0);
+  plan.idx.clear();
   for (int o = 0; o < Nb; ++o) {
-    ptr[o + 1] = ptr[o] + int(lists[o].size());
-    idx.insert(idx.end(), lists[o].begin(), lists[o].end());
+    plan.ptr[o + 1] = plan.ptr[o] + int(lists[o].size());
+    plan.idx.insert(plan.idx.end(), lists[o].begin(), lists[o].end());
   }
-  (void)nb;
+  plan.nrecords = tgt.size();
+}
+
+int zq_build_jk_plan(zq_ctx *ctx) {
+  JKPlan plan;
+  zq_jk_plan(ctx->Nb, ctx->nquartets, 0, plan);
+  if (plan.nrecords >= (size_t(1) << 30)) return zq_fail(ctx, ZQ_ERR_INVALID, "too many J/K records");
+  const std::vector<JKItem> &items = plan.items;
+  const std::vector<int> &ptr = plan.ptr, &idx = plan.idx;
   ctx->nitems = int(items.size());
-  ctx->nrecords = tgt.size();
+  ctx->nrecords = plan.nrecords;
   const size_t npad2 = size_t(ctx->npad) * ctx->npad;
   ZQ_TRY(ctx, hipMalloc(&ctx->d_items, items.size() * sizeof(JKItem)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_records, tgt.size() * 64 * sizeof(double)));
+  ZQ_TRY(ctx, hipMalloc(&ctx->d_records, plan.nrecords * 64 * sizeof(double)));
   ZQ_TRY(ctx, hipMalloc(&ctx->d_csr_ptr, ptr.size() * sizeof(int)));
   ZQ_TRY(ctx, hipMalloc(&ctx->d_csr_idx, std::max<size_t>(1, idx.size()) * sizeof(int)));
   ZQ_TRY(ctx, hipMalloc(&ctx->d_P, npad2 * sizeof(double)));
@@ -316,19 +347,31 @@ int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_
   if (f32 && !ctx->eri32) return zq_fail(ctx, ZQ_ERR_STATE, "zq_fock_jk_device: no FP32 ERI copy");
   zq_prof_begin(ctx, f32 ? "jk_records32" : "jk_records");
   if (f32)
-    hipLaunchKernelGGL(zq_jk_records<float>, dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream, ctx->eri32,
-                       ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records, skip);
+    hipLaunchKernelGGL((zq_jk_records<float, false>), dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream,
+                       ctx->eri32, ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records, skip, nullptr,
+                       nullptr);
   else
-    hipLaunchKernelGGL(zq_jk_records<double>, dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream, ctx->eri,
-                       ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records, skip);
+    hipLaunchKernelGGL((zq_jk_records<double, false>), dim3(unsigned(ctx->nitems)), dim3(256), 0, ctx->stream,
+                       ctx->eri, ctx->d_items, ctx->nitems, dP, ctx->npad, x_exch, ctx->d_records, skip, nullptr,
+                       nullptr);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_records");
   zq_prof_begin(ctx, "jk_reduce");
-  hipLaunchKernelGGL(zq_jk_reduce, dim3(unsigned(ctx->Nb)), dim3(256), 0, ctx->stream, ctx->d_records, ctx->d_csr_ptr,
-                     ctx->d_csr_idx, dH, ctx->n, dF, skip);
+  hipLaunchKernelGGL(zq_jk_reduce<false>, dim3(unsigned(ctx->Nb)), dim3(256), 0, ctx->stream, ctx->d_records,
+                     ctx->d_csr_ptr, ctx->d_csr_idx, dH, ctx->n, dF, skip, ctx->n, nullptr, nullptr, nullptr);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_reduce");
   return ZQ_OK;
+}
+
+hipError_t zq_jk_group(hipStream_t stream, const double *eri, const JKItem *items, int nitems, const double *P,
+                       double x_exch, double *rec, const JKMol *mols, const int *done, const int *csr_ptr,
+                       const int *csr_idx, const int2 *blk, int nblk, const double *H, double *F) {
+  hipLaunchKernelGGL((zq_jk_records<double, true>), dim3(unsigned(nitems)), dim3(256), 0, stream, eri, items, nitems, P,
+                     0, x_exch, rec, nullptr, mols, done);
+  hipLaunchKernelGGL(zq_jk_reduce<true>, dim3(unsigned(nblk)), dim3(256), 0, stream, rec, csr_ptr, csr_idx, H, 0, F,
+                     nullptr, 0, mols, blk, done);
+  return hipGetLastError();
 }
 
 namespace {

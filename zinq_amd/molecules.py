@@ -72,3 +72,22 @@ def to_xyz(Z, xyz, comment=""):
 def water_system(k, basis="builtin:cc-pvdz", **kw):
     Z, xyz = water_cluster(k, **kw)
     return MolecularSystem.from_atoms(Z, xyz / BOHR_TO_ANGSTROM, basis)
+
+
+BATCH_BASES = ("builtin:sto-3g", "builtin:cc-pvdz")
+
+
+def batch_specs(count=512, seed=4242, kmax=3):
+    """Seeded molecule list of the batched workload: k ~ U{1..kmax} waters, basis ~ U{STO-3G, cc-pVDZ},
+    each monomer with +-0.05 Angstrom bond and +-5 degree angle distortions (own seed per molecule)."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        k = 1 + min(kmax - 1, int(rng.uniform() * kmax))
+        basis = BATCH_BASES[min(1, int(rng.uniform() * 2))]
+        out.append(dict(k=k, basis=basis, seed=int(rng.next() & 0xFFFFFFFF)))
+    return out
+
+
+def batch_system(spec):
+    return water_system(spec["k"], basis=spec["basis"], seed=spec["seed"], bond_jitter=0.05, angle_jitter=5.0)

@@ -90,6 +90,21 @@ int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches);
 /* Device memory held by the context, in bytes (ERIs + scratch). */
 size_t zq_device_bytes(const zq_ctx *ctx);
 
+/* Resident ERI sets for serving: the quartets of one molecule (and, with with_fp32, their FP32 copy for
+ * the mixed Fock path) stay in HBM independently of any context, and several contexts -- one per
+ * concurrent request, each with its own stream, J/K plan and scratch -- compute on them at once.
+ * zq_eri_set_create takes the zq_eri_load_packed layout; NULL on failure.  zq_eri_set_bytes is the
+ * set's HBM footprint.  zq_attach points ctx at the set (no copy; the set must outlive the attachment,
+ * which ends with the next zq_attach, zq_eri_load* or zq_destroy of ctx). */
+typedef struct zq_eri_set zq_eri_set;
+zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int with_fp32);
+void zq_eri_set_destroy(zq_eri_set *s);
+size_t zq_eri_set_bytes(const zq_eri_set *s);
+int zq_attach(zq_ctx *ctx, zq_eri_set *s);
+
+/* Free and total HBM of `device` in bytes (for admission control). */
+int zq_mem_info(int device, size_t *free_bytes, size_t *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -319,3 +319,60 @@ def test_batch_autotune_and_rerun():
     np.testing.assert_allclose(r1["E_rhf"], r0["E_rhf"], atol=1e-11)
     np.testing.assert_allclose(r1["E_corr"], r0["E_corr"], atol=1e-11)
     b.close()
+
+
+# ---------------------------------------------------------------- serving
+def _serve_molecules():
+    from zinq_amd.molecules import water_system
+    from zinq_amd.serve import Molecule
+    return [Molecule(f"w{k}", water_system(k)) for k in (1, 2, 3)] + \
+           [Molecule("s2", water_system(2, basis="builtin:sto-3g"))]
+
+
+def test_serve_same_alone_and_in_the_mix():
+    """Requests overlapping on 4 slots (continuous batching) give bitwise the result each molecule gets
+    when served alone; also under a budget so tight that idle resident sets are evicted and rebuilt."""
+    from zinq_amd.serve import Server, poisson_arrivals, eri_set_bytes, scratch_bytes
+    mols = _serve_molecules()
+    keys = [m.key for m in mols]
+    alone = {}
+    for m in mols:
+        srv = Server(mols, slots=1)
+        rec, _ = srv.serve([(0.0, m.key)])
+        srv.close()
+        assert "error" not in rec[0], rec[0]
+        alone[m.key] = rec[0]
+    reqs = poisson_arrivals(500.0, 16, keys, seed=3)
+    srv = Server(mols, slots=4)
+    rec, wall = srv.serve(reqs)
+    srv.close()
+    assert len({r["slot"] for r in rec}) > 1
+    for r in rec:
+        assert "error" not in r, r
+        a = alone[r["key"]]
+        assert r["E_mp2"] == a["E_mp2"] and r["iterations"] == a["iterations"], (r, a)
+    big = max(eri_set_bytes(m.n, True) for m in mols)
+    tight = 2 * max(scratch_bytes(m.n, m.o, 4) for m in mols) + int(1.5 * big)
+    srv = Server(mols, slots=2, hbm_budget=tight)
+    rec, _ = srv.serve(poisson_arrivals(500.0, 10, keys, seed=4))
+    srv.close()
+    for r in rec:
+        assert "error" not in r, r
+        assert r["E_mp2"] == alone[r["key"]]["E_mp2"]
+
+
+def test_serve_mixed_results_within_bar():
+    """Served mixed-path energies against the plain FP64 device path: |dE_MP2| <= 1e-6 Eh."""
+    from zinq_amd import hip
+    from zinq_amd.serve import Server
+    mols = _serve_molecules()[:3]
+    srv = Server(mols, slots=3)
+    rec, _ = srv.serve([(0.0, m.key) for m in mols])
+    srv.close()
+    with hip.Context(0) as ctx:
+        for m, r in zip(mols, rec):
+            ctx.eri_load_packed(m.packed, m.n)
+            ref = ctx.scf(m.H, m.S, m.VN, m.o, threshold=1e-10, device=True)
+            e2 = ctx.mp2(ref["C"], ref["e"], m.o)
+            print(f"{m.key}: served dE_MP2 = {r['E_mp2'] - ref['E'] - e2:+.2e}")
+            assert abs(r["E_mp2"] - ref["E"] - e2) < 1e-6

@@ -88,8 +88,11 @@ static void prof_collect(zq_ctx *ctx) {
 }
 
 void zq_free_eri(zq_ctx *ctx) {
-  for (void *p : {(void *)ctx->eri, (void *)ctx->eri32, (void *)ctx->d_items, (void *)ctx->d_records,
-                  (void *)ctx->d_csr_ptr, (void *)ctx->d_csr_idx, (void *)ctx->d_P, (void *)ctx->d_H, (void *)ctx->d_F})
+  if (ctx->eri && ctx->eri_owned) (void)hipFree(ctx->eri);
+  if (ctx->eri32 && ctx->eri32_owned) (void)hipFree(ctx->eri32);
+  ctx->eri_owned = ctx->eri32_owned = true;
+  for (void *p : {(void *)ctx->d_items, (void *)ctx->d_records, (void *)ctx->d_csr_ptr, (void *)ctx->d_csr_idx,
+                  (void *)ctx->d_P, (void *)ctx->d_H, (void *)ctx->d_F})
     if (p) (void)hipFree(p);
   ctx->eri = nullptr, ctx->eri32 = nullptr, ctx->d_items = nullptr, ctx->d_records = nullptr;
   ctx->d_csr_ptr = nullptr, ctx->d_csr_idx = nullptr;
@@ -252,10 +255,85 @@ int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches) {
   return ZQ_OK;
 }
 
+zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int with_fp32) {
+  int count = 0;
+  if (!g_packed || n == 0 || n > 4096 || hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  auto *s = new zq_eri_set;
+  s->device = device;
+  s->n = int(n);
+  s->nb = int((n + ZQ_BLK - 1) / ZQ_BLK);
+  s->npad = s->nb * ZQ_BLK;
+  s->Nb = s->nb * (s->nb + 1) / 2;
+  s->nquartets = size_t(s->Nb) * (s->Nb + 1) / 2;
+  const size_t np = n * (n + 1) / 2, nu = np * (np + 1) / 2, cnt = s->nquartets * ZQ_QUARTET;
+  double *d_canon = nullptr;
+  hipStream_t st = nullptr;
+  bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&d_canon, nu * sizeof(double)) == hipSuccess && hipMalloc(&s->eri, cnt * sizeof(double)) == hipSuccess &&
+            hipMemcpyAsync(d_canon, g_packed, nu * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(zq_pack_quartets, dim3(unsigned(s->nquartets)), dim3(256), 0, st, d_canon, s->eri, int(n),
+                       s->nquartets);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  if (ok && with_fp32) {
+    ok = hipMalloc(&s->eri32, cnt * sizeof(float)) == hipSuccess;
+    if (ok) {
+      zq_narrow_eri(st, s->eri, cnt, s->eri32);
+      ok = hipGetLastError() == hipSuccess;
+    }
+  }
+  if (st) ok = (hipStreamSynchronize(st) == hipSuccess) && ok;
+  if (d_canon) (void)hipFree(d_canon);
+  if (st) (void)hipStreamDestroy(st);
+  if (!ok) {
+    (void)hipGetLastError();
+    zq_eri_set_destroy(s);
+    return nullptr;
+  }
+  return s;
+}
+
+int zq_mem_info(int device, size_t *free_bytes, size_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return ZQ_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return ZQ_ERR_INVALID;
+  return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? ZQ_OK : ZQ_ERR_LAUNCH;
+}
+
+void zq_eri_set_destroy(zq_eri_set *s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->eri) (void)hipFree(s->eri);
+  if (s->eri32) (void)hipFree(s->eri32);
+  delete s;
+}
+
+size_t zq_eri_set_bytes(const zq_eri_set *s) {
+  if (!s) return 0;
+  return s->nquartets * ZQ_QUARTET * (sizeof(double) + (s->eri32 ? sizeof(float) : 0));
+}
+
+int zq_attach(zq_ctx *ctx, zq_eri_set *s) {
+  if (!ctx || !s) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_attach: null argument");
+  if (s->device != ctx->device) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_attach: set and context on different devices");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->eri == s->eri && ctx->n == s->n) return ZQ_OK;  // already attached: plan and buffers stay
+  zq_free_eri(ctx);
+  ++ctx->eri_gen;
+  ctx->n = s->n, ctx->nb = s->nb, ctx->npad = s->npad, ctx->Nb = s->Nb, ctx->nquartets = s->nquartets;
+  ctx->eri = s->eri, ctx->eri_owned = false;
+  ctx->eri32 = s->eri32, ctx->eri32_owned = s->eri32 == nullptr;
+  return zq_build_jk_plan(ctx);
+}
+
 size_t zq_device_bytes(const zq_ctx *ctx) {
   if (!ctx) return 0;
-  size_t b = ctx->nquartets * ZQ_QUARTET * sizeof(double) + ctx->nrecords * 64 * sizeof(double);
-  if (ctx->eri32) b += ctx->nquartets * ZQ_QUARTET * sizeof(float);
+  size_t b = ctx->eri_owned ? ctx->nquartets * ZQ_QUARTET * sizeof(double) : 0;
+  b += ctx->nrecords * 64 * sizeof(double);
+  if (ctx->eri32 && ctx->eri32_owned) b += ctx->nquartets * ZQ_QUARTET * sizeof(float);
   for (auto &kv : ctx->scratch) b += kv.second.bytes;
   return b;
 }

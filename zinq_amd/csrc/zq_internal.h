@@ -82,6 +82,7 @@ struct zq_ctx {
   size_t nquartets = 0;
   double *eri = nullptr;
   float *eri32 = nullptr;  // FP32 copy for the mixed-precision Fock builds (zq_eri32), same layout
+  bool eri_owned = true, eri32_owned = true;  // false while attached to a shared zq_eri_set
 
   // J/K work decomposition
   JKItem *d_items = nullptr;
@@ -116,6 +117,16 @@ struct zq_ctx {
   std::map<std::string, std::pair<double, int>> prof;
 };
 
+// Resident ERIs of one molecule, shared read-only by the contexts attached to it (zq_attach).
+struct zq_eri_set {
+  int device = 0;
+  int n = 0, nb = 0, npad = 0, Nb = 0;
+  size_t nquartets = 0;
+  double *eri = nullptr;
+  float *eri32 = nullptr;
+  std::string err;
+};
+
 // ---- helpers implemented in zq_context.hip
 int zq_fail(zq_ctx *ctx, int code, const std::string &msg);
 int zq_check(zq_ctx *ctx, hipError_t e, const char *what);
@@ -128,6 +139,7 @@ void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip, also drops the captured
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF,
                       const int *skip = nullptr, bool f32 = false);  // zq_jk.hip
 int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, made once per load
+void zq_narrow_eri(hipStream_t stream, const double *eri, size_t cnt, float *out);  // zq_jk.hip
 // MP2 correlation energy of the resident ERIs (zq_transform.hip); C and e are host arrays.
 int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr);
 // Grouped J/K on `stream`: F = H + G(P) (x_exch) for every molecule of a grouped plan whose done flag

@@ -381,13 +381,18 @@ __global__ void zq_narrow(const double *__restrict__ x, size_t n, float *__restr
 }
 }  // namespace
 
+void zq_narrow_eri(hipStream_t stream, const double *eri, size_t cnt, float *out) {
+  hipLaunchKernelGGL(zq_narrow, dim3(4096), dim3(256), 0, stream, eri, cnt, out);
+}
+
 // FP32 copy of the resident quartets (round to nearest), made once per ERI load.
 int zq_eri32(zq_ctx *ctx) {
   if (ctx->eri32) return ZQ_OK;
   if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "zq_eri32: no ERIs loaded");
   const size_t cnt = ctx->nquartets * ZQ_QUARTET;
   ZQ_TRY(ctx, hipMalloc(&ctx->eri32, cnt * sizeof(float)));
-  hipLaunchKernelGGL(zq_narrow, dim3(4096), dim3(256), 0, ctx->stream, ctx->eri, cnt, ctx->eri32);
+  ctx->eri32_owned = true;
+  zq_narrow_eri(ctx->stream, ctx->eri, cnt, ctx->eri32);
   ZQ_LAUNCH_CHECK(ctx, "zq_narrow");
   return ZQ_OK;
 }

@@ -16,7 +16,8 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32c": 3}
 
 EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_load_packed", "zq_fock_jk", "zq_ao2mo",
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
-            "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget"]
+            "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
+            "zq_eri_set_bytes", "zq_attach", "zq_mem_info"]
 
 
 class GpuError(RuntimeError):
@@ -83,6 +84,13 @@ def lib():
         L.zq_host_sygv.argtypes = [sz, vp, vp, vp]
         L.zq_scf_run_device.argtypes = L.zq_scf_run.argtypes
         L.zq_scf_set_sp2_budget.argtypes = [vp, ci]
+        L.zq_eri_set_create.restype = vp
+        L.zq_eri_set_create.argtypes = [ci, vp, sz, ci]
+        L.zq_eri_set_destroy.argtypes = [vp]
+        L.zq_eri_set_bytes.restype = sz
+        L.zq_eri_set_bytes.argtypes = [vp]
+        L.zq_attach.argtypes = [vp, vp]
+        L.zq_mem_info.argtypes = [ci, vp, vp]
         L._zq_typed = True
     return L
 
@@ -99,6 +107,36 @@ def host_sygv(A, B):
     if rc:
         raise LapackError("generalized eigensolver failed")
     return w, A
+
+
+def mem_info(device=0):
+    """(free, total) HBM bytes of the device (zq_mem_info)."""
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if lib().zq_mem_info(device, ctypes.byref(free), ctypes.byref(total)):
+        raise GpuError("zq_mem_info failed")
+    return free.value, total.value
+
+
+class EriSet:
+    """Resident ERIs of one molecule shared by any number of contexts (zq_eri_set_create)."""
+
+    def __init__(self, packed, n, device=0, with_fp32=False):
+        self._L = lib()
+        pk = _f64(packed)
+        h = self._L.zq_eri_set_create(device, ptr(pk), n, int(bool(with_fp32)))
+        if not h:
+            raise GpuOutOfMemory(f"zq_eri_set_create(n={n}) failed")
+        self._h = ctypes.c_void_p(h)
+        self.n = n
+        self.nbytes = int(self._L.zq_eri_set_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.zq_eri_set_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Context:
@@ -143,6 +181,11 @@ class Context:
             raise InvalidInput("packed ERI array has the wrong length")
         self._check(self._L.zq_eri_load_packed(self._h, ptr(pk), n))
         self.n = n
+
+    def attach(self, eri_set):
+        """Compute on a shared resident ERI set from now on (zq_attach)."""
+        self._check(self._L.zq_attach(self._h, eri_set._h))
+        self.n = eri_set.n
 
     def fock_jk(self, H, P, x_exch=0.5):
         H, P = _f64(H), _f64(P)

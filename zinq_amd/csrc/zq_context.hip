@@ -8,6 +8,8 @@
 #include <cstring>
 #include <thread>
 
+#include <shared_mutex>
+
 #include "zq_internal.h"
 
 int zq_build_jk_plan(zq_ctx *ctx);  // zq_jk.hip
@@ -21,6 +23,35 @@ int zq_check(zq_ctx *ctx, hipError_t e, const char *what) {
   if (e == hipSuccess) return ZQ_OK;
   const int code = (e == hipErrorOutOfMemory) ? ZQ_ERR_OOM : ZQ_ERR_LAUNCH;
   return zq_fail(ctx, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+std::shared_mutex g_gate;
+thread_local int tl_depth = 0;     // shared holds of this thread (outermost one locks)
+thread_local bool tl_excl = false;  // this thread is inside a capture window
+}  // namespace
+
+ZqGate::ZqGate() {
+  if (!tl_excl && tl_depth++ == 0) g_gate.lock_shared();
+}
+ZqGate::~ZqGate() {
+  if (!tl_excl && --tl_depth == 0) g_gate.unlock_shared();
+}
+ZqCaptureWindow::ZqCaptureWindow() : saved(tl_depth) {
+  if (saved) {
+    tl_depth = 0;
+    g_gate.unlock_shared();
+  }
+  g_gate.lock();
+  tl_excl = true;
+}
+ZqCaptureWindow::~ZqCaptureWindow() {
+  tl_excl = false;
+  g_gate.unlock();
+  if (saved) {
+    g_gate.lock_shared();
+    tl_depth = saved;
+  }
 }
 
 void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh) {
@@ -172,6 +203,7 @@ static int load_canonical(zq_ctx *ctx, const double *canon, size_t n) {
 extern "C" {
 
 zq_ctx *zq_create(int device) {
+  ZqGate _gate;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
   if (hipSetDevice(device) != hipSuccess) return nullptr;
@@ -185,6 +217,7 @@ zq_ctx *zq_create(int device) {
 }
 
 void zq_destroy(zq_ctx *ctx) {
+  ZqGate _gate;
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
@@ -201,9 +234,13 @@ void zq_destroy(zq_ctx *ctx) {
 
 const char *zq_last_error(const zq_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int zq_eri_load_packed(zq_ctx *ctx, const double *g_packed, size_t n) { return load_canonical(ctx, g_packed, n); }
+int zq_eri_load_packed(zq_ctx *ctx, const double *g_packed, size_t n) {
+  ZqGate _gate;
+  return load_canonical(ctx, g_packed, n);
+}
 
 int zq_eri_load(zq_ctx *ctx, const double *g, size_t n) {
+  ZqGate _gate;
   if (!ctx || !g || n == 0 || n > 4096) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_eri_load: bad arguments");
   // dense physicist <pr|qs> = (pq|rs) -> canonical packed chemist, in parallel over bra pairs
   const size_t np = n * (n + 1) / 2, nu = np * (np + 1) / 2;
@@ -233,16 +270,19 @@ int zq_eri_load(zq_ctx *ctx, const double *g, size_t n) {
 }
 
 void zq_profile_enable(zq_ctx *ctx, int on) {
+  ZqGate _gate;
   if (ctx) ctx->profile = on != 0;
 }
 
 void zq_profile_reset(zq_ctx *ctx) {
+  ZqGate _gate;
   if (!ctx) return;
   prof_collect(ctx);
   ctx->prof.clear();
 }
 
 int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches) {
+  ZqGate _gate;
   if (!ctx || !prefix) return ZQ_ERR_INVALID;
   prof_collect(ctx);
   double t = 0;
@@ -256,6 +296,7 @@ int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches) {
 }
 
 zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int with_fp32) {
+  ZqGate _gate;
   int count = 0;
   if (!g_packed || n == 0 || n > 4096 || hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
     return nullptr;
@@ -297,12 +338,14 @@ zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int 
 }
 
 int zq_mem_info(int device, size_t *free_bytes, size_t *total_bytes) {
+  ZqGate _gate;
   if (!free_bytes || !total_bytes) return ZQ_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return ZQ_ERR_INVALID;
   return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? ZQ_OK : ZQ_ERR_LAUNCH;
 }
 
 void zq_eri_set_destroy(zq_eri_set *s) {
+  ZqGate _gate;
   if (!s) return;
   (void)hipSetDevice(s->device);
   if (s->eri) (void)hipFree(s->eri);
@@ -316,6 +359,7 @@ size_t zq_eri_set_bytes(const zq_eri_set *s) {
 }
 
 int zq_attach(zq_ctx *ctx, zq_eri_set *s) {
+  ZqGate _gate;
   if (!ctx || !s) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_attach: null argument");
   if (s->device != ctx->device) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_attach: set and context on different devices");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));

@@ -606,6 +606,7 @@ void bind_view(zq_ctx *v, zq_batch *b, int m) {
 extern "C" {
 
 zq_batch *zq_batch_create(int device) {
+  ZqGate _gate;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
   if (hipSetDevice(device) != hipSuccess) return nullptr;
@@ -619,6 +620,7 @@ zq_batch *zq_batch_create(int device) {
 }
 
 void zq_batch_destroy(zq_batch *b) {
+  ZqGate _gate;
   if (!b) return;
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
@@ -645,6 +647,7 @@ int zq_batch_add(zq_batch *b, size_t n, size_t nocc, const double *H, const doub
   const size_t np = n * (n + 1) / 2, nu = np * (np + 1) / 2;
   h.eri.assign(eri_packed, eri_packed + nu);
   if (b->uploaded) {
+    ZqGate _gate;
     free_all(b);  // residency is rebuilt on the next run
   }
   b->mols.push_back(std::move(h));
@@ -652,6 +655,7 @@ int zq_batch_add(zq_batch *b, size_t n, size_t nocc, const double *H, const doub
 }
 
 int zq_batch_autotune(zq_batch *b, int chosen[2]) {
+  ZqGate _gate;
   if (!b) return ZQ_ERR_INVALID;
   BTRY(b, hipSetDevice(b->device));
   if (int rc = upload(b)) return rc;
@@ -700,6 +704,7 @@ int zq_batch_autotune(zq_batch *b, int chosen[2]) {
 
 int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, double *E_rhf, double *E_corr,
                  int *iterations, int *status) {
+  ZqGate _gate;
   if (!b || !opt || !E_rhf || !E_corr || !iterations || !status) return bfail(b, ZQ_ERR_INVALID, "zq_batch_run: null argument");
   if (opt->generalized || opt->fock_precision != ZQ_PREC_FP64 || opt->diis < 0 || opt->diis > kHist || opt->iterations < 1)
     return bfail(b, ZQ_ERR_INVALID, "zq_batch_run: RHF, FP64 Fock builds, diis <= 8, iterations >= 1");
@@ -715,6 +720,7 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
   if (!b->graph || b->graph_key != key) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
+    ZqCaptureWindow window;
     BTRY(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_iteration(b, opt->diis, opt->threshold, opt->iterations);
     hipGraph_t g = nullptr;
@@ -835,6 +841,7 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
   std::vector<std::thread> th;
   for (int w = 0; w < nworkers; ++w)
     th.emplace_back([&, w] {
+      ZqGate gate;
       (void)hipSetDevice(b->device);
       for (int m; (m = next.fetch_add(1)) < M;) {
         E_corr[m] = 0.0;

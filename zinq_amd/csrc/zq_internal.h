@@ -101,6 +101,7 @@ struct zq_ctx {
   std::string scf_graph_key;
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 48;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
+  hipEvent_t scf_events[2] = {nullptr, nullptr};  // the asynchronous SCF's lagged state reads
 
   // named scratch buffers (grow-only)
   std::map<std::string, DevBuf> scratch;
@@ -125,6 +126,24 @@ struct zq_eri_set {
   double *eri = nullptr;
   float *eri32 = nullptr;
   std::string err;
+};
+
+// ---- process-wide gate between hipGraph stream captures and the runtime calls that break them
+// (allocation, free, synchronous copies, rocSOLVER/rocBLAS) in other threads.  Every public entry point
+// holds a ZqGate (shared, re-entrant per thread); a capture runs inside a ZqCaptureWindow (exclusive;
+// the thread's own shared hold is released for the window and taken back after it).
+struct ZqGate {
+  ZqGate();
+  ~ZqGate();
+  ZqGate(const ZqGate &) = delete;
+  ZqGate &operator=(const ZqGate &) = delete;
+};
+struct ZqCaptureWindow {
+  int saved;
+  ZqCaptureWindow();
+  ~ZqCaptureWindow();
+  ZqCaptureWindow(const ZqCaptureWindow &) = delete;
+  ZqCaptureWindow &operator=(const ZqCaptureWindow &) = delete;
 };
 
 // ---- helpers implemented in zq_context.hip

@@ -398,6 +398,7 @@ int zq_eri32(zq_ctx *ctx) {
 }
 
 extern "C" int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double x_exch, double *F_out) {
+  ZqGate _gate;
   if (!ctx || !H || !P || !F_out) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_fock_jk: null argument");
   if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "zq_fock_jk: no ERIs loaded");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <vector>
 
 #include "../../include/zinq_scf.h"
@@ -540,17 +541,14 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   zq_ctx *ctx = d.ctx;
   const int n = d.n, nocc = r.nocc;
   const size_t nn = d.nn;
+  std::unique_ptr<ZqGate> gate(new ZqGate);  // set-up allocates; released before the replay loop
   auto *st = (ScfState *)zq_scratch(ctx, "scf_state", sizeof(ScfState));
   auto *hst = (int *)zq_pinned(ctx, "scf_state_host", 2 * 4 * sizeof(int));
-  if (!st || !hst) return zq_fail(ctx, ZQ_ERR_OOM, "SCF state");
+  if (!st || !hst || !zq_pinned(ctx, "scf_state_full", sizeof(ScfState))) return zq_fail(ctx, ZQ_ERR_OOM, "SCF state");
   double *part_rms = d.part + (1 + kMaxHist) * kRedBlocks;
-  hipEvent_t ev[2];
-  ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  struct EvGuard {
-    hipEvent_t *e;
-    ~EvGuard() { (void)hipEventDestroy(e[0]), (void)hipEventDestroy(e[1]); }
-  } guard{ev};
+  hipEvent_t *ev = ctx->scf_events;
+  if (!ev[0]) ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  if (!ev[1]) ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
 
   if (r.mixed) {
     if (int rc = zq_eri32(ctx)) return rc;
@@ -566,6 +564,7 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   if (r.mixed) ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_F, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
   const int budget = ctx->sp2_budget;
 
+  gate.reset();
   // one iteration as a hipGraph, captured once per context and key, replayed with one launch
   const bool use_graph = !ctx->profile;
   char key[256];
@@ -574,6 +573,7 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   if (use_graph && (!ctx->scf_graph || ctx->scf_graph_key != key)) {
     if (ctx->scf_graph) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
     ctx->scf_graph = nullptr;
+    ZqCaptureWindow window;  // no allocation or synchronous call in any thread while capturing
     ZQ_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_iteration(d, r, budget, st, part_rms);
     hipGraph_t graph = nullptr;
@@ -605,9 +605,11 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
       if (hst[4 * ((k - 1) & 1)] != 0) break;
     }
   }
+  auto *hs_p = (ScfState *)zq_pinned(ctx, "scf_state_full", sizeof(ScfState));
+  if (!hs_p) return zq_fail(ctx, ZQ_ERR_OOM, "SCF state");
+  ZQ_TRY(ctx, hipMemcpyAsync(hs_p, st, sizeof(ScfState), hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  ScfState hs;
-  ZQ_TRY(ctx, hipMemcpy(&hs, st, sizeof(ScfState), hipMemcpyDeviceToHost));
+  const ScfState hs = *hs_p;
   *status = hs.done == 0 ? 2 : hs.done;
   *energy = hs.e_new;
   *iterations = hs.it;
@@ -716,6 +718,8 @@ void zq_release_solver(zq_ctx *ctx) {
     (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
     ctx->scf_graph = nullptr;
   }
+  for (hipEvent_t &e : ctx->scf_events)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
 }
 
 extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const double *H, const double *S, double VN,
@@ -725,6 +729,9 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
     return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: bad arguments");
   if (!ctx->eri || size_t(ctx->n) != n_) return zq_fail(ctx, ZQ_ERR_STATE, "zq_scf_run_device: ERIs of this size not loaded");
   if (opt->diis > 64) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: DIIS history above 64");
+  // Allocation, synchronous copies and rocSOLVER run under the shared gate; the graph-replay loop of
+  // scf_async does not, so another thread may capture while this one iterates.
+  std::unique_ptr<ZqGate> gate(new ZqGate);
   if (opt->fock_precision != ZQ_PREC_FP64 && opt->fock_precision != ZQ_PREC_FP32)
     return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: fock_precision must be ZQ_PREC_FP64 or ZQ_PREC_FP32");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
@@ -740,7 +747,10 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   double energy = VN;
   if (int rc = setup(d, H, S)) return rc;
   if (opt->diis <= kMaxHist) {
-    if (int rc = scf_async(d, r, &status, &energy, &iterations)) return rc;
+    gate.reset();
+    const int rc = scf_async(d, r, &status, &energy, &iterations);
+    gate.reset(new ZqGate);
+    if (rc) return rc;
   }
   if (status == 3) ctx->sp2_budget = std::min(100, ctx->sp2_budget + 16);
   if (status == 0 || status == 3) {

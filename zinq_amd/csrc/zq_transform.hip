@@ -659,6 +659,7 @@ int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int p
 }
 
 extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out) {
+  ZqGate _gate;
   if (!ctx || !C || !out) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_ao2mo: null argument");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   void *d = nullptr;
@@ -704,6 +705,7 @@ int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int p
 }
 
 extern "C" int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr) {
+  ZqGate _gate;
   if (!ctx || !C || !e || !e_corr) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2: null argument");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   return zq_mp2_device(ctx, C, e, int(nocc), precision, e_corr);

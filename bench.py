@@ -231,6 +231,31 @@ def serve(lams, slots, kmax):
                 max_abs_dE_mp2_vs_fp64=worst, resident_sets_gb=round(resident_gb, 2), integrals_s=round(t_int, 1))
 
 
+def kernel_only(which, k, reps, basis):
+    """Single-kernel runs for hardware-counter passes: `jk` repeats the FP64 Fock build of (H2O)_k on a
+    fixed density; `copy` copies a device buffer of 1 GiB (reads 1 GiB, writes 1 GiB), the calibration
+    for FETCH_SIZE / WRITE_SIZE."""
+    if which == "copy":
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(reps):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        return dict(kernel="copy", bytes_read=x.numel() * 8, bytes_written=y.numel() * 8, reps=reps)
+    from zinq_amd import hip
+    from zinq_amd.molecules import water_system
+    s = water_system(k, basis=basis)
+    ctx = hip.Context(0)
+    ctx.eri_load_packed(s.coulomb_packed(), s.nbf)
+    H = s.hcore()
+    P = np.eye(s.nbf) * (s.nel / s.nbf)
+    for _ in range(reps):
+        ctx.fock_jk(H, P)
+    ctx.close()
+    return dict(kernel="zq_jk_records", n=s.nbf, algorithmic_bytes=jk_bytes(s.nbf), reps=reps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +273,7 @@ def main():
     ap.add_argument("--serve", default="", metavar="L1,L2,..", help="serving workload at these arrival rates instead")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--kmax", type=int, default=10)
+    ap.add_argument("--kernel", default="", choices=["", "jk", "copy"], help="single-kernel run for counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: only the one-GPU path is built; a multi-GPU run is not supported.", file=sys.stderr)
@@ -257,6 +283,9 @@ def main():
         return
     if args.batch:
         print(json.dumps(batch(args.batch, args.steps, args.warmup)), flush=True)
+        return
+    if args.kernel:
+        print(json.dumps(kernel_only(args.kernel, int(args.ks.split(",")[-1]), args.steps, args.basis)), flush=True)
         return
     if args.serve:
         print(json.dumps(serve([float(x) for x in args.serve.split(",")], args.slots, args.kmax)), flush=True)

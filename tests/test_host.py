@@ -153,3 +153,26 @@ def test_batch_specs_cover_the_workload():
     assert len(sizes) == 6
     n = sorted({batch_system(s).nbf for s in specs[:40]})
     assert set(n) <= {7, 14, 21, 24, 48, 72}
+
+
+def test_matrix_text_format_round_trip(tmp_path):
+    """The reference's text format (read_write.zig:24-80): header, {d:20.14} elements, n^2 x n^2 ERIs."""
+    from zinq_amd import matio
+    from conftest import system_ints
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((3, 5)) * 10.0 ** rng.integers(-3, 3, (3, 5))
+    p = tmp_path / "a.mat"
+    matio.write_matrix(p, A)
+    lines = p.read_text().splitlines()
+    assert lines[0] == "3 5" and len(lines) == 4
+    assert len(lines[1]) == 5 * 20 + 4
+    np.testing.assert_allclose(matio.read_matrix(p), A, rtol=0, atol=5e-15)
+    s, H, S, g, VN = system_ints("sto3g")
+    q = tmp_path / "coulomb.mat"
+    matio.write_tensor(q, g)
+    assert q.read_text().splitlines()[0] == f"{s.nbf ** 2} {s.nbf ** 2}"
+    np.testing.assert_allclose(matio.read_coulomb(q), g, rtol=0, atol=5e-15)
+    # readMatrix ignores where the line breaks fall
+    r = tmp_path / "b.mat"
+    r.write_text("2 2\n1.5 -2.25\n3.0\n4.0\n")
+    np.testing.assert_array_equal(matio.read_matrix(r), [[1.5, -2.25], [3.0, 4.0]])

@@ -153,16 +153,32 @@ __global__ __launch_bounds__(512) void zq_q1(const double *__restrict__ eri, con
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = MF::zero();
 
+  // the next mu block's values are loaded into registers while the current one is multiplied
+  T rv[8];
+  {
+    const Q1Src src = q1_src(0, bnu, br, bs, Jb);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 512 * j;
+      rv[j] = T(q1_val(eri, src, e >> 6, (e & 63) >> 3, e & 7));
+    }
+  }
   for (int bmu = 0; bmu < nb; ++bmu) {
-    const Q1Src src = q1_src(bmu, bnu, br, bs, Jb);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int e = tid + 512 * j;
-      const int K = e >> 6, uw = e & 63;
-      Ts[K][uw] = T(q1_val(eri, src, K, uw >> 3, uw & 7));
+      Ts[e >> 6][e & 63] = rv[j];
     }
     __syncthreads();
+    if (bmu + 1 < nb) {
+      const Q1Src nsrc = q1_src(bmu + 1, bnu, br, bs, Jb);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = tid + 512 * j;
+        rv[j] = T(q1_val(eri, nsrc, e >> 6, (e & 63) >> 3, e & 7));
+      }
+    }
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       const int mu = bmu * 8 + ks * 4 + (lane >> 4);
@@ -296,12 +312,61 @@ __device__ inline T gemm_b(const GemmArgs<T> &g, const T *B, int k, int n) {
   return B[k * g.sBk + n * g.sBn];
 }
 
-// FP64 / FP32: 64 x 64 tiles, BK = 16, 4 waves of 32 x 32 (2 x 2 MFMA tiles).
+// FP64 / FP32 / FP32C: 64 x 64 tiles, BK = 16, 4 waves of 32 x 32 (2 x 2 MFMA tiles).  The next K tile
+// is loaded into registers while the current one is multiplied out of a double-buffered LDS tile, so
+// each K step waits on one barrier and no global-load round trip.
+template <class MF>
+__device__ inline void gemm_load(const GemmArgs<typename MF::T> &g, const typename MF::T *A, const typename MF::T *B,
+                                 int m0, int n0, int k0, bool a_kfast, bool b_nfast, typename MF::T ra[4],
+                                 typename MF::T rb[4]) {
+  typedef typename MF::T T;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    int mm, kk;
+    if (a_kfast)
+      mm = e >> 4, kk = e & 15;
+    else
+      kk = e >> 6, mm = e & 63;
+    const int m = m0 + mm, k = k0 + kk;
+    ra[j] = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : T(0);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    int nn, kk;
+    if (b_nfast)
+      kk = e >> 6, nn = e & 63;
+    else
+      nn = e >> 4, kk = e & 15;
+    const int n = n0 + nn, k = k0 + kk;
+    rb[j] = (n < g.N && k < g.K) ? gemm_b(g, B, k, n) : T(0);
+  }
+}
+
+template <class T>
+__device__ inline void gemm_stage(T (*As)[65], T (*Bs)[65], const T ra[4], const T rb[4], bool a_kfast, bool b_nfast) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    if (a_kfast)
+      As[e & 15][e >> 4] = ra[j];
+    else
+      As[e >> 6][e & 63] = ra[j];
+    if (b_nfast)
+      Bs[e >> 6][e & 63] = rb[j];
+    else
+      Bs[e & 15][e >> 4] = rb[j];
+  }
+}
+
 template <class MF>
 __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
   typedef typename MF::T T;
-  __shared__ T As[16][65];
-  __shared__ T Bs[16][65];
+  __shared__ T As[2][16][65];
+  __shared__ T Bs[2][16][65];
   const int b = blockIdx.z;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -315,39 +380,22 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
     for (int y = 0; y < 2; ++y) acc[x][y] = MF::zero();
   const bool a_kfast = (g.sAk == 1);
   const bool b_nfast = (g.mode == 1) || (g.sBn == 1);
+  T ra[4], rb[4];
+  gemm_load<MF>(g, A, B, m0, n0, 0, a_kfast, b_nfast, ra, rb);
+  gemm_stage<T>(As[0], Bs[0], ra, rb, a_kfast, b_nfast);
+  __syncthreads();
+  int cur = 0;
   for (int k0 = 0; k0 < g.K; k0 += 16) {
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      int mm, kk;
-      if (a_kfast)
-        mm = e >> 4, kk = e & 15;
-      else
-        kk = e >> 6, mm = e & 63;
-      const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : T(0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      int nn, kk;
-      if (b_nfast)
-        kk = e >> 6, nn = e & 63;
-      else
-        nn = e >> 4, kk = e & 15;
-      const int n = n0 + nn, k = k0 + kk;
-      Bs[kk][nn] = (n < g.N && k < g.K) ? gemm_b(g, B, k, n) : T(0);
-    }
-    __syncthreads();
+    const bool more = k0 + 16 < g.K;
+    if (more) gemm_load<MF>(g, A, B, m0, n0, k0 + 16, a_kfast, b_nfast, ra, rb);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
       T a[2], bb[2];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) a[x] = As[kk][wm + x * 16 + (lane & 15)];
+      for (int x = 0; x < 2; ++x) a[x] = As[cur][kk][wm + x * 16 + (lane & 15)];
 #pragma unroll
-      for (int y = 0; y < 2; ++y) bb[y] = Bs[kk][wn + y * 16 + (lane & 15)];
+      for (int y = 0; y < 2; ++y) bb[y] = Bs[cur][kk][wn + y * 16 + (lane & 15)];
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -357,6 +405,9 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
     for (int x = 0; x < 2; ++x)
 #pragma unroll
       for (int y = 0; y < 2; ++y) MF::flush(acc[x][y]);
+    if (more) gemm_stage<T>(As[cur ^ 1], Bs[cur ^ 1], ra, rb, a_kfast, b_nfast);
+    __syncthreads();
+    cur ^= 1;
   }
   T *C = g.C + b * g.sCb;
 #pragma unroll

@@ -17,6 +17,9 @@
  */
 #include <math.h>
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <string.h>
 
 #define AT2(A, n, i, j) ((A)[(size_t)(i) * (n) + (j)])
@@ -405,6 +408,76 @@ void orc_ao2mo_oovv(int N, int nocc, const double *g, const double *C, double *o
 #undef T3
 #undef OUT
   free(t1), free(t2), free(t3);
+}
+
+/* The same arithmetic on all cores (OpenMP), for the CPU baseline's many-core mode: every output element
+ * is accumulated in the faithful functions' order, so the results are bitwise those of orc_get_fock and
+ * orc_ao2mo_oovv; only independent outputs are split across threads. */
+void orc_get_fock_par(int n, const double *g, const double *H, const double *P, int generalized, double *F) {
+  const double exch_factor = generalized ? 1.0 : 0.5;
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; ++k)
+    for (int l = 0; l < n; ++l) {
+      double f = AT2(H, n, k, l);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) f += AT2(P, n, i, j) * (AT4(g, n, i, k, j, l) - exch_factor * AT4(g, n, i, j, k, l));
+      AT2(F, n, k, l) = f;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double avg = (AT2(F, n, i, j) + AT2(F, n, j, i)) / 2;
+      AT2(F, n, i, j) = avg;
+      AT2(F, n, j, i) = avg;
+    }
+}
+
+void orc_ao2mo_oovv_par(int N, int nocc, const double *g, const double *C, double *out) {
+  const int nvir = N - nocc;
+  const size_t o = nocc, v = nvir, n = N;
+  double *t1 = calloc(o * n * n * n, sizeof(double)), *t2 = calloc(o * o * n * n, sizeof(double));
+  double *t3 = calloc(o * o * v * n, sizeof(double));
+  memset(out, 0, sizeof(double) * o * o * v * v);
+#define T1(i, l, nu, s) t1[(((i)*n + (l)) * n + (nu)) * n + (s)]
+#define T2(i, j, nu, s) t2[(((i)*o + (j)) * n + (nu)) * n + (s)]
+#define T3(i, j, a, s) t3[(((i)*o + (j)) * v + (a)) * n + (s)]
+#define OUT(i, j, a, b) out[(((i)*o + (j)) * v + (a)) * v + (b)]
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (size_t i = 0; i < o; ++i)
+    for (size_t lam = 0; lam < n; ++lam)
+      for (size_t nu = 0; nu < n; ++nu)
+        for (size_t mu = 0; mu < n; ++mu)
+          for (size_t sg = 0; sg < n; ++sg) T1(i, lam, nu, sg) += AT2(C, N, mu, i) * AT4(g, N, mu, lam, nu, sg);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (size_t i = 0; i < o; ++i)
+    for (size_t j = 0; j < o; ++j)
+      for (size_t nu = 0; nu < n; ++nu)
+        for (size_t lam = 0; lam < n; ++lam)
+          for (size_t sg = 0; sg < n; ++sg) T2(i, j, nu, sg) += AT2(C, N, lam, j) * T1(i, lam, nu, sg);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (size_t i = 0; i < o; ++i)
+    for (size_t j = 0; j < o; ++j)
+      for (size_t a = o; a < n; ++a)
+        for (size_t nu = 0; nu < n; ++nu)
+          for (size_t sg = 0; sg < n; ++sg) T3(i, j, a - o, sg) += AT2(C, N, nu, a) * T2(i, j, nu, sg);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (size_t i = 0; i < o; ++i)
+    for (size_t j = 0; j < o; ++j)
+      for (size_t a = 0; a < v; ++a)
+        for (size_t sg = 0; sg < n; ++sg)
+          for (size_t b = o; b < n; ++b) OUT(i, j, a, b - o) += AT2(C, N, sg, b) * T3(i, j, a, sg);
+#undef T1
+#undef T2
+#undef T3
+#undef OUT
+  free(t1), free(t2), free(t3);
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 /* ao2so_coef, src/integral_transform.zig:82-101. */

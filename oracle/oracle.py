@@ -48,7 +48,33 @@ def lib():
         L.orc_mp2_closed.argtypes = [c_int, c_int, vp, vp]
         L.orc_mp2_closed.restype = c_double
         L.orc_ci.argtypes = [c_int, vp, vp, vp, c_int, c_int, vp, c_int, c_double, c_int, vp]
+        L.orc_get_fock_par.argtypes = [c_int, vp, vp, vp, c_int, vp]
+        L.orc_ao2mo_oovv_par.argtypes = [c_int, c_int, vp, vp, vp]
+        L.orc_max_threads.restype = c_int
     return _lib
+
+
+def max_threads():
+    """Threads the *_par functions use (OpenMP)."""
+    return int(lib().orc_max_threads())
+
+
+def get_fock_par(g, H, P, generalized=False):
+    """get_fock on all cores; bitwise equal to get_fock (same per-element accumulation order)."""
+    g, H, P = _c(g), _c(H), _c(P)
+    n = H.shape[0]
+    F = np.zeros((n, n))
+    lib().orc_get_fock_par(n, _p(g), _p(H), _p(P), int(generalized), _p(F))
+    return F
+
+
+def ao2mo_oovv_par(g, C, nocc):
+    """ao2mo_oovv on all cores; bitwise equal to ao2mo_oovv."""
+    g, C = _c(g), _c(C)
+    n = C.shape[0]
+    out = np.zeros((nocc, nocc, n - nocc, n - nocc))
+    lib().orc_ao2mo_oovv_par(n, nocc, _p(g), _p(C), _p(out))
+    return out
 
 
 def _p(a):

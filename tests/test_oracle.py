@@ -204,3 +204,17 @@ def test_packed_matches_dense():
     chem = g.transpose(0, 2, 1, 3)
     M = chem[idx[:, 0], idx[:, 1]][:, idx[:, 0], idx[:, 1]]
     np.testing.assert_array_equal(M[np.tril_indices(len(idx))], pk)
+
+
+def test_many_core_oracle_is_bitwise_the_faithful_one():
+    """The CPU baseline's many-core mode (OpenMP over independent outputs) reproduces the faithful
+    single-core arithmetic bit for bit."""
+    s, H, S, g, VN = system_ints("ccpvdz")
+    n, o = s.nbf, s.nel // 2
+    rng = np.random.default_rng(11)
+    P = rng.standard_normal((n, n))
+    P = P + P.T
+    np.testing.assert_array_equal(O.get_fock_par(g, H, P), O.get_fock(g, H, P))
+    C = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    np.testing.assert_array_equal(O.ao2mo_oovv_par(g, C, o), O.ao2mo_oovv(g, C, o))
+    assert O.max_threads() >= 1

@@ -593,6 +593,10 @@ int enqueue_iteration(zq_batch *b, int diis, double thr, int maxit) {
 // A view context on molecule m's resident quartets, for the transform code of zq_transform.hip.
 void bind_view(zq_ctx *v, zq_batch *b, int m) {
   const int n = b->mols[m].n;
+  if (v->eri32 && v->eri32_owned) (void)hipFree(v->eri32);  // an FP32 copy of the previous molecule
+  v->eri32 = nullptr;
+  v->eri32_owned = true;
+  v->eri_owned = false;
   v->n = n;
   v->nb = nb_of(n);
   v->npad = v->nb * ZQ_BLK;

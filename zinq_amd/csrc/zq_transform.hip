@@ -114,9 +114,11 @@ __device__ inline Q1Src q1_src(int bmu, int bnu, int br, int bs, int Jb) {
   return s;
 }
 
-// (mu = bmu*8+u, nu = bnu*8+w | K): 0 for non-canonical ket slots of a diagonal ket block.
-__device__ inline double q1_val(const double *__restrict__ eri, const Q1Src &s, int K, int u, int w) {
-  if (s.diagK && (K >> 3) < (K & 7)) return 0.0;
+// (mu = bmu*8+u, nu = bnu*8+w | K): 0 for non-canonical ket slots of a diagonal ket block.  E is the
+// element type of the resident quartets (the FP64 set, or its FP32 copy for the FP32 forms).
+template <class E>
+__device__ inline E q1_val(const E *__restrict__ eri, const Q1Src &s, int K, int u, int w) {
+  if (s.diagK && (K >> 3) < (K & 7)) return E(0);
   int pl = s.swp ? w : u, ql = s.swp ? u : w;
   if (s.diagB && pl < ql) {
     const int t = pl;
@@ -138,8 +140,8 @@ __device__ inline double q1_val(const double *__restrict__ eri, const Q1Src &s, 
 // Loop over mu blocks: stage T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then
 // X[K][i][nu] += C1[mu,i] T via MFMA.  Output tiles: MT (<= 4) tiles of 16 i by 32 tiles of 16 (K,nu)
 // columns, split evenly over the 8 waves.
-template <class MF, int MT>
-__global__ __launch_bounds__(512) void zq_q1(const double *__restrict__ eri, const typename MF::T *__restrict__ C1,
+template <class MF, int MT, class E>
+__global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
                                              int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
   typedef typename MF::T T;
   constexpr int TPW = MT * 4;  // tiles per wave
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(512) void zq_q1_bf16x3(const double *__restrict__ e
           const int e = tid + 512 * j;
           const int K = e >> 6, uw = e & 63, u = uw >> 3, w = uw & 7;
           bf16 h, l;
-          split_bf16(float(q1_val(eri, src, K, u, w)), h, l);
+          split_bf16(float(q1_val<double>(eri, src, K, u, w)), h, l);
           Th[K][w][bi * 8 + u] = h;
           Tl[K][w][bi * 8 + u] = l;
         }
@@ -580,6 +582,10 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
     for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = T(C[size_t(mu) * n + j]);
     for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = T(C[size_t(mu) * n + d.off3 + a]);
   }
+  if constexpr (sizeof(T) == 4) {
+    if (precision != ZQ_PREC_BF16X3)
+      if (int rc = zq_eri32(ctx)) return rc;
+  }
   T *d1, *d2t, *d3;
   if (int rc = upload(ctx, "C1", h1, &d1)) return rc;
   if (int rc = upload(ctx, "C2T", h2t, &d2t)) return rc;
@@ -624,11 +630,23 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
         default: hipLaunchKernelGGL(zq_q1_bf16x3<4>, grid, dim3(512), 0, ctx->stream, ctx->eri, d1h, d1l, ldt, m1, nb, np, Xf); break;
       }
     } else {
-      switch (mt) {
-        case 1: hipLaunchKernelGGL((zq_q1<MF, 1>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
-        case 2: hipLaunchKernelGGL((zq_q1<MF, 2>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
-        case 3: hipLaunchKernelGGL((zq_q1<MF, 3>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
-        default: hipLaunchKernelGGL((zq_q1<MF, 4>), grid, dim3(512), 0, ctx->stream, ctx->eri, d1, m1, m1, nb, np, X); break;
+      if constexpr (sizeof(T) == 4) {
+        // FP32 forms: read the FP32 copy of the quartets (the same rounding as converting on the fly)
+        const float *e32 = ctx->eri32;
+        switch (mt) {
+          case 1: hipLaunchKernelGGL((zq_q1<MF, 1, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          case 2: hipLaunchKernelGGL((zq_q1<MF, 2, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          case 3: hipLaunchKernelGGL((zq_q1<MF, 3, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          default: hipLaunchKernelGGL((zq_q1<MF, 4, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+        }
+      } else {
+        const double *e64 = ctx->eri;
+        switch (mt) {
+          case 1: hipLaunchKernelGGL((zq_q1<MF, 1, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          case 2: hipLaunchKernelGGL((zq_q1<MF, 2, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          case 3: hipLaunchKernelGGL((zq_q1<MF, 3, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          default: hipLaunchKernelGGL((zq_q1<MF, 4, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+        }
       }
     }
     zq_prof_end(ctx);

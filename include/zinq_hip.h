@@ -66,6 +66,13 @@ int zq_eri_load_packed(zq_ctx *ctx, const double *g_packed, size_t n);
  * with x_exch = 0.5 for RHF and 1.0 for GHF (spin-blocked ERIs loaded).  H, P, F_out are n x n. */
 int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double x_exch, double *F_out);
 
+/* getFock for ndens densities in one pass over the resident ERIs (the densities of a work item run back to
+ * back and share its quartet reads): F_out[d] = H + G(P[d]), each symmetrised, P and F_out ndens x n x n.
+ * H may be NULL, giving the bare two-electron part G(P[d]): the response contraction
+ * V_x = sum P_x (g - x g) of CPHF (src/cphf.zig:167-172).  The DFT branch's J + exx K
+ * (src/hartree_fock.zig:701-719) is zq_fock_jk with x_exch = exx / 2. */
+int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, size_t ndens, double x_exch, double *F_out);
+
 /* AO->MO transform of the loaded ERIs with MO coefficients C (n x n, columns = MOs).
  *   ZQ_BLOCK_FULL: out[p,q,r,s] = <pq|rs> for all MOs, the result of ao2mo_pppp
  *                  (src/integral_transform.zig:37-79); out has n^4 entries.

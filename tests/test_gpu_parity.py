@@ -376,3 +376,17 @@ def test_serve_mixed_results_within_bar():
             e2 = ctx.mp2(ref["C"], ref["e"], m.o)
             print(f"{m.key}: served dE_MP2 = {r['E_mp2'] - ref['E'] - e2:+.2e}")
             assert abs(r["E_mp2"] - ref["E"] - e2) < 1e-6
+
+
+def test_fock_jk_multi_matches_single_builds(ctx):
+    """Several densities in one ERI pass equal the single-density builds bitwise; H = None gives the
+    bare two-electron part G(P) used by the CPHF response contraction."""
+    s, H, S, g, VN = _load(ctx, "w2")
+    rng = np.random.default_rng(21)
+    Ps = rng.standard_normal((3, s.nbf, s.nbf))
+    Ps = Ps + Ps.transpose(0, 2, 1)
+    F = ctx.fock_jk_multi(H, Ps)
+    G = ctx.fock_jk_multi(None, Ps, x_exch=1.0)
+    for d in range(3):
+        np.testing.assert_array_equal(F[d], ctx.fock_jk(H, Ps[d]))
+        np.testing.assert_allclose(G[d], O.get_fock(g, np.zeros_like(H), Ps[d], generalized=True), atol=1e-11)

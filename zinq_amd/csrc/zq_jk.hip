@@ -50,13 +50,20 @@ __device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
 // LDS in a fixed order.
 // GROUP: the item's molecule selects the quartets, P and its leading dimension from mols[], and the
 // item is skipped once done[molecule] is set.
-template <class E, bool GROUP>
+// MULTI: blockIdx.x is the density d (P + d * pstride, records + d * rstride) and blockIdx.y the item,
+// so the densities of one item run back to back and share its quartet reads through the caches.
+template <class E, bool GROUP, bool MULTI = false>
 __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, const JKItem *__restrict__ items,
                                                      int nitems, const double *__restrict__ P, int npad, double xexch,
                                                      double *__restrict__ rec, const int *__restrict__ skip,
-                                                     const JKMol *__restrict__ mols, const int *__restrict__ done) {
+                                                     const JKMol *__restrict__ mols, const int *__restrict__ done,
+                                                     long long pstride = 0, long long rstride = 0) {
   __shared__ double red[4][2][64];
-  const int item_id = blockIdx.x;
+  const int item_id = MULTI ? blockIdx.y : blockIdx.x;
+  if (MULTI) {
+    P += blockIdx.x * pstride;
+    rec += blockIdx.x * rstride;
+  }
   if (item_id >= nitems || (skip && *skip)) return;
   if (GROUP) {
     const int m = items[item_id].mol;
@@ -198,13 +205,19 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
 // a fixed order, so the result is bitwise reproducible.
 // GROUP: blk[blockIdx.x] = (molecule, its output block); H, F, n and the leading dimension come from
 // mols[].  Otherwise H and F are n x n with leading dimension ldf and blockIdx.x is the output block.
-template <bool GROUP>
+// MULTI: blockIdx.y is the density (records + d * rstride, F + d * fstride); H may be null (base 0).
+template <bool GROUP, bool MULTI = false>
 __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
                                                     const int *__restrict__ csr_idx, const double *H, int n, double *F,
                                                     const int *__restrict__ skip, int ldf, const JKMol *__restrict__ mols,
-                                                    const int2 *__restrict__ blk, const int *__restrict__ done) {
+                                                    const int2 *__restrict__ blk, const int *__restrict__ done,
+                                                    long long rstride = 0, long long fstride = 0) {
   __shared__ double part[4][64];
   if (skip && *skip) return;
+  if (MULTI) {
+    rec += blockIdx.y * rstride;
+    F += blockIdx.y * fstride;
+  }
   int o = blockIdx.x;
   if (GROUP) {
     const int2 b = blk[blockIdx.x];
@@ -243,7 +256,7 @@ __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ r
   __syncthreads();
   if (wave != 0) return;
   const int gi = x * 8 + i, gj = y * 8 + j;
-  const double h = (gi < n && gj < n) ? H[size_t(gi) * ldf + gj] : 0.0;
+  const double h = (H && gi < n && gj < n) ? H[size_t(gi) * ldf + gj] : 0.0;
   const double f = h + (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
   // on diagonal blocks lane (i,j) and lane (j,i) sum the same terms in different orders: let only
   // i >= j write, so both triangles hold one value
@@ -394,6 +407,42 @@ int zq_eri32(zq_ctx *ctx) {
   ctx->eri32_owned = true;
   zq_narrow_eri(ctx->stream, ctx->eri, cnt, ctx->eri32);
   ZQ_LAUNCH_CHECK(ctx, "zq_narrow");
+  return ZQ_OK;
+}
+
+extern "C" int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, size_t ndens, double x_exch,
+                                double *F_out) {
+  ZqGate _gate;
+  if (!ctx || !P || !F_out || ndens == 0 || ndens > 65535)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_fock_jk_multi: bad arguments");
+  if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "zq_fock_jk_multi: no ERIs loaded");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  const int n = ctx->n, np = ctx->npad;
+  const size_t nn = size_t(n) * n, pstride = size_t(np) * np, rstride = ctx->nrecords * 64;
+  auto *dP = (double *)zq_scratch(ctx, "jkm_P", ndens * pstride * sizeof(double));
+  auto *dR = (double *)zq_scratch(ctx, "jkm_rec", ndens * rstride * sizeof(double));
+  auto *dF = (double *)zq_scratch(ctx, "jkm_F", ndens * nn * sizeof(double));
+  if (!dP || !dR || !dF) return zq_fail(ctx, ZQ_ERR_OOM, "zq_fock_jk_multi: scratch");
+  // each density into its own zero-padded npad x npad slot
+  ZQ_TRY(ctx, hipMemsetAsync(dP, 0, ndens * pstride * sizeof(double), ctx->stream));
+  for (size_t d = 0; d < ndens; ++d)
+    ZQ_TRY(ctx, hipMemcpy2DAsync(dP + d * pstride, np * sizeof(double), P + d * nn, n * sizeof(double),
+                                 n * sizeof(double), n, hipMemcpyHostToDevice, ctx->stream));
+  if (H) ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_H, H, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  zq_prof_begin(ctx, "jk_records");
+  hipLaunchKernelGGL((zq_jk_records<double, false, true>), dim3(unsigned(ndens), unsigned(ctx->nitems)), dim3(256), 0,
+                     ctx->stream, ctx->eri, ctx->d_items, ctx->nitems, dP, np, x_exch, dR, nullptr, nullptr, nullptr,
+                     (long long)pstride, (long long)rstride);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, "zq_jk_records (multi)");
+  zq_prof_begin(ctx, "jk_reduce");
+  hipLaunchKernelGGL((zq_jk_reduce<false, true>), dim3(unsigned(ctx->Nb), unsigned(ndens)), dim3(256), 0, ctx->stream, dR,
+                     ctx->d_csr_ptr, ctx->d_csr_idx, H ? ctx->d_H : nullptr, n, dF, nullptr, n, nullptr, nullptr, nullptr,
+                     (long long)rstride, (long long)nn);
+  zq_prof_end(ctx);
+  ZQ_LAUNCH_CHECK(ctx, "zq_jk_reduce (multi)");
+  ZQ_TRY(ctx, hipMemcpyAsync(F_out, dF, ndens * nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return ZQ_OK;
 }
 

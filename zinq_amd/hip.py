@@ -17,7 +17,7 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32c": 3}
 EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_load_packed", "zq_fock_jk", "zq_ao2mo",
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
-            "zq_eri_set_bytes", "zq_attach", "zq_mem_info"]
+            "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi"]
 
 
 class GpuError(RuntimeError):
@@ -91,6 +91,7 @@ def lib():
         L.zq_eri_set_bytes.argtypes = [vp]
         L.zq_attach.argtypes = [vp, vp]
         L.zq_mem_info.argtypes = [ci, vp, vp]
+        L.zq_fock_jk_multi.argtypes = [vp, vp, vp, sz, cd, vp]
         L._zq_typed = True
     return L
 
@@ -191,6 +192,15 @@ class Context:
         H, P = _f64(H), _f64(P)
         F = np.zeros_like(H)
         self._check(self._L.zq_fock_jk(self._h, ptr(H), ptr(P), float(x_exch), ptr(F)))
+        return F
+
+    def fock_jk_multi(self, H, Ps, x_exch=0.5):
+        """F[d] = H + G(P[d]) for a stack of densities in one ERI pass (H=None: bare G(P[d]))."""
+        Ps = _f64(Ps)
+        nd, n = Ps.shape[0], Ps.shape[1]
+        F = np.zeros_like(Ps)
+        Hp = ptr(_f64(H)) if H is not None else None
+        self._check(self._L.zq_fock_jk_multi(self._h, Hp, ptr(Ps), nd, float(x_exch), ptr(F)))
         return F
 
     def ao2mo(self, C, nocc=0, block="full", precision="fp64"):

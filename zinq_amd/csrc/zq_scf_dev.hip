@@ -34,6 +34,7 @@
 #include "../../include/zinq_scf.h"
 #include "zq_internal.h"
 #include "zq_linalg.h"
+#include "zq_tile.h"
 
 namespace {
 
@@ -374,22 +375,29 @@ __device__ bool lu_solve(int n, double *A, double *b) {
 
 // E = sum of the energy partials + V_NN; DIIS: the new row of B, the bordered system, its LU solve
 // (src/hartree_fock.zig:176-227), or a reset to the current entry when it is singular (:821-845).
-__global__ void k_diis_solve(const double *__restrict__ part, int diis, double VN, ScfState *__restrict__ st) {
-  if (threadIdx.x != 0 || st->done) return;
-  double e = 0.0;
-  for (int b = 0; b < kRedBlocks; ++b) e += part[b];
-  st->e_new = e + VN;
-  st->combine = 0;
-  if (diis <= 0) return;
+// One wave: lane b holds partial b of every sum (kRedBlocks == 64), each sum a fixed-order butterfly;
+// lane 0 solves the <= 17 x 17 system.
+__global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ part, int diis, double VN,
+                                                   ScfState *__restrict__ st) {
+  static_assert(kRedBlocks == 64, "one partial per lane");
+  if (st->done) return;
+  const int lane = threadIdx.x;
+  const double e = zqt::wave_sum(part[lane]);
   const int slot = st->cur, m = st->hist_n;
+  bool live_k[kMaxHist];
   for (int k = 0; k < kMaxHist; ++k) {
     bool live = false;
     for (int h = 0; h < m; ++h) live |= st->hist[h] == k;
-    if (!live) continue;
-    double d = 0.0;
-    for (int b = 0; b < kRedBlocks; ++b) d += part[(1 + k) * kRedBlocks + b];
-    st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = d;
+    live_k[k] = live && diis > 0;
   }
+  double dots[kMaxHist];
+  for (int k = 0; k < kMaxHist; ++k) dots[k] = live_k[k] ? zqt::wave_sum(part[(1 + k) * kRedBlocks + lane]) : 0.0;
+  if (lane != 0) return;
+  st->e_new = e + VN;
+  st->combine = 0;
+  if (diis <= 0) return;
+  for (int k = 0; k < kMaxHist; ++k)
+    if (live_k[k]) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
   if (m < 2) return;
   const int dd = m + 1;
   double A[(kMaxHist + 1) * (kMaxHist + 1)], c[kMaxHist + 1];
@@ -440,11 +448,12 @@ __global__ void k_sp2_check(const int *__restrict__ fin, ScfState *__restrict__ 
   if (fin[0] < 2) st->done = 3;
 }
 
-// rms(dP) and the convergence test of src/hartree_fock.zig:853-870.
-__global__ void k_converge(const double *__restrict__ part, int n, double thr, int maxit, ScfState *__restrict__ st) {
-  if (threadIdx.x != 0 || st->done) return;
-  double s = 0.0;
-  for (int b = 0; b < kRedBlocks; ++b) s += part[b];
+// rms(dP) and the convergence test of src/hartree_fock.zig:853-870 (one wave, butterfly sum).
+__global__ __launch_bounds__(64) void k_converge(const double *__restrict__ part, int n, double thr, int maxit,
+                                                 ScfState *__restrict__ st) {
+  if (st->done) return;
+  const double s = zqt::wave_sum(part[threadIdx.x]);
+  if (threadIdx.x != 0) return;
   const double rms = sqrt(s / (double(n) * n));
   const double dE = fabs(st->e_new - st->e_old);
   st->e_old = st->e_new;
@@ -613,6 +622,12 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   *status = hs.done == 0 ? 2 : hs.done;
   *energy = hs.e_new;
   *iterations = hs.it;
+  // no-op SP2 steps still cost a dispatch: after a converged run the budget follows the largest step
+  // count seen (+8), shrinking only when that saves at least 4 steps (each change recaptures the graph)
+  if (hs.done == 1) {
+    const int want = std::max(16, std::min(100, hs.sp2_max + 8));
+    if (want > ctx->sp2_budget || want + 4 <= ctx->sp2_budget) ctx->sp2_budget = want;
+  }
   return ZQ_OK;
 }
 

@@ -53,10 +53,10 @@ int zq_scf_run(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double
 int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double *S, double VN,
                       const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);
 
-/* SP2 steps per density in zq_scf_run_device's captured iteration (1..100, initially 48).  Steps after
- * a density has settled return at once.  After each converged run the context sets the budget to the
- * largest step count it saw + 8.  A density that needs more than the budget restarts the run on the
- * host-synchronised driver and raises the budget by 16. */
+/* SP2 steps per density in zq_scf_run_device's captured iteration (1..100, default 48; steps after a
+ * density has settled return at once).  A density that needs more restarts the run on the
+ * host-synchronised driver and raises the budget by 16.  The budget is per context and is not lowered
+ * automatically: a context serves molecules of different sizes, and every restart costs a whole SCF. */
 int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
 
 /* Host generalized symmetric-definite eigensolver used by zq_scf_run (the dsygvd stand-in):

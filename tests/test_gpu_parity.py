@@ -210,6 +210,7 @@ def test_device_scf_graph_replays_match_eager(ctx):
     call after call, with eager work in between."""
     s, H, S, g, VN = _load(ctx, "w2")
     o = s.nel // 2
+    ctx.set_sp2_budget(48)
     ctx.profile(True)
     eager = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
     ctx.profile(False)

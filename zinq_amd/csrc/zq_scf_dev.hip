@@ -622,12 +622,6 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   *status = hs.done == 0 ? 2 : hs.done;
   *energy = hs.e_new;
   *iterations = hs.it;
-  // no-op SP2 steps still cost a dispatch: after a converged run the budget follows the largest step
-  // count seen (+8), shrinking only when that saves at least 4 steps (each change recaptures the graph)
-  if (hs.done == 1) {
-    const int want = std::max(16, std::min(100, hs.sp2_max + 8));
-    if (want > ctx->sp2_budget || want + 4 <= ctx->sp2_budget) ctx->sp2_budget = want;
-  }
   return ZQ_OK;
 }
 

@@ -319,6 +319,22 @@ def main():
                                              "the threshold, then one full FP64 build; transform: FP32 MFMA "
                                              "products, FP64 sums per K block"),
                      results=res_m)
+    if mixed is not None:
+        # one profiled mixed pass on the largest molecule: the FP32-ERI incremental Fock build
+        big_m = max(mols, key=lambda x: x["n"])
+        c = big_m["ctx"]
+        c.profile(True)
+        c.profile_reset()
+        run_one(big_m, MIXED, dev)
+        ms32, cnt32 = c.profile_get("jk_records32")
+        c.profile(False)
+        n_b = big_m["n"]
+        p_b = n_b * (n_b + 1) // 2
+        b32 = 4 * (p_b * (p_b + 1) // 2) + 24 * n_b * n_b
+        if cnt32:
+            mixed["jk_fp32"] = dict(n=n_b, ms_avg=round(ms32 / cnt32, 4), launches=cnt32,
+                                    GBs=round(b32 / (ms32 / cnt32 * 1e-3) / 1e9, 1),
+                                    per_launch=f"4*N_u + 24 n^2 = {b32} bytes")
     for r in results:
         for m in mols:
             if m["k"] == r["k"]:

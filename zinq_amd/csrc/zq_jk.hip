@@ -95,7 +95,10 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
   // this wave's two rows of the item's first quartet: element (r*8+s)*64 + lane, r = 2w + rr
   const E *src = eri + (size_t(Ib) * (Ib + 1) / 2 + size_t(k0)) * ZQ_QUARTET + (2 * wave) * 512 + lane;
   double Jbra = 0.0;
-  double Kps[8], Kqs[8], Pp_s[8], Pq_s[8], cur[16], nxt[16], nxt2[16];
+  // quartets in flight ahead of the contraction, kept in their stored type (4-byte ones leave room for 3)
+  constexpr int D = sizeof(E) == 4 ? 3 : 2;
+  double Kps[8], Kqs[8], Pp_s[8], Pq_s[8], cur[16];
+  E pf[D][16];
   double Pr_cur[4], Pr_nxt[4];  // Prow[br*8+r], Qrow[br*8+r] for the wave's two rows
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
@@ -108,19 +111,21 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
     Pr_cur[2 * rr] = Prow[br * 8 + 2 * wave + rr];
     Pr_cur[2 * rr + 1] = Qrow[br * 8 + 2 * wave + rr];
   }
-  // the quartet stream is read once per build: non-temporal loads, two quartets ahead of the contraction
+  // the quartet stream is read once per build: non-temporal loads, D quartets ahead of the contraction
 #pragma unroll
-  for (int j = 0; j < 16; ++j) cur[j] = __builtin_nontemporal_load(src + j * 64);
-  if (k0 + 1 < k1) {
+  for (int j = 0; j < 16; ++j) cur[j] = double(__builtin_nontemporal_load(src + j * 64));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) nxt[j] = __builtin_nontemporal_load(src + ZQ_QUARTET + j * 64);
-  }
+  for (int d = 0; d + 1 < D; ++d)
+    if (k0 + 1 + d < k1) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) pf[d][j] = __builtin_nontemporal_load(src + size_t(1 + d) * ZQ_QUARTET + j * 64);
+    }
 
   for (int k = k0; k < k1; ++k) {
-    if (k + 2 < k1) {
-      const E *nx2 = src + size_t(k + 2 - k0) * ZQ_QUARTET;
+    if (k + D < k1) {
+      const E *nxd = src + size_t(k + D - k0) * ZQ_QUARTET;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) nxt2[j] = __builtin_nontemporal_load(nx2 + j * 64);
+      for (int j = 0; j < 16; ++j) pf[D - 1][j] = __builtin_nontemporal_load(nxd + j * 64);
     }
     const bool run_end = (br + 1 > (bs <= bq ? bp : bp - 1));
     int nbs = bs, nbr = br + 1;
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
         Pr_nxt[2 * rr] = Prow[nbr * 8 + 2 * wave + rr];
         Pr_nxt[2 * rr + 1] = Qrow[nbr * 8 + 2 * wave + rr];
       }
-      // nxt (quartet k + 1) was issued one iteration earlier
+      // quartet k + 1 is already in flight (pf[0])
     }
     const int Jb = zq_bpair(br, bs);
     const bool diagJ = (br == bs), diagQ = (Jb == Ib);
@@ -199,7 +204,11 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
     }
     bs = nbs, br = nbr;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) cur[j] = nxt[j], nxt[j] = nxt2[j];
+    for (int j = 0; j < 16; ++j) {
+      cur[j] = double(pf[0][j]);
+#pragma unroll
+      for (int d = 0; d + 1 < D; ++d) pf[d][j] = pf[d + 1][j];
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) Pr_cur[j] = Pr_nxt[j];
   }

@@ -23,6 +23,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per zq_jk_records launch measured with FETCH_SIZE / WRITE_SIZE counters, scaled by a 1 GiB
+# copy (profiles/zq_jk_records_water8_hbm_traffic.md), by basis size n
+JK_TRAFFIC = {192: 1561160602}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3}
 METRIC = "MO-transform TFLOP/s + molecules/s, mixed vs FP64 at 1µEh; p50/p99 latency"
 
@@ -378,7 +381,7 @@ def main():
     if dom.startswith("jk"):
         achieved = big["jk_GBs"]
         roof = dict(kernel="zq_jk_records", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=JK_TRAFFIC.get(big["n"]),
                     per_launch=f"8*N_u + 24 n^2 = {jk_bytes(big['n'])} bytes at n={big['n']}")
     else:
         q1_ms = big["q_ms"]["ao2mo_q1"]

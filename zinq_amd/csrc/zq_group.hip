@@ -141,6 +141,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, cons
                                                    BState *__restrict__ st, int *__restrict__ done, int mode, int diis,
                                                    double thr, int maxit) {
   __shared__ double red[2 * (NT / 64)];
+  __shared__ double Am[(kHist + 1) * (kHist + 1)], c[kHist + 1];  // the bordered DIIS system (thread 0)
   const int mi = blockIdx.x, tid = threadIdx.x;
   if (mode == MODE_ITER && done[mi]) return;
   const BMol M = mols[mi];
@@ -209,7 +210,6 @@ __global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, cons
         S.combine = 0;
         if (hn >= 2) {
           const int dd = hn + 1;
-          double Am[(kHist + 1) * (kHist + 1)], c[kHist + 1];
           for (int i = 0; i < dd * dd; ++i) Am[i] = 0.0;
           for (int i = 0; i < dd; ++i) c[i] = 0.0;
           for (int i = 0; i < hn; ++i)

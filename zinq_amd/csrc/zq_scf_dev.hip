@@ -380,31 +380,62 @@ __device__ bool lu_solve(int n, double *A, double *b) {
 __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ part, int diis, double VN,
                                                    ScfState *__restrict__ st) {
   static_assert(kRedBlocks == 64, "one partial per lane");
+  __shared__ double Bs[kMaxHist * kMaxHist];
   if (st->done) return;
   const int lane = threadIdx.x;
+  const int m = st->hist_n, slot = st->cur;
+  int hist[kMaxHist];  // the history in registers: every lane needs it, and st-> reads serialise
+#pragma unroll
+  for (int h = 0; h < kMaxHist; ++h) hist[h] = h < m ? st->hist[h] : -1;
   const double e = zqt::wave_sum(part[lane]);
-  const int slot = st->cur, m = st->hist_n;
-  bool live_k[kMaxHist];
+  double dots[kMaxHist];
+#pragma unroll
   for (int k = 0; k < kMaxHist; ++k) {
     bool live = false;
-    for (int h = 0; h < m; ++h) live |= st->hist[h] == k;
-    live_k[k] = live && diis > 0;
+#pragma unroll
+    for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
+    dots[k] = (live && diis > 0) ? zqt::wave_sum(part[(1 + k) * kRedBlocks + lane]) : 0.0;
   }
-  double dots[kMaxHist];
-  for (int k = 0; k < kMaxHist; ++k) dots[k] = live_k[k] ? zqt::wave_sum(part[(1 + k) * kRedBlocks + lane]) : 0.0;
+  // the m x m block of B over the history, one entry per lane, with the new row and column from dots
+  for (int idx = lane; diis > 0 && m >= 2 && idx < m * m; idx += 64) {
+    const int i = idx / m, j = idx % m;
+    int si = 0, sj = 0;
+#pragma unroll
+    for (int h = 0; h < kMaxHist; ++h) {
+      if (h == i) si = hist[h];
+      if (h == j) sj = hist[h];
+    }
+    double b;
+    if (si == slot || sj == slot) {
+      const int other = si == slot ? sj : si;
+      b = 0.0;
+#pragma unroll
+      for (int k = 0; k < kMaxHist; ++k)
+        if (k == other) b = dots[k];
+    } else {
+      b = st->B[si * kMaxHist + sj];
+    }
+    Bs[i * kMaxHist + j] = b;
+  }
+  __syncthreads();
   if (lane != 0) return;
   st->e_new = e + VN;
   st->combine = 0;
   if (diis <= 0) return;
-  for (int k = 0; k < kMaxHist; ++k)
-    if (live_k[k]) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
+#pragma unroll
+  for (int k = 0; k < kMaxHist; ++k) {
+    bool live = false;
+#pragma unroll
+    for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
+    if (live) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
+  }
   if (m < 2) return;
   const int dd = m + 1;
-  double A[(kMaxHist + 1) * (kMaxHist + 1)], c[kMaxHist + 1];
+  __shared__ double A[(kMaxHist + 1) * (kMaxHist + 1)], c[kMaxHist + 1];  // LDS, not per-lane scratch
   for (int i = 0; i < dd * dd; ++i) A[i] = 0.0;
   for (int i = 0; i < dd; ++i) c[i] = 0.0;
   for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j) A[i * dd + j] = st->B[st->hist[i] * kMaxHist + st->hist[j]];
+    for (int j = 0; j < m; ++j) A[i * dd + j] = Bs[i * kMaxHist + j];
   for (int i = 0; i < m; ++i) A[i * dd + m] = A[m * dd + i] = -1.0;
   c[m] = -1.0;
   if (lu_solve(dd, A, c)) {

@@ -135,36 +135,6 @@ __device__ inline E q1_val(const E *__restrict__ eri, const Q1Src &s, int K, int
   return eri[s.base + idx];
 }
 
-// Scatter form of q1_val for staging: the element stored at position e of the quartet of `s`
-// (row e >> 6, column e & 63) belongs to tile cells (K, u*8 + w).  Every tile cell is reached from
-// exactly one stored element (zeros included), so lanes can read the quartet contiguously and write
-// the LDS tile without conflicts between writers.  Returns the number of cells (0, 1, 2 or 4) in
-// cells[] as K*64 + u*8 + w.
-__device__ inline int q1_cells(const Q1Src &s, int e, int cells[4]) {
-  const int row = e >> 6, col = e & 63;
-  int Ks[2], bls[2], nk = 0;
-  if (s.Ib > s.Jb) {  // stored [K][bra]
-    Ks[0] = row, bls[0] = col, nk = 1;
-  } else if (s.Ib < s.Jb) {  // stored [bra][K]
-    Ks[0] = col, bls[0] = row, nk = 1;
-  } else if (row <= col) {  // diagonal quartet, upper triangle stored: (K, bra) = (row, col) and (col, row)
-    Ks[0] = row, bls[0] = col, nk = 1;
-    if (row != col) Ks[1] = col, bls[1] = row, nk = 2;
-  }
-  int nc = 0;
-  for (int t = 0; t < nk; ++t) {
-    const int pl = bls[t] >> 3, ql = bls[t] & 7;
-    if (s.diagB) {
-      if (pl < ql) continue;  // duplicate of (ql, pl), which writes both cells
-      cells[nc++] = Ks[t] * 64 + pl * 8 + ql;
-      if (pl != ql) cells[nc++] = Ks[t] * 64 + ql * 8 + pl;
-    } else {
-      cells[nc++] = Ks[t] * 64 + (s.swp ? ql * 8 + pl : pl * 8 + ql);
-    }
-  }
-  return nc;
-}
-
 // ---------------------------------------------------------------- Q1, FP64 / FP32
 // Workgroup (512 threads) owns ket block pair Jb (64 slots) x AO block bnu (8 nu) x a group of 64 i.
 // Loop over mu blocks: stage T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then
@@ -185,25 +155,31 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = MF::zero();
 
-  // The next mu block's quartet is read contiguously into registers while the current one is
-  // multiplied, then scattered into the tile (q1_cells).
+  // the next mu block's values are loaded into registers while the current one is multiplied
   T rv[8];
-  Q1Src src = q1_src(0, bnu, br, bs, Jb);
+  {
+    const Q1Src src = q1_src(0, bnu, br, bs, Jb);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) rv[j] = T(eri[src.base + tid + 512 * j]);
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 512 * j;
+      rv[j] = T(q1_val(eri, src, e >> 6, (e & 63) >> 3, e & 7));
+    }
+  }
   for (int bmu = 0; bmu < nb; ++bmu) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      int cells[4];
-      const int nc = q1_cells(src, tid + 512 * j, cells);
-      for (int c = 0; c < nc; ++c) Ts[cells[c] >> 6][cells[c] & 63] = rv[j];
+      const int e = tid + 512 * j;
+      Ts[e >> 6][e & 63] = rv[j];
     }
     __syncthreads();
     if (bmu + 1 < nb) {
-      src = q1_src(bmu + 1, bnu, br, bs, Jb);
+      const Q1Src nsrc = q1_src(bmu + 1, bnu, br, bs, Jb);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) rv[j] = T(eri[src.base + tid + 512 * j]);
+      for (int j = 0; j < 8; ++j) {
+        const int e = tid + 512 * j;
+        rv[j] = T(q1_val(eri, nsrc, e >> 6, (e & 63) >> 3, e & 7));
+      }
     }
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {

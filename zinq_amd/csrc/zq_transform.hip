@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "zq_internal.h"
 
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
   constexpr int TPW = MT * 4;  // tiles per wave
   __shared__ T Ts[64][65];
   const int Jb = blockIdx.x / nb, bnu = blockIdx.x % nb;
-  const int i0 = blockIdx.y * 64;
+  const int i0 = blockIdx.y * (MT * 16);  // MT tiles of 16 occupied orbitals per workgroup
   int br, bs;
   zq_unpair(Jb, br, bs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -155,6 +156,22 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = MF::zero();
 
+  // A wave's tiles span at most two 16-row i tiles (it_lo, it_lo + 1): their C1 operands for a mu block
+  // are four values per lane, loaded with the block instead of once per MFMA.
+  const int it_lo = (wave * TPW) >> 5;
+  auto load_a = [&](int bm, T (&a)[2][2]) {
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int mu = bm * 8 + ks * 4 + (lane >> 4);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = i0 + (it_lo + h) * 16 + (lane & 15);
+        a[ks][h] = (i < m1) ? C1[size_t(mu) * ldc1 + i] : T(0);
+      }
+    }
+  };
+  T a_cur[2][2], a_nxt[2][2];
+  load_a(0, a_cur);
   // the next mu block's values are loaded into registers while the current one is multiplied
   T rv[8];
   {
@@ -180,20 +197,23 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
         const int e = tid + 512 * j;
         rv[j] = T(q1_val(eri, nsrc, e >> 6, (e & 63) >> 3, e & 7));
       }
+      load_a(bmu + 1, a_nxt);
     }
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
-      const int mu = bmu * 8 + ks * 4 + (lane >> 4);
       const int uu = ks * 4 + (lane >> 4);
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
         const int tile = wave * TPW + t, it = tile >> 5, nt = tile & 31;
-        const int i = i0 + it * 16 + (lane & 15);
-        const T a = (i < m1) ? C1[size_t(mu) * ldc1 + i] : T(0);
+        const T a = (it == it_lo) ? a_cur[ks][0] : a_cur[ks][1];
         const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
         acc[t] = MF::mma(a, Ts[K][uu * 8 + wl], acc[t]);
       }
     }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) a_cur[ks][h] = a_nxt[ks][h];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) MF::flush(acc[t]);
   }
@@ -615,10 +635,12 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
   T *O = (T *)zq_scratch(ctx, "OUT", nO * sizeof(T));
   if (!X || !Y || !W || !O) return zq_fail(ctx, ZQ_ERR_OOM, "transform scratch");
 
-  // Q1
+  // Q1: MT tiles of 16 occupied orbitals per workgroup (at most 3 for FP32C, whose FP32 + FP64
+  // accumulator pairs would otherwise spill), ceil(m1 / (16 MT)) workgroups along i
   {
-    const int groups = (m1 + 63) / 64;
-    const int mt = std::min(4, (std::min(m1, 64) + 15) / 16);
+    const int cap = std::is_same<MF, MF32C>::value ? 3 : 4;
+    const int mt = std::min(cap, (std::min(m1, 16 * cap) + 15) / 16);
+    const int groups = precision == ZQ_PREC_BF16X3 ? (m1 + 63) / 64 : (m1 + 16 * mt - 1) / (16 * mt);
     dim3 grid(unsigned(Nb * nb), unsigned(groups));
     zq_prof_begin(ctx, "ao2mo_q1");
     if (precision == ZQ_PREC_BF16X3) {

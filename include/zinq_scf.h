@@ -47,8 +47,8 @@ int zq_scf_run(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double
  * the density P = f X D X^T (X = L^-T from S = L L^T), where D, the projector onto the nocc lowest
  * eigenvectors of X^T F X, comes from SP2 purification (GEMMs on MFMA).  The iteration is captured once
  * as a hipGraph and replayed; the host reads the convergence state one iteration late, so the GPU is
- * never idle between iterations.  After convergence one rocSOLVER dsyevd of the last extrapolated F
- * gives C and e.  A density whose SP2 does not settle within the step budget (no HOMO-LUMO gap) makes
+ * never idle between iterations.  After convergence one zq_eigh decomposition of the last
+ * extrapolated F (in the orthonormal basis) gives C and e.  A density whose SP2 does not settle within the step budget (no HOMO-LUMO gap) makes
  * the run restart on a host-synchronised driver that falls back to dsyevd for that density. */
 int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double *S, double VN,
                       const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);
@@ -58,6 +58,14 @@ int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const
  * host-synchronised driver and raises the budget by 16.  The budget is per context and is not lowered
  * automatically: a context serves molecules of different sizes, and every restart costs a whole SCF. */
 int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
+
+/* GPU symmetric eigensolver, eigh of src/linear_algebra.zig:45-71 (LAPACKE_dsyevd, row-major, 'V'):
+ * A = U diag(W) U^T for the symmetric n x n A (row-major), W ascending, U (n x n row-major) with the
+ * eigenvectors in its columns.  For n <= 192 the tridiagonal reduction is one workgroup holding A in
+ * registers, the tridiagonal problem is rocSOLVER dstedc, and the back-transform one MFMA product;
+ * larger n use rocSOLVER dsyevd.  zq_scf_run_device's final diagonalisation runs the same code on
+ * device memory.  Host arrays; returns ZQ_OK, ZQ_ERR_LAPACK or a GPU error. */
+int zq_eigh(zq_ctx *ctx, size_t n, const double *A, double *W, double *U);
 
 /* Host generalized symmetric-definite eigensolver used by zq_scf_run (the dsygvd stand-in):
  * A x = w B x; on return A holds the eigenvectors (columns), w ascending; B is overwritten.

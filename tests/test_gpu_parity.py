@@ -391,3 +391,36 @@ def test_fock_jk_multi_matches_single_builds(ctx):
     for d in range(3):
         np.testing.assert_array_equal(F[d], ctx.fock_jk(H, Ps[d]))
         np.testing.assert_allclose(G[d], O.get_fock(g, np.zeros_like(H), Ps[d], generalized=True), atol=1e-11)
+
+
+def _check_eigh(A, W, U):
+    n = A.shape[0]
+    nrm = max(1.0, np.abs(A).sum(1).max())
+    tol = 1e-13 * max(n, 8) * nrm
+    assert np.all(np.diff(W) >= 0)
+    assert np.allclose(W, np.linalg.eigvalsh(A), rtol=0, atol=tol)
+    assert np.abs(U.T @ U - np.eye(n)).max() < 1e-13 * max(n, 8)
+    assert np.abs((U * W) @ U.T - A).max() < tol
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 33, 64, 96, 150, 192, 200])
+def test_eigh_matches_lapack(ctx, n):
+    """zq_eigh (the eigh of src/linear_algebra.zig:45-71) against numpy's LAPACK: eigenvalues, U^T U = I
+    and U diag(W) U^T = A to 1e-13 max(n, 8) ||A||_inf.  Sizes cover every register tiling of the
+    one-workgroup reduction (n <= 192); n = 200 takes the dsyevd path."""
+    A = _rand_sym(n, 11 + n)
+    W, U = ctx.eigh(A)
+    _check_eigh(A, W, U)
+
+
+def test_eigh_degenerate_and_fock_like_spectrum(ctx):
+    """Exactly repeated eigenvalues and 1e-10 splittings in a Fock-like spectrum (deep core, valence,
+    virtual bands): the reduction must leave the invariant subspaces intact."""
+    n = 150
+    Q = np.linalg.qr(np.random.default_rng(5).standard_normal((n, n)))[0]
+    lam = np.concatenate([np.full(8, -20.55), -20.5 + 1e-10 * np.arange(4), np.linspace(-1.4, -0.4, 30),
+                          np.repeat([0.2, 0.5, 0.9], 6), np.linspace(1.0, 6.0, n - 60)])
+    A = (Q * lam) @ Q.T
+    A = (A + A.T) / 2
+    W, U = ctx.eigh(A)
+    _check_eigh(A, W, U)

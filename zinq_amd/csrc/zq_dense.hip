@@ -18,10 +18,13 @@
 //   a gap between eigenvalues N and N+1.  idem_k = tr(X_k) - tr(X_k^2) = sum lambda (1 - lambda) >= 0
 //   bounds every eigenvalue's distance to {0, 1}; once idem_k < tol two more steps square every
 //   eigenvalue's distance at least once, and the iteration stops.
-//   One kernel per step: X symmetric, so only tiles ti <= tj are computed and mirrored; the epilogue
-//   applies the branch and writes per-tile partial traces (tr X, ||X||_F^2 = tr X^2), which every
-//   workgroup of the next step sums in the same fixed order, so all of them take the same branch.
-//   X, the partials and the stop state are ping-ponged between two buffers (input [p], output [p^1]).
+//   X symmetric, so only tiles ti <= tj are computed and mirrored; the epilogue applies the branch and
+//   writes per-tile partial traces (tr X, ||X||_F^2 = tr X^2), which every workgroup of the next step
+//   sums in the same fixed order, so all of them take the same branch.  X is ping-ponged between two
+//   buffers (input [p], output [p^1]).
+//   Two forms, bitwise equal: k_sp2_run takes every step in one launch (tiles handed out by ticket, a
+//   step's tiles counted before the next reads X), which the device SCF uses; k_sp2_step is one launch
+//   per step, used by the host-checked driver.
 
 #include <algorithm>
 
@@ -108,10 +111,11 @@ __device__ inline void tile_partials(const double v[4], int ti, int tj, int i0, 
   if (lane == 0) part[t] = make_double2(s1, ti == tj ? s2 : 2.0 * s2);
 }
 
-// X_0 on the m x m padded grid (zero outside n x n), its partials, and the stop state.
+// X_0 on the m x m padded grid (zero outside n x n), its partials, the stop state, and the zeroed
+// counters of the one-launch form.
 __global__ __launch_bounds__(64) void k_sp2_init(const double *__restrict__ F, int n, int m, const double *__restrict__ scal,
                                                  double *__restrict__ X, double2 *__restrict__ part, int *__restrict__ state,
-                                                 const int *__restrict__ skip) {
+                                                 unsigned *__restrict__ sync, const int *__restrict__ skip) {
   if (skip && *skip) return;
   const int T = m / 16;
   int ti, tj;
@@ -130,15 +134,80 @@ __global__ __launch_bounds__(64) void k_sp2_init(const double *__restrict__ F, i
     if (ti != tj) X[size_t(col) * m + row] = x;
   }
   tile_partials(v, ti, tj, i0, j0, part, blockIdx.x);
-  if (blockIdx.x == 0 && lane == 0) state[0] = -1, state[1] = 0, state[4] = 0;
+  if (blockIdx.x == 0) {
+    for (int u = lane; u < 2 * kSp2MaxSteps; u += 64) sync[u] = 0u;
+    if (lane == 0) state[0] = -1, state[1] = 0, state[4] = 0, state[5] = 0;
+  }
+}
+
+// The operands of one SP2 tile (ti <= tj): this lane's X_in elements and, for m <= 256 (PRE), every
+// operand row block of the product, so that the tile costs one memory round trip; larger m walk K with
+// the double-buffered tile_nt instead.
+template <bool PRE>
+struct Sp2Ops {
+  double x[4];
+  Frag f[PRE ? 4 : 1];
+};
+
+template <bool PRE>
+__device__ inline void sp2_load(Sp2Ops<PRE> &o, const double *__restrict__ Xin, int m, int i0, int j0) {
+  const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) o.x[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
+  if (PRE) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) load_block<true>(o.f[b], Xin, m, Xin, m, i0 + r, j0 + r, m, m, m, 64 * b, q);
+  }
+}
+
+// X_out tile = X_in^2 (square) or 2 X_in - X_in^2, stored with its mirror, and its partial traces in pout[t].
+template <bool PRE>
+__device__ inline void sp2_finish(const Sp2Ops<PRE> &o, const double *__restrict__ Xin, double *__restrict__ Xout, int m,
+                                  int ti, int tj, bool square, double2 *__restrict__ pout, int t) {
+  const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  if (PRE) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(o.f[b].a[c][u], o.f[b].b[c][u], acc, 0, 0, 0);
+  } else {
+    acc = tile_nt<true>(Xin, m, Xin, m, i0, j0, m, m, m);
+  }
+  double v[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = square ? acc[j] : 2.0 * o.x[j] - acc[j];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int row = i0 + 4 * j + q, col = j0 + r;
+    Xout[size_t(row) * m + col] = v[j];
+    if (ti != tj) Xout[size_t(col) * m + row] = v[j];
+  }
+  tile_partials(v, ti, tj, i0, j0, pout, t);
+}
+
+// The stop rule and branch of one step from the summed partials (t1 = tr X_k, t2 = tr X_k^2): count < 0
+// before idem < tol was first seen, then the number of steps taken since; the step that brings count to
+// 2 produces the settled projector.
+__device__ inline bool sp2_decide(double t1, double t2, double tol, int nocc, int &count) {
+  const double N = double(nocc);
+  if (count < 0 && t1 - t2 < tol) count = 0;
+  if (count >= 0) ++count;
+  return fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
+}
+
+__device__ inline void sum_partials(const double2 *__restrict__ pin, int ntiles, double &t1, double &t2) {
+  t1 = 0.0, t2 = 0.0;
+  for (int u = threadIdx.x; u < ntiles; u += 64) t1 += pin[u].x, t2 += pin[u].y;
+  t1 = wave_sum(t1);
+  t2 = wave_sum(t2);
 }
 
 // One SP2 step: X_out from X_in (m x m, symmetric), partials and state from [p] to [p^1].
-// state = {count, steps}: count < 0 before idem < tol was first seen, then the number of steps taken
-// since; count >= 2 means settled.  The step that brings count to 2 stores the parity of its output
-// buffer in *final; later steps only carry the state forward and return.
-// For m <= 256 (PRE) every operand of the step is loaded before the first use, so the step costs one
-// memory round trip; larger m walk K with the double-buffered tile_nt.
+// state = {count, steps}; count >= 2 means settled.  The step that brings count to 2 stores the parity
+// of its output buffer in *final; later steps only carry the state forward and return.
 template <bool PRE>
 __global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin, double *__restrict__ Xout, int m, int nocc,
                                                  const double2 *__restrict__ pin, double2 *__restrict__ pout,
@@ -152,52 +221,106 @@ __global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin,
   const int T = m / 16, ntiles = T * (T + 1) / 2;
   int ti, tj;
   tile_of(blockIdx.x, T, ti, tj);
-  const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x, r = lane & 15, q = lane >> 4;
-  // issue every load first: partials, state, this lane's X elements, the operand rows
-  double t1 = 0.0, t2 = 0.0;
-  for (int u = lane; u < ntiles; u += 64) t1 += pin[u].x, t2 += pin[u].y;
+  // issue every load first: state, this tile's operands, then the partials
   int count = sin[0];
   const int steps = sin[1];
-  double x[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) x[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
-  Frag f[PRE ? 4 : 1];
-  if (PRE) {
-#pragma unroll
-    for (int b = 0; b < 4; ++b) load_block<true>(f[b], Xin, m, Xin, m, i0 + r, j0 + r, m, m, m, 64 * b, q);
-  }
-  t1 = wave_sum(t1);
-  t2 = wave_sum(t2);
-  const double N = double(nocc);
-  const double idem = t1 - t2;
-  if (count < 0 && idem < tol) count = 0;
-  if (count >= 0) ++count;
-  if (blockIdx.x == 0 && lane == 0) {
+  Sp2Ops<PRE> o;
+  sp2_load<PRE>(o, Xin, m, ti * 16, tj * 16);
+  double t1, t2;
+  sum_partials(pin, ntiles, t1, t2);
+  const bool square = sp2_decide(t1, t2, tol, nocc, count);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     sout[0] = count, sout[1] = steps + 1;
     if (count == 2) *final = pout_parity;  // X_out is the settled projector
   }
-  const bool square = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
-  d4 acc = {0.0, 0.0, 0.0, 0.0};
-  if (PRE) {
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f[b].a[c][t], f[b].b[c][t], acc, 0, 0, 0);
-  } else {
-    acc = tile_nt<true>(Xin, m, Xin, m, i0, j0, m, m, m);
+  sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, square, pout, blockIdx.x);
+}
+
+// Bounded waits of the one-launch SP2: a wait that runs out raises *abort, and so makes the launch's
+// result count as unsettled; a wait that sees *abort raised gives up too.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+__device__ inline unsigned claim(unsigned *ticket) {
+  unsigned t = 0;
+  if (threadIdx.x == 0) t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __shfl(t, 0, 64);
+}
+
+__device__ inline void publish(unsigned *done) {
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline bool wait_done(const unsigned *done, unsigned target, int *abort) {
+  unsigned spins = 0;
+  while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if ((spins & 63) == 63 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (++spins > kSpinLimit) {
+      __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
   }
-  double v[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) v[j] = square ? acc[j] : 2.0 * x[j] - acc[j];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int row = i0 + 4 * j + q, col = j0 + r;
-    Xout[size_t(row) * m + col] = v[j];
-    if (ti != tj) Xout[size_t(col) * m + row] = v[j];
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __syncthreads();
+  return true;
+}
+
+// Every step of SP2 in one launch.  The tiles of step k are handed out by the ticket counter sync[k];
+// a workgroup takes tiles until none is left, counts each finished one in sync[kSp2MaxSteps + k], and
+// waits for all of step k before it reads X_(k+1).  A wait only ever depends on tiles that running
+// workgroups have taken, so the launch finishes whichever of its workgroups are resident.  Partials are
+// kept per step (part[k] holds those of X_k), so a workgroup that starts late replays the branch and
+// stop decisions of the steps it missed from them; every workgroup sums them in the same order and so
+// takes the same decisions.  Each tile is computed as by k_sp2_step, so the result is bitwise that of
+// the step-per-launch form.  state gets {count, steps} in the pair 2 * (nsteps & 1) and the settled
+// buffer's parity in [4], as k_sp2_step leaves them; [5] is the abort flag.
+template <bool PRE>
+__global__ __launch_bounds__(64) void k_sp2_run(double *__restrict__ X, size_t mm, int m, int nocc,
+                                                double2 *__restrict__ part, unsigned *__restrict__ sync,
+                                                int *__restrict__ state, int nsteps, double tol,
+                                                const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  const int T = m / 16, ntiles = T * (T + 1) / 2;
+  unsigned *ticket = sync, *done = sync + kSp2MaxSteps;
+  int *abort = state + 5;
+  int count = -1, steps = 0, fin = 0;
+  unsigned t = claim(&ticket[0]);
+  for (int k = 0; k < nsteps; ++k) {
+    const double *Xin = X + (k & 1) * mm;
+    double *Xout = X + ((k + 1) & 1) * mm;
+    // the operands of the tile already taken are loaded before the partials, so that both loads overlap
+    Sp2Ops<PRE> o;
+    int ti = 0, tj = 0;
+    if (t < unsigned(ntiles)) {
+      tile_of(int(t), T, ti, tj);
+      sp2_load<PRE>(o, Xin, m, ti * 16, tj * 16);
+    }
+    double t1, t2;
+    sum_partials(part + size_t(k) * ntiles, ntiles, t1, t2);
+    const bool square = sp2_decide(t1, t2, tol, nocc, count);
+    steps = k + 1;
+    if (count == 2) fin = (k + 1) & 1;
+    double2 *pout = part + size_t(k + 1) * ntiles;
+    while (t < unsigned(ntiles)) {
+      sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, square, pout, int(t));
+      publish(&done[k]);
+      t = claim(&ticket[k]);
+      if (t < unsigned(ntiles)) {
+        tile_of(int(t), T, ti, tj);
+        sp2_load<PRE>(o, Xin, m, ti * 16, tj * 16);
+      }
+    }
+    if (count >= 2 || k + 1 == nsteps) break;  // the launch's end publishes the last step
+    t = claim(&ticket[k + 1]);
+    if (!wait_done(&done[k], unsigned(ntiles), abort)) break;
   }
-  tile_partials(v, ti, tj, i0, j0, pout, blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int *pair = state + 2 * (nsteps & 1);
+    pair[0] = count, pair[1] = steps;
+    state[4] = fin;
+  }
 }
 
 }  // namespace
@@ -223,26 +346,31 @@ int zq_gemm_nt(zq_ctx *ctx, const double *A, int lda, const double *B, int ldb, 
 
 namespace {
 
+// X: two m x m buffers; part: the partials of X_0 .. X_kSp2MaxSteps (ntiles each; the step-per-launch
+// form ping-pongs between the first two); sync: the tickets and finished counts of the one-launch form.
 struct Sp2Bufs {
   int m, T, ntiles;
   size_t mm;
   double *X, *scal;
   double2 *part;
+  unsigned *sync;
 };
 
 int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
   b.m = (n + 15) / 16 * 16, b.T = b.m / 16, b.ntiles = b.T * (b.T + 1) / 2;
   b.mm = size_t(b.m) * b.m;
   b.X = (double *)zq_scratch(ctx, "sp2_X", 2 * b.mm * sizeof(double));
-  b.part = (double2 *)zq_scratch(ctx, "sp2_part", 2 * size_t(b.ntiles) * sizeof(double2));
+  b.part = (double2 *)zq_scratch(ctx, "sp2_part", size_t(kSp2MaxSteps + 1) * b.ntiles * sizeof(double2));
   b.scal = (double *)zq_scratch(ctx, "sp2_scal", 2 * sizeof(double));
-  if (!b.X || !b.part || !b.scal) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  b.sync = (unsigned *)zq_scratch(ctx, "sp2_sync", 2 * kSp2MaxSteps * sizeof(unsigned));
+  if (!b.X || !b.part || !b.scal || !b.sync) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
   return ZQ_OK;
 }
 
 void sp2_start(zq_ctx *ctx, const Sp2Bufs &b, const double *dF, int n, int *state, const int *skip) {
   hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(std::min(1024, (n + 63) / 64 * 64)), 0, ctx->stream, dF, n, b.scal, skip);
-  hipLaunchKernelGGL(k_sp2_init, dim3(b.ntiles), dim3(64), 0, ctx->stream, dF, n, b.m, b.scal, b.X, b.part, state, skip);
+  hipLaunchKernelGGL(k_sp2_init, dim3(b.ntiles), dim3(64), 0, ctx->stream, dF, n, b.m, b.scal, b.X, b.part, state, b.sync,
+                     skip);
 }
 
 void sp2_step(zq_ctx *ctx, const Sp2Bufs &b, int k, int nocc, double tol, int *state, const int *skip) {
@@ -260,15 +388,22 @@ void sp2_step(zq_ctx *ctx, const Sp2Bufs &b, int k, int nocc, double tol, int *s
 
 }  // namespace
 
-int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state5,
+int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state6,
                    const double **dX, long long *stride, int *ld) {
+  if (nsteps < 1 || nsteps > kSp2MaxSteps) return zq_fail(ctx, ZQ_ERR_INVALID, "SP2 step budget out of range");
   Sp2Bufs b;
   if (int rc = sp2_bufs(ctx, n, b)) return rc;
   zq_prof_begin(ctx, "scf_sp2");
-  sp2_start(ctx, b, dF, n, state5, skip);
-  for (int k = 0; k < nsteps; ++k) sp2_step(ctx, b, k, nocc, tol, state5, skip);
+  sp2_start(ctx, b, dF, n, state6, skip);
+  // one workgroup per tile: each normally takes one tile a step
+  if (b.m <= 256)
+    hipLaunchKernelGGL(k_sp2_run<true>, dim3(b.ntiles), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc, b.part, b.sync,
+                       state6, nsteps, tol, skip);
+  else
+    hipLaunchKernelGGL(k_sp2_run<false>, dim3(std::min(b.ntiles, 1024)), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc,
+                       b.part, b.sync, state6, nsteps, tol, skip);
   zq_prof_end(ctx);
-  ZQ_LAUNCH_CHECK(ctx, "k_sp2_step");
+  ZQ_LAUNCH_CHECK(ctx, "k_sp2_run");
   *dX = b.X;
   *stride = (long long)b.mm;
   *ld = b.m;
@@ -279,7 +414,7 @@ int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, i
                    const double **dD, int *ld, int *steps) {
   Sp2Bufs b;
   if (int rc = sp2_bufs(ctx, n, b)) return rc;
-  auto *state = (int *)zq_scratch(ctx, "sp2_state", 5 * sizeof(int));
+  auto *state = (int *)zq_scratch(ctx, "sp2_state", 6 * sizeof(int));
   if (!state) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
   zq_prof_begin(ctx, "scf_sp2");
   sp2_start(ctx, b, dF, n, state, nullptr);

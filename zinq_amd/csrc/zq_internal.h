@@ -182,12 +182,17 @@ int zq_gemm_nt(zq_ctx *ctx, const double *A, int lda, const double *B, int ldb, 
 // steps before the first convergence check and is updated on success.
 int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int max_steps, int *steps_hint,
                    const double **dD, int *ld, int *steps);  // zq_dense.hip
-// Enqueues SP2 with a fixed budget of nsteps steps and no host synchronisation.  state5 = two ping-pong
-// {count, steps} pairs and the parity of the buffer that holds the settled projector: the projector is
-// *dX + state5[4] * *stride (m x m, ld m), which a kernel selects at run time.  The final pair is
-// state5 + 2 * (nsteps & 1); count >= 2 there means settled.  All kernels return at once while
-// *skip != 0.
-int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state5,
+// Enqueues SP2 with a budget of nsteps <= kSp2MaxSteps steps (one launch that stops once settled) and no
+// host synchronisation.  state6 = two {count, steps} pairs, the parity of the buffer that holds the
+// settled projector, and an abort flag: the projector is *dX + state6[4] * *stride (m x m, ld m), which
+// a kernel selects at run time.  The final pair is state6 + 2 * (nsteps & 1); the result is settled when
+// count >= 2 there and state6[5] == 0.  All kernels return at once while *skip != 0.
+constexpr int kSp2MaxSteps = 128;
+// Eigen-decomposition of the symmetric n x n A (ld n, overwritten): w = eigenvalues ascending, row k of
+// A = eigenvector k.  offd: n doubles of workspace; info: device int (0 on success).  n <= 192 by the
+// one-workgroup tridiagonal reduction of zq_eigen.hip and rocSOLVER dstedc, larger n by dsyevd.
+int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *info);  // zq_eigen.hip
+int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state6,
                    const double **dX, long long *stride, int *ld);  // zq_dense.hip
 
 #define ZQ_TRY(ctx, call)                                \

@@ -193,16 +193,6 @@ int alloc(zq_ctx *ctx, Dev &d, int n, int maxh) {
   return ZQ_OK;
 }
 
-rocblas_handle solver(zq_ctx *ctx) {
-  if (!ctx->solver) {
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    rocblas_set_stream(h, ctx->stream);
-    ctx->solver = h;
-  }
-  return (rocblas_handle)ctx->solver;
-}
-
 // Fp = X^T A X for the symmetric n x n A (X^T stored as XT, so both products are A B^T forms).
 int to_orthonormal(Dev &d, const double *A) {
   const int n = d.n;
@@ -210,19 +200,13 @@ int to_orthonormal(Dev &d, const double *A) {
   return zq_gemm_nt(d.ctx, d.T, n, d.XT, n, d.Fp, n, n, n, n, 1.0, "scf_gemm");                 // Fp = T X
 }
 
-// Eigen-decomposition of Fp (symmetric, column-major == row-major) by rocSOLVER dsyevd, C = X V,
-// eigenvalues ascending in d.eig.  Fp is overwritten by the eigenvectors.
+// Eigen-decomposition of Fp (symmetric) by zq_syev_rows, C = X V, eigenvalues ascending in d.eig.
+// Fp is overwritten by the eigenvectors.
 int diag_orthonormal(Dev &d, double *Cout) {
   zq_ctx *ctx = d.ctx;
   const int n = d.n;
-  rocblas_handle h = solver(ctx);
-  if (!h) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocblas handle");
-  zq_prof_begin(ctx, "scf_eig");
-  const rocblas_status st =
-      rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, d.Fp, n, d.eig, d.offd, d.info);
-  zq_prof_end(ctx);
-  if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dsyevd");
-  // eigenvector k is row k of the column-major result: C[i][k] = sum_p X[i][p] V[k][p]
+  if (int rc = zq_syev_rows(ctx, d.Fp, n, d.eig, d.offd, d.info)) return rc;
+  // eigenvector k is row k: C[i][k] = sum_p X[i][p] V[k][p]
   return zq_gemm_nt(ctx, d.X, n, d.Fp, n, Cout, n, n, n, n, 1.0, "scf_gemm");
 }
 
@@ -271,7 +255,7 @@ struct ScfState {
   int sp2_max;    // most SP2 steps of any density so far
   int hist_n, cur, combine, pad;
   int hist[kMaxHist];  // DIIS slots, oldest first
-  int sp2[6];          // SP2 ping-pong {count, steps} pairs and the settled buffer's parity
+  int sp2[6];          // SP2 {count, steps} pairs, the settled buffer's parity, the abort flag
   double e_old, e_new, rms, dE;
   double coef[kMaxHist + 1];
   double B[kMaxHist * kMaxHist];  // B[slot][slot] = <err_slot, err_slot'>
@@ -472,11 +456,14 @@ __global__ void k_combine_state(const double *__restrict__ Fh, const double *__r
   }
 }
 
-__global__ void k_sp2_check(const int *__restrict__ fin, ScfState *__restrict__ st) {
+// The SP2 density of this iteration: steps taken, and done = 3 (the caller falls back to the
+// host-checked driver) when it did not settle in its budget or a bounded wait gave up.
+__global__ void k_sp2_check(int parity, ScfState *__restrict__ st) {
   if (threadIdx.x != 0 || st->done) return;
+  const int *fin = st->sp2 + 2 * parity;
   st->sp2_steps = fin[1];
   st->sp2_max = max(st->sp2_max, fin[1]);
-  if (fin[0] < 2) st->done = 3;
+  if (fin[0] < 2 || st->sp2[5] != 0) st->done = 3;
 }
 
 // rms(dP) and the convergence test of src/hartree_fock.zig:853-870 (one wave, butterfly sum).
@@ -509,7 +496,7 @@ int density_async(Dev &d, const double *A, int nocc, double factor, int nsteps, 
   long long stride = 0;
   int ld = 0;
   if (int rc = zq_sp2_enqueue(ctx, d.Fp, n, nocc, 1e-9, nsteps, skip, st->sp2, &D, &stride, &ld)) return rc;
-  hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, ctx->stream, st->sp2 + 2 * (nsteps & 1), st);
+  hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, ctx->stream, nsteps & 1, st);
   if (int rc = zq_gemm_nt(ctx, d.X, n, D, ld, d.T, n, n, n, n, 1.0, "scf_gemm", skip, st->sp2 + 4, stride)) return rc;
   if (int rc = zq_gemm_nt(ctx, d.T, n, d.X, n, d.Pn, n, n, n, n, factor, "scf_gemm", skip)) return rc;
   hipLaunchKernelGGL(k_store_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.Pn, n, ctx->d_P, np, part_rms, skip,

@@ -4,15 +4,16 @@
 // rocSOLVER's dsyevd spends most of its time at these sizes launching the many small kernels of its
 // tridiagonal reduction (about 2.4 ms of 3.9 ms at n = 192).  Here the reduction is one workgroup:
 //
-// k_tridiag: 1024 threads hold the n x n matrix in registers, thread (tr, tc) = (tid / 32, tid % 32)
-//   owning rows tr + 32 a and columns tc + 32 b (a, b < R = ceil(n / 32)).  A wave holds two thread
-//   rows, so every row reduction is a 32-lane shuffle reduction inside a half-wave.  Householder
-//   reduction of the lower triangle as LAPACK dsytd2 (A = Q T Q^T, Q = H_0 ... H_(n-3)), step k:
-//     column k -> LDS; alpha = A[k+1][k], beta = -sign(alpha) |A[k+1:][k]|, tau = (beta - alpha) / beta,
+// k_tridiag: 512 threads hold the n x n matrix in registers (thread (tr, tc) = (tid / 8, tid % 8) owns
+//   rows tr + 64 a and columns tc + 8 b), two waves per SIMD, so the per-step scalar work is issued
+//   twice per SIMD rather than for every one of many waves.  Householder reduction of the lower
+//   triangle as LAPACK dsytd2 (A = Q T Q^T, Q = H_0 ... H_(n-3)), step k:
+//     row k -> LDS; alpha = A[k+1][k], beta = -sign(alpha) |A[k+1:][k]|, tau = (beta - alpha) / beta,
 //     v = (1, A[k+2:][k] / (alpha - beta));  p = tau A v;  w = p - tau/2 (p.v) v;  A -= v w^T + w v^T,
 //   two barriers a step; every wave computes the step's scalars itself, in the same order, so all agree
-//   bitwise.  The reflectors go to global memory, and Q^T = H_(n-3) ... H_0 is then accumulated in
-//   the same registers by right multiplications M <- M H_k, whose row sums need no barrier.
+//   bitwise; column blocks wholly left of k are skipped.  The reflectors stay in LDS, and Q^T =
+//   H_(n-3) ... H_0 is then accumulated in the same registers by right multiplications M <- M H_k,
+//   whose row sums (quad sums) need no barrier.
 // The tridiagonal problem T = Z diag(w) Z^T is rocSOLVER dstedc's divide and conquer, and the
 // eigenvectors Q Z are one MFMA product.
 
@@ -27,180 +28,228 @@
 
 namespace {
 
-constexpr int kMaxR = 6;  // n <= 32 * kMaxR: the matrix and the step's vectors fit 128 VGPRs a lane
+constexpr int kTridiagMax = 192;  // 16 * the largest RB instantiated
 
-// row[q] for a run-time q, as a sum with 0/1 weights: a select between elements would be folded into
-// an indexed load, which moves the whole register array to scratch memory.
-template <int R>
-__device__ inline double pick(const double (&row)[R], int q) {
-  double v = 0.0;
-#pragma unroll
-  for (int t = 0; t < R; ++t) v = fma(t == q ? 1.0 : 0.0, row[t], v);
+// DPP lane moves within a 16-lane row; a source outside the row reads 0.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of a DPP row, complete in the row's last lane (tc == 15).
+__device__ inline double row_sum_last(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm 1 0 3 2
+  v += dpp_move<0x4E>(v);   // quad_perm 2 3 0 1: every lane holds its quad's sum
+  v += dpp_move<0x114>(v);  // row_shr 4
+  v += dpp_move<0x118>(v);  // row_shr 8: lanes 12..15 hold the row's sum
   return v;
 }
 
-template <int R>
-__device__ inline double pick2(const double (&m)[R][R], int ri, int ci) {
-  double v = 0.0;
-#pragma unroll
-  for (int t = 0; t < R; ++t) v = fma(t == ri ? 1.0 : 0.0, pick<R>(m[t], ci), v);
-  return v;
-}
+// Offset of reflector k in the packed store: entries k+1 .. n-1.
+__device__ inline int vofs(int k, int n) { return k * (n - 1) - k * (k - 1) / 2; }
 
-__device__ inline double half_sum(double v) {  // sum over the 32 lanes of this half-wave
-#pragma unroll
-  for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ inline double wave_sum64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// A (n x n symmetric, ld lda) -> d (n), e (n - 1), Q (n x n row-major, ld n: A = Q T Q^T), with Vh
-// (n x n) holding the reflectors.  One workgroup per matrix of a strided batch.
-template <int R>
-__global__ __launch_bounds__(1024) void k_tridiag(const double *__restrict__ A, int lda, long long sA, int n,
+// A (n x n symmetric, ld lda) -> d (n), e (n - 1), Q (n x n row-major, ld n: A = Q T Q^T), n <= 16 RB.
+// One workgroup of 16 TR threads per matrix of a strided batch.  Thread (tr, tc) = (tid / 16, tid % 16)
+// owns rows tr + TR a (a < RA) and columns tc + 16 b (b < RB), at most 72 doubles (TR = 32 keeps two
+// waves per SIMD, so they fit the 256 registers a lane then has); a DPP row of 16 lanes holds one
+// thread row, so a row sum is four DPP steps.  Row k (= column k, A symmetric) reaches
+// LDS from the 16 lanes that own it, together with |A[k+2:][k]|^2, which they sum on the way.  p.v is
+// summed per wave into LDS before the step's second barrier and then added up in a fixed order, so no
+// step waits on a 64-lane shuffle reduction.  Row vectors in LDS are stored column-permuted,
+// [c % 16][c / 16] with a padded stride, so a lane reads its own columns contiguously and the 16
+// column groups fall in different banks.  The reflectors (raw row tails and 1 / (alpha - beta)), d, e
+// and tau stay in LDS (at most 156 KB for n = 192), so no barrier waits for a global store.
+template <int RB, int TR>
+__global__ __launch_bounds__(16 * TR) void k_tridiag(const double *__restrict__ A, int lda, long long sA, int n,
                                                   double *__restrict__ d, double *__restrict__ e, long long sDE,
-                                                  double *__restrict__ Q, long long sQ, double *__restrict__ Vh) {
-  __shared__ double col[2][32 * R];
-  __shared__ double pv[32 * R];
-  __shared__ double taus[32 * R];
+                                                  double *__restrict__ Q, long long sQ) {
+  constexpr int NT = 16 * TR, NM = 16 * RB, RA = (NM + TR - 1) / TR, CS = RB + 1;
+  __shared__ double cbp[2][16 * CS];  // row k, permuted
+  __shared__ double pvp[16 * CS];     // p = tau A v, permuted
+  __shared__ double xn[2], pdot[2][TR / 4];
+  __shared__ double taus[NM], scals[NM], dl[NM], el[NM];
+  __shared__ double vl[(NM - 2) * (NM + 1) / 2];
   A += blockIdx.x * sA;
   d += blockIdx.x * sDE;
   e += blockIdx.x * sDE;
   Q += blockIdx.x * sQ;
-  Vh += blockIdx.x * sQ;
-  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31, lane = tid & 63;
-  double a[R][R];
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15, wv = tid >> 6;
+  auto P = [](int j) { return (j & 15) * CS + (j >> 4); };
+  double a[RA][RB];
 #pragma unroll
-  for (int ai = 0; ai < R; ++ai)
+  for (int ai = 0; ai < RA; ++ai)
 #pragma unroll
-    for (int bj = 0; bj < R; ++bj) {
-      const int r = tr + 32 * ai, c = tc + 32 * bj;
+    for (int bj = 0; bj < RB; ++bj) {
+      const int r = tr + TR * ai, c = tc + 16 * bj;
       a[ai][bj] = (r < n && c < n) ? A[size_t(r) * lda + c] : 0.0;
     }
+  // p stays 0 on the padding columns (no row >= n writes it), so the update leaves them 0
+  for (int t = tid; t < 16 * CS; t += NT) pvp[t] = 0.0;
+  // row k of the registers -> dst (permuted) by the 16 lanes that own it, and |A[k][k+2:]|^2 -> *xsq.
+  // (the row block is picked with 0/1 weights: branches per block get merged into an indexed access,
+  // which moves the register array to scratch memory)
+  auto gather_row = [&](int k, double *dst, double *xsq) {
+    if (tr == k % TR) {
+      const int ak = k / TR;
+      double s2 = 0.0;
+#pragma unroll
+      for (int bj = 0; bj < RB; ++bj) {
+        double x = 0.0;
+#pragma unroll
+        for (int ai = 0; ai < RA; ++ai) x = fma(ai == ak ? 1.0 : 0.0, a[ai][bj], x);
+        dst[tc * CS + bj] = x;
+        s2 += tc + 16 * bj >= k + 2 ? x * x : 0.0;
+      }
+      s2 = row_sum_last(s2);
+      if (tc == 15 && xsq) *xsq = s2;
+    }
+  };
 
   for (int k = 0; k + 2 < n; ++k) {
-    double *cb = col[k & 1];
-    if (tc == (k & 31)) {
-#pragma unroll
-      for (int ai = 0; ai < R; ++ai) cb[tr + 32 * ai] = pick<R>(a[ai], k >> 5);
-    }
+    const int par = k & 1;
+    double *cb = cbp[par];
+    gather_row(k, cb, &xn[par]);
     __syncthreads();
-    const double alpha = cb[k + 1];
-    double s = 0.0;
-    for (int i = k + 2 + lane; i < n; i += 64) s += cb[i] * cb[i];
-    const double xnorm2 = wave_sum64(s);
+    const double alpha = cb[P(k + 1)], xnorm2 = xn[par];
     double beta = alpha, tau = 0.0, scal = 0.0;
     if (xnorm2 != 0.0) {
       beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
       tau = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
     }
-    auto vval = [&](int i) { return i == k + 1 ? 1.0 : (i > k + 1 && i < n ? cb[i] * scal : 0.0); };
     if (tid == 0) {
-      d[k] = cb[k];
-      e[k] = beta;
+      dl[k] = cb[P(k)];
+      el[k] = beta;
       taus[k] = tau;
+      scals[k] = scal;
     }
-    if (tid < n) Vh[size_t(k) * n + tid] = vval(tid);
-    double vc[R];
+    // raw tail of row k for the back-accumulation: v_k[j] = scal * tail[j] for j > k + 1, v_k[k+1] = 1
+    for (int j = k + 1 + tid; j < n; j += NT) vl[vofs(k, n) + j - k - 1] = cb[P(j)];
+    auto vval = [&](int i) { return i == k + 1 ? 1.0 : (i > k + 1 && i < n ? cb[P(i)] * scal : 0.0); };
+    // v on this lane's columns (0 at and left of k), read from the row in LDS at each use: registers
+    // hold A
+    auto vcol = [&](int bj) {
+      const int j = tc + 16 * bj;
+      const double x = cb[tc * CS + bj] * scal;
+      return j > k + 1 ? x : (j == k + 1 ? 1.0 : 0.0);
+    };
+    // p = tau A v on this lane's rows; the row's last lane stores p_i and adds p_i v_i to its partial
+    double pt[RA];
 #pragma unroll
-    for (int bj = 0; bj < R; ++bj) vc[bj] = vval(tc + 32 * bj);
-    // p = tau A v on the rows below k (rows above are not used)
+    for (int ai = 0; ai < RA; ++ai) pt[ai] = 0.0;
 #pragma unroll
-    for (int ai = 0; ai < R; ++ai) {
-      if (32 * ai + 31 <= k) continue;
-      double t = 0.0;
+    for (int bj = 0; bj < RB; ++bj) {
+      const double v = vcol(bj);
 #pragma unroll
-      for (int bj = 0; bj < R; ++bj) t += a[ai][bj] * vc[bj];
-      t = half_sum(t);
-      if (tc == 0) pv[tr + 32 * ai] = tau * t;
+      for (int ai = 0; ai < RA; ++ai) pt[ai] += a[ai][bj] * v;
+      if (bj % 4 == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight (registers hold A)
     }
+    double pd = 0.0;
+#pragma unroll
+    for (int ai = 0; ai < RA; ++ai) {
+      const double t = tau * row_sum_last(pt[ai]);
+      const int r = tr + TR * ai;
+      if (tc == 15 && r > k && r < n) {
+        pvp[P(r)] = t;
+        pd += t * vval(r);
+      }
+    }
+    // per-wave sum of p.v over the wave's 4 row-last lanes (15, 31, 47, 63)
+    pd = pd + __shfl_xor(pd, 16, 64);
+    pd = pd + __shfl_xor(pd, 32, 64);
+    if ((tid & 63) == 63) pdot[par][wv] = pd;
     __syncthreads();
-    s = 0.0;
-    for (int i = k + 1 + lane; i < n; i += 64) s += pv[i] * vval(i);
-    const double K = -0.5 * tau * wave_sum64(s);
-    double wc[R];
+    double dot = 0.0;
 #pragma unroll
-    for (int bj = 0; bj < R; ++bj) {
-      const int j = tc + 32 * bj;
-      wc[bj] = (j > k && j < n) ? pv[j] + K * vc[bj] : 0.0;
+    for (int q = 0; q < TR / 4; ++q) dot += pdot[par][q];
+    const double K = -0.5 * tau * dot;
+    double vr[RA], wr[RA];
+#pragma unroll
+    for (int ai = 0; ai < RA; ++ai) {
+      const int r = tr + TR * ai;
+      vr[ai] = vval(r);
+      wr[ai] = (r > k && r < n) ? pvp[P(r)] + K * vr[ai] : 0.0;
     }
+    // A -= v w^T + w v^T on the trailing block (w = 0 at and left of k)
 #pragma unroll
-    for (int ai = 0; ai < R; ++ai) {
-      if (32 * ai + 31 <= k) continue;
-      const int i = tr + 32 * ai;
-      const double vr = vval(i), wr = (i > k && i < n) ? pv[i] + K * vr : 0.0;
+    for (int bj = 0; bj < RB; ++bj) {
+      const double v = vcol(bj), w = tc + 16 * bj > k ? pvp[tc * CS + bj] + K * v : 0.0;
 #pragma unroll
-      for (int bj = 0; bj < R; ++bj) a[ai][bj] -= vr * wc[bj] + wr * vc[bj];
+      for (int ai = 0; ai < RA; ++ai) a[ai][bj] -= vr[ai] * w + wr[ai] * v;
+      if (bj % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // the last 2 x 2 block
+  // the last 2 x 2 block (or the 1 x 1 matrix)
   __syncthreads();
-  if (n >= 2) {
-    if (tr == ((n - 2) & 31) && tc == ((n - 2) & 31)) d[n - 2] = pick2<R>(a, (n - 2) >> 5, (n - 2) >> 5);
-    if (tr == ((n - 1) & 31) && tc == ((n - 2) & 31)) e[n - 2] = pick2<R>(a, (n - 1) >> 5, (n - 2) >> 5);
+  gather_row(n >= 2 ? n - 2 : 0, cbp[0], nullptr);
+  if (n >= 2) gather_row(n - 1, cbp[1], nullptr);
+  __syncthreads();
+  if (tid == 0) {
+    if (n >= 2) {
+      dl[n - 2] = cbp[0][P(n - 2)];
+      el[n - 2] = cbp[1][P(n - 2)];
+      dl[n - 1] = cbp[1][P(n - 1)];
+    } else {
+      dl[0] = cbp[0][P(0)];
+    }
   }
-  if (tr == ((n - 1) & 31) && tc == ((n - 1) & 31)) d[n - 1] = pick2<R>(a, (n - 1) >> 5, (n - 1) >> 5);
-  __syncthreads();  // the reflectors in Vh and taus are complete
+  __syncthreads();  // reflectors, taus, d and e complete
+  for (int t = tid; t < n; t += NT) d[t] = dl[t];
+  for (int t = tid; t + 1 < n; t += NT) e[t] = el[t];
 
   // M = Q^T = H_(n-3) ... H_0, from M = I by M <- M H_k = M - tau (M v) v^T for k = n-3 .. 0
 #pragma unroll
-  for (int ai = 0; ai < R; ++ai)
+  for (int ai = 0; ai < RA; ++ai)
 #pragma unroll
-    for (int bj = 0; bj < R; ++bj) {
-      const int r = tr + 32 * ai, c = tc + 32 * bj;
+    for (int bj = 0; bj < RB; ++bj) {
+      const int r = tr + TR * ai, c = tc + 16 * bj;
       a[ai][bj] = (r == c && r < n) ? 1.0 : 0.0;
     }
-  double vn[R];
-  if (n >= 3) {
-#pragma unroll
-    for (int bj = 0; bj < R; ++bj) {
-      const int j = tc + 32 * bj;
-      vn[bj] = j < n ? Vh[size_t(n - 3) * n + j] : 0.0;
-    }
-  }
   for (int k = n - 3; k >= 0; --k) {
-    double vc[R];
+    const double tau = taus[k], scal = scals[k];
+    const double *tail = vl + vofs(k, n) - k - 1;  // tail[j], j > k
+    auto vcol = [&](int bj) {
+      const int j = tc + 16 * bj, jc = min(max(j, k + 1), n - 1);  // reads stay inside reflector k
+      const double x = tail[jc] * scal;
+      return (j > k + 1 && j < n) ? x : (j == k + 1 ? 1.0 : 0.0);
+    };
+    // rows <= k of M are still those of I and v vanishes there, so their u is 0
+    double u[RA];
 #pragma unroll
-    for (int bj = 0; bj < R; ++bj) vc[bj] = vn[bj];
-    if (k > 0) {  // the next reflector's loads overlap this step
+    for (int ai = 0; ai < RA; ++ai) u[ai] = 0.0;
 #pragma unroll
-      for (int bj = 0; bj < R; ++bj) {
-        const int j = tc + 32 * bj;
-        vn[bj] = j < n ? Vh[size_t(k - 1) * n + j] : 0.0;
-      }
+    for (int bj = 0; bj < RB; ++bj) {
+      const double v = vcol(bj);
+#pragma unroll
+      for (int ai = 0; ai < RA; ++ai) u[ai] += a[ai][bj] * v;
+      if (bj % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
-    const double tau = taus[k];
 #pragma unroll
-    for (int ai = 0; ai < R; ++ai) {
-      if (32 * ai + 31 <= k) continue;  // rows <= k of M are still those of I, and v vanishes there
-      double u = 0.0;
+    for (int ai = 0; ai < RA; ++ai) u[ai] = tau * __shfl(row_sum_last(u[ai]), (tid & ~15) | 15, 64);
 #pragma unroll
-      for (int bj = 0; bj < R; ++bj) u += a[ai][bj] * vc[bj];
-      u = tau * half_sum(u);
+    for (int bj = 0; bj < RB; ++bj) {
+      const double v = vcol(bj);
 #pragma unroll
-      for (int bj = 0; bj < R; ++bj) a[ai][bj] -= u * vc[bj];
+      for (int ai = 0; ai < RA; ++ai) a[ai][bj] -= u[ai] * v;
+      if (bj % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
   // Q[c][r] = M[r][c]
 #pragma unroll
-  for (int ai = 0; ai < R; ++ai)
+  for (int ai = 0; ai < RA; ++ai)
 #pragma unroll
-    for (int bj = 0; bj < R; ++bj) {
-      const int r = tr + 32 * ai, c = tc + 32 * bj;
+    for (int bj = 0; bj < RB; ++bj) {
+      const int r = tr + TR * ai, c = tc + 16 * bj;
       if (r < n && c < n) Q[size_t(c) * n + r] = a[ai][bj];
     }
 }
 
-template <int R>
+template <int RB, int TR>
 void launch_tridiag(hipStream_t s, int count, const double *A, int lda, long long sA, int n, double *d, double *e,
-                    long long sDE, double *Q, long long sQ, double *Vh) {
-  hipLaunchKernelGGL(k_tridiag<R>, dim3(count), dim3(1024), 0, s, A, lda, sA, n, d, e, sDE, Q, sQ, Vh);
+                    long long sDE, double *Q, long long sQ) {
+  hipLaunchKernelGGL((k_tridiag<RB, TR>), dim3(count), dim3(16 * TR), 0, s, A, lda, sA, n, d, e, sDE, Q, sQ);
 }
 
 rocblas_handle eig_handle(zq_ctx *ctx) {
@@ -220,7 +269,7 @@ int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *in
   rocblas_handle h = eig_handle(ctx);
   if (!h) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocblas handle");
   zq_prof_begin(ctx, "scf_eig");
-  if (n > 32 * kMaxR) {
+  if (n > kTridiagMax) {
     const rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, offd, info);
     zq_prof_end(ctx);
     if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dsyevd");
@@ -228,22 +277,18 @@ int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *in
   }
   const size_t nn = size_t(n) * n;
   auto *Qm = (double *)zq_scratch(ctx, "eig_Q", nn * sizeof(double));
-  auto *Vh = (double *)zq_scratch(ctx, "eig_V", nn * sizeof(double));
   auto *Z = (double *)zq_scratch(ctx, "eig_Z", nn * sizeof(double));
-  if (!Qm || !Vh || !Z) {
+  if (!Qm || !Z) {
     zq_prof_end(ctx);
     return zq_fail(ctx, ZQ_ERR_OOM, "eigen-solver buffers");
   }
-  const int R = (n + 31) / 32;
   hipStream_t s = ctx->stream;
-  switch (R) {
-    case 1: launch_tridiag<1>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0, Vh); break;
-    case 2: launch_tridiag<2>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0, Vh); break;
-    case 3: launch_tridiag<3>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0, Vh); break;
-    case 4: launch_tridiag<4>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0, Vh); break;
-    case 5: launch_tridiag<5>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0, Vh); break;
-    default: launch_tridiag<6>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0, Vh); break;
-  }
+  if (n <= 64)
+    launch_tridiag<4, 64>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0);
+  else if (n <= 128)
+    launch_tridiag<8, 64>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0);
+  else
+    launch_tridiag<12, 32>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0);
   ZQ_LAUNCH_CHECK(ctx, "k_tridiag");
   // T = Z diag(w) Z^T: column k of the column-major Z (= row k of Z as stored) is eigenvector k
   const rocblas_status st = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, offd, Z, n, info);

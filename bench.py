@@ -71,8 +71,9 @@ MIXED = dict(fock="fp32", transform="fp32c", threshold=1e-6)
 def run_one(m, mode, device_scf):
     r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=mode["threshold"],
                      device=device_scf, fock_precision=mode["fock"] if device_scf else "fp64")
+    driver = m["ctx"].last_scf_driver() if device_scf else "host"
     e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], mode["transform"])
-    return dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"])
+    return dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"], scf_driver=driver)
 
 
 def step(mols, mode, results=None, device_scf=True, concurrent=True):

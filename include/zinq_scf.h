@@ -53,11 +53,14 @@ int zq_scf_run(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double
 int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double *S, double VN,
                       const zq_scf_options *opt, double *C, double *e, double *P, double *F, zq_scf_result *res);
 
-/* SP2 steps per density in zq_scf_run_device's captured iteration (1..100, default 48; steps after a
- * density has settled return at once).  A density that needs more restarts the run on the
- * host-synchronised driver and raises the budget by 16.  The budget is per context and is not lowered
- * automatically: a context serves molecules of different sizes, and every restart costs a whole SCF. */
+/* SP2 steps per density in zq_scf_run_device's device-decided iteration (1..128, default 100).  SP2 is
+ * one launch that stops as soon as the density has settled, so the budget only bounds it.  A density
+ * that needs more restarts the run on the host-synchronised driver and raises the budget by 16. */
 int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
+
+/* Diagnostics, with no counterpart in the reference: which driver finished the last zq_scf_run_device
+ * on ctx -- 1 the device-decided iteration, 2 the host-synchronised restart -- or 0 before any run. */
+int zq_scf_last_driver(const zq_ctx *ctx);
 
 /* GPU symmetric eigensolver, eigh of src/linear_algebra.zig:45-71 (LAPACKE_dsyevd, row-major, 'V'):
  * A = U diag(W) U^T for the symmetric n x n A (row-major), W ascending, U (n x n row-major) with the

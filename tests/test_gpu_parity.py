@@ -169,6 +169,7 @@ def test_device_scf_matches_oracle(ctx, kind):
     s, H, S, g, VN = _load(ctx, kind)
     o = s.nel // 2
     r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    assert ctx.last_scf_driver() == "device"  # decided on the device, no host-synchronised restart
     ro = O.scf(H, S, g, VN, o, threshold=1e-10)
     assert r["E"] == pytest.approx(ro["E"], abs=1e-10)
     # geigh contract: F C = S C e, C^T S C = I, e ascending; MP2 from these orbitals agrees
@@ -210,12 +211,13 @@ def test_device_scf_graph_replays_match_eager(ctx):
     call after call, with eager work in between."""
     s, H, S, g, VN = _load(ctx, "w2")
     o = s.nel // 2
-    ctx.set_sp2_budget(48)
     ctx.profile(True)
     eager = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
     ctx.profile(False)
+    assert ctx.last_scf_driver() == "device"
     for _ in range(3):
         r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+        assert ctx.last_scf_driver() == "device"
         assert r["E"] == eager["E"] and r["iterations"] == eager["iterations"]
         np.testing.assert_array_equal(r["P"], eager["P"])
         np.testing.assert_array_equal(r["F"], eager["F"])
@@ -229,13 +231,17 @@ def test_device_scf_sp2_budget_fallback(ctx):
     s, H, S, g, VN = _load(ctx, "ccpvdz")
     o = s.nel // 2
     ref = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    assert ctx.last_scf_driver() == "device"
     ctx.set_sp2_budget(4)
     r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
-    ctx.set_sp2_budget(48)
+    assert ctx.last_scf_driver() == "synced"
+    ctx.set_sp2_budget(100)
     assert r["E"] == pytest.approx(ref["E"], abs=1e-11)
     assert r["iterations"] == ref["iterations"]
     with pytest.raises(hip.InvalidInput):
         ctx.set_sp2_budget(0)
+    with pytest.raises(hip.InvalidInput):
+        ctx.set_sp2_budget(129)
 
 
 @pytest.mark.parametrize("kind", ["ccpvdz", "w2"])
@@ -247,6 +253,7 @@ def test_mixed_path_within_bar(ctx, kind):
     ro = O.scf(H, S, g, VN, o, threshold=1e-10)
     e2o = O.mp2_closed(O.ao2mo_oovv(g, ro["C"], o), ro["e"])
     r = ctx.scf(H, S, VN, o, threshold=1e-6, device=True, fock_precision="fp32")
+    assert ctx.last_scf_driver() == "device"  # the FP32 incremental builds ran (no FP64 restart)
     e2 = ctx.mp2(r["C"], r["e"], o, "fp32c")
     print(f"{kind} mixed: dE_rhf={r['E'] - ro['E']:+.2e} dE_corr={e2 - e2o:+.2e} it={r['iterations']}")
     assert abs(r["E"] - ro["E"]) < 1e-7

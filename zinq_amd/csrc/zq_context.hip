@@ -111,6 +111,8 @@ static void prof_collect(zq_ctx *ctx) {
       auto &p = ctx->prof[ev.name];
       p.first += ms;
       p.second += 1;
+    } else {
+      (void)hipGetLastError();  // an unrecorded pair must not surface as the next launch's error
     }
     (void)hipEventDestroy(ev.a);
     (void)hipEventDestroy(ev.b);

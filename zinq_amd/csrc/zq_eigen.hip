@@ -292,14 +292,10 @@ int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *in
   ZQ_LAUNCH_CHECK(ctx, "k_tridiag");
   // T = Z diag(w) Z^T: column k of the column-major Z (= row k of Z as stored) is eigenvector k
   const rocblas_status st = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, offd, Z, n, info);
-  if (st != rocblas_status_success) {
-    zq_prof_end(ctx);
-    return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dstedc");
-  }
+  zq_prof_end(ctx);  // before the product, which times itself (profile regions do not nest)
+  if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dstedc");
   // row k of A = (Q z_k)^T: A[k][p] = sum_q Z[k][q] Q[p][q]
-  const int rc = zq_gemm_nt(ctx, Z, n, Qm, n, A, n, n, n, n, 1.0, "scf_eig");
-  zq_prof_end(ctx);
-  return rc;
+  return zq_gemm_nt(ctx, Z, n, Qm, n, A, n, n, n, n, 1.0, "scf_eig");
 }
 
 extern "C" int zq_eigh(zq_ctx *ctx, size_t n, const double *A, double *W, double *U) {

@@ -100,7 +100,8 @@ struct zq_ctx {
   void *scf_graph = nullptr;
   std::string scf_graph_key;
   unsigned scratch_gen = 0, eri_gen = 0;
-  int sp2_budget = 48;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
+  int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
+  int scf_driver = 0;    // driver that finished the last device SCF: 1 device-decided, 2 host-synchronised
   hipEvent_t scf_events[2] = {nullptr, nullptr};  // the asynchronous SCF's lagged state reads
 
   // named scratch buffers (grow-only)

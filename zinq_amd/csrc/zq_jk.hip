@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
   // this wave's two rows of the item's first quartet: element (r*8+s)*64 + lane, r = 2w + rr
   const E *src = eri + (size_t(Ib) * (Ib + 1) / 2 + size_t(k0)) * ZQ_QUARTET + (2 * wave) * 512 + lane;
   double Jbra = 0.0;
-  // quartets in flight ahead of the contraction, kept in their stored type (4-byte ones leave room for 3)
-  constexpr int D = sizeof(E) == 4 ? 3 : 2;
+  // quartets in flight ahead of the contraction, kept in their stored type (4-byte ones leave room for 4)
+  constexpr int D = sizeof(E) == 4 ? 4 : 2;
   double Kps[8], Kqs[8], Pp_s[8], Pq_s[8], cur[16];
   E pf[D][16];
   double Pr_cur[4], Pr_nxt[4];  // Prow[br*8+r], Qrow[br*8+r] for the wave's two rows

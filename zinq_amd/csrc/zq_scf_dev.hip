@@ -586,7 +586,8 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, ctx->stream, st, r.VN);
   // core-Hamiltonian guess, src/hartree_fock.zig:425-431; W = H until the first iteration replaces it
   ZQ_TRY(ctx, hipMemcpyAsync(d.W, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, 64, st, part_rms, r.mixed ? d.dP : nullptr)) return rc;
+  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, kSp2MaxSteps, st, part_rms, r.mixed ? d.dP : nullptr))
+    return rc;
   // the incremental builds start from F = H (dP = P_0 after the guess)
   if (r.mixed) ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_F, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
   const int budget = ctx->sp2_budget;
@@ -779,7 +780,8 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
     gate.reset(new ZqGate);
     if (rc) return rc;
   }
-  if (status == 3) ctx->sp2_budget = std::min(100, ctx->sp2_budget + 16);
+  if (status == 3) ctx->sp2_budget = std::min(kSp2MaxSteps, ctx->sp2_budget + 16);
+  ctx->scf_driver = (status == 0 || status == 3) ? 2 : 1;
   if (status == 0 || status == 3) {
     // DIIS history beyond the device bookkeeping, or an SP2 budget too small: synchronised driver
     // (FP64 Fock builds throughout)
@@ -816,7 +818,10 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
 }
 
 extern "C" int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps) {
-  if (!ctx || steps < 1 || steps > 100) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_set_sp2_budget: steps outside 1..100");
+  if (!ctx || steps < 1 || steps > kSp2MaxSteps)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_set_sp2_budget: steps outside 1..128");
   ctx->sp2_budget = steps;
   return ZQ_OK;
 }
+
+extern "C" int zq_scf_last_driver(const zq_ctx *ctx) { return ctx ? ctx->scf_driver : 0; }

@@ -17,7 +17,8 @@ PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32c": 3}
 EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_load_packed", "zq_fock_jk", "zq_ao2mo",
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
-            "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh"]
+            "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
+            "zq_scf_last_driver"]
 
 
 class GpuError(RuntimeError):
@@ -85,6 +86,7 @@ def lib():
         L.zq_scf_run_device.argtypes = L.zq_scf_run.argtypes
         L.zq_scf_set_sp2_budget.argtypes = [vp, ci]
         L.zq_eigh.argtypes = [vp, sz, vp, vp, vp]
+        L.zq_scf_last_driver.argtypes = [vp]
         L.zq_eri_set_create.restype = vp
         L.zq_eri_set_create.argtypes = [ci, vp, sz, ci]
         L.zq_eri_set_destroy.argtypes = [vp]
@@ -243,6 +245,10 @@ class Context:
         W, U = np.zeros(n), np.zeros((n, n))
         self._check(self._L.zq_eigh(self._h, n, ptr(A), ptr(W), ptr(U)))
         return W, U
+
+    def last_scf_driver(self):
+        """'device' or 'synced': the driver that finished the last device SCF (zq_scf_last_driver)."""
+        return {0: None, 1: "device", 2: "synced"}[self._L.zq_scf_last_driver(self._h)]
 
     def set_sp2_budget(self, steps):
         """SP2 steps per density in the captured device-SCF iteration (zq_scf_set_sp2_budget)."""

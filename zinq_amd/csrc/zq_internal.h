@@ -157,7 +157,7 @@ void zq_prof_begin(zq_ctx *ctx, const char *name);
 void zq_prof_end(zq_ctx *ctx);
 void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip, also drops the captured SCF graph
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF,
-                      const int *skip = nullptr, bool f32 = false);  // zq_jk.hip
+                      const int *skip = nullptr, bool f32 = false, double *epart = nullptr);  // zq_jk.hip
 int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, made once per load
 void zq_narrow_eri(hipStream_t stream, const double *eri, size_t cnt, float *out);  // zq_jk.hip
 // MP2 correlation energy of the resident ERIs (zq_transform.hip); C and e are host arrays.

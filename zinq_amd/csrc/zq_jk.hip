@@ -11,8 +11,8 @@
 //
 // Work decomposition: one workgroup per (bra block pair, run of <= CH ket blocks); lane = bra pair
 // (p,q) of the 8x8 bra block; the quartets stream through registers (coalesced 512-byte rows) and the six
-// updates accumulate there, reduced across lanes with butterfly shuffles.  Partial results are written as
-// 8x8 "records" with a fixed target block.  A second kernel sums, for each
+// updates accumulate there, reduced across lanes by DPP moves and permlane swaps (no LDS).  Partial
+// results are written as 8x8 "records" with a fixed target block.  A second kernel sums, for each
 // output block, its records in a fixed order (a host-built CSR list): no atomics, bitwise reproducible.
 
 #include <algorithm>
@@ -39,6 +39,70 @@ __device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
   }
   const double send = hc ? b[0] : b[1], keep = hc ? b[1] : b[0];
   return keep + __shfl_xor(send, MC);
+}
+
+// Lane exchanges without LDS: DPP moves inside a 16-lane row and the gfx950 permlane swaps across rows.
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double dpp_xor1(double v) { return dpp_move<0xB1>(v); }                 // quad_perm 1 0 3 2
+__device__ inline double dpp_xor2(double v) { return dpp_move<0x4E>(v); }                 // quad_perm 2 3 0 1
+__device__ inline double dpp_xor4(double v) { return dpp_move<0x1B>(dpp_move<0x141>(v)); }  // half-row mirror, then ^3
+__device__ inline double dpp_xor8(double v) { return dpp_move<0x128>(v); }                // row rotate by 8
+
+// Lanes < 32 get x summed with lane ^ 32, lanes >= 32 get y summed likewise (v_permlane32_swap).
+__device__ inline double perm32_sum(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(unsigned(__double2loint(x)), unsigned(__double2loint(y)), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(unsigned(__double2hiint(x)), unsigned(__double2hiint(y)), false, false);
+  return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+}
+// Lanes with bit 4 clear get x summed with lane ^ 16, the others y (v_permlane16_swap).
+__device__ inline double perm16_sum(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(unsigned(__double2loint(x)), unsigned(__double2loint(y)), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(unsigned(__double2hiint(x)), unsigned(__double2hiint(y)), false, false);
+  return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+}
+
+// Sum of each of 16 values over the 64 lanes; lane l ends with the sum of value l >> 2 (reduce-scatter
+// over lane bits 5, 4, 3, 2, then the two low bits).
+__device__ inline double sum16_scatter(const double (&v)[16], int lane) {
+  double u[8], w[4], z[2];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) u[i] = perm32_sum(v[i], v[i + 8]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) w[i] = perm16_sum(u[i], u[i + 4]);
+  const bool b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = b3 ? w[i] : w[i + 2], keep = b3 ? w[i + 2] : w[i];
+    z[i] = keep + dpp_xor8(send);
+  }
+  const double send = b2 ? z[0] : z[1], keep = b2 ? z[1] : z[0];
+  double t = keep + dpp_xor4(send);
+  t += dpp_xor2(t);
+  t += dpp_xor1(t);
+  return t;
+}
+
+// Sums of a[0], a[1] over lane bits 0-2 (q); lane l ends with the sum of a[l & 1].
+__device__ inline double sum2_over_q(const double (&a)[2], int lane) {
+  const bool b0 = lane & 1;
+  const double send = b0 ? a[0] : a[1], keep = b0 ? a[1] : a[0];
+  double t = keep + dpp_xor1(send);
+  t += dpp_xor2(t);
+  t += dpp_xor4(t);
+  return t;
+}
+
+// Sums of a[0], a[1] over lane bits 3-5 (p); lane l ends with the sum of a[l >> 5].
+__device__ inline double sum2_over_p(const double (&a)[2]) {
+  double t = perm32_sum(a[0], a[1]);
+  t = perm16_sum(t, t);
+  t += dpp_xor8(t);
+  return t;
 }
 
 // One workgroup (4 waves) per work item; wave w handles ket rows r = 2w, 2w+1 of every quartet for all
@@ -144,12 +208,13 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
     }
     const int Jb = zq_bpair(br, bs);
     const bool diagJ = (br == bs), diagQ = (Jb == Ib);
+    double jt[16], accpr[2], accqr[2];
 #pragma unroll
     for (int rr = 0; rr < 2; ++rr) {
       const int r = 2 * wave + rr;
       const double Pp_r = Pr_cur[2 * rr], Pq_r = Pr_cur[2 * rr + 1];
       const double *Prs = P + size_t(br * 8 + r) * npad + bs * 8;  // wave-uniform
-      double jt[8], accpr = 0.0, accqr = 0.0;
+      double apr = 0.0, aqr = 0.0;
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
         const double v = cur[rr * 8 + s];
@@ -158,26 +223,22 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
         if (diagQ && lane == r * 8 + s) w *= 0.5;
         const double a = v * w;
         Jbra += Prs[s] * a;
-        jt[s] = Ppq * a;
+        jt[rr * 8 + s] = Ppq * a;
         Kqs[s] += Pp_r * a;
         Kps[s] += Pq_r * a;
-        accqr += Pp_s[s] * a;
-        accpr += Pq_s[s] * a;
+        aqr += Pp_s[s] * a;
+        apr += Pq_s[s] * a;
       }
-      double t = reduce_scatter8<4, 2, 1>(jt, lane);
-      t += __shfl_xor(t, 8);
-      t += __shfl_xor(t, 16);
-      t += __shfl_xor(t, 32);
-      if (p == 0) out[r * 8 + q] = 2.0 * t;  // J of ket pair (r, s = q)
-      accpr += __shfl_xor(accpr, 1);
-      accpr += __shfl_xor(accpr, 2);
-      accpr += __shfl_xor(accpr, 4);
-      if (q == 0) out[64 + p * 8 + r] = -xexch * accpr;  // K[p, br*8+r]
-      accqr += __shfl_xor(accqr, 8);
-      accqr += __shfl_xor(accqr, 16);
-      accqr += __shfl_xor(accqr, 32);
-      if (p == 0) out[128 + q * 8 + r] = -xexch * accqr;  // K[q, br*8+r]
+      accpr[rr] = apr;
+      accqr[rr] = aqr;
     }
+    // J of the wave's 16 ket pairs (r, s), over the 64 bra pairs: lane l holds pair l >> 2
+    const double tj = sum16_scatter(jt, lane);
+    if ((lane & 3) == 0) out[(2 * wave + (lane >> 5)) * 8 + ((lane >> 2) & 7)] = 2.0 * tj;
+    // K[p, br*8+r] over q (lanes with q = rr write row rr) and K[q, br*8+r] over p (p = 4 rr writes row rr)
+    const double kp = sum2_over_q(accpr, lane), kq = sum2_over_p(accqr);
+    if (q < 2) out[64 + p * 8 + 2 * wave + q] = -xexch * kp;
+    if ((p & 3) == 0) out[128 + q * 8 + 2 * wave + (p >> 2)] = -xexch * kq;
     out += 192;
 
     if (run_end || k == k1 - 1) {
@@ -223,12 +284,17 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
 // GROUP: blk[blockIdx.x] = (molecule, its output block); H, F, n and the leading dimension come from
 // mols[].  Otherwise H and F are n x n with leading dimension ldf and blockIdx.x is the output block.
 // MULTI: blockIdx.y is the density (records + d * rstride, F + d * fstride); H may be null (base 0).
+// epart (optional, plain form): epart[block] = this block's share of 1/2 sum P (Hc + F), the electronic
+// energy of getEnergy (src/hartree_fock.zig:643-655), from the padded density Pe (ld pld) and the core
+// Hamiltonian Hc (ld ldf); blocks off the diagonal count twice, as their mirrors are not visited.
 template <bool GROUP, bool MULTI = false>
 __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
                                                     const int *__restrict__ csr_idx, const double *H, int n, double *F,
                                                     const int *__restrict__ skip, int ldf, const JKMol *__restrict__ mols,
                                                     const int2 *__restrict__ blk, const int *__restrict__ done,
-                                                    long long rstride = 0, long long fstride = 0) {
+                                                    long long rstride = 0, long long fstride = 0,
+                                                    const double *__restrict__ Pe = nullptr, int pld = 0,
+                                                    const double *__restrict__ Hc = nullptr, double *__restrict__ epart = nullptr) {
   __shared__ double part[4][64];
   if (skip && *skip) return;
   if (MULTI) {
@@ -280,6 +346,13 @@ __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ r
   if (gi < n && gj < n && (x != y || i >= j)) {
     F[size_t(gi) * ldf + gj] = f;
     F[size_t(gj) * ldf + gi] = f;
+  }
+  if (!GROUP && !MULTI && epart) {
+    const double w = x != y ? 1.0 : (i > j ? 1.0 : (i == j ? 0.5 : 0.0));
+    double c = (gi < n && gj < n) ? w * Pe[size_t(gi) * pld + gj] * (Hc[size_t(gi) * ldf + gj] + f) : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) epart[blockIdx.x] = c;
   }
 }
 
@@ -371,9 +444,10 @@ int zq_build_jk_plan(zq_ctx *ctx) {
 // With skip != nullptr both kernels return at once while *skip != 0 (device-side early exit).  With
 // f32 the records kernel streams the FP32 copy of the quartets (zq_eri32); products and sums stay FP64.
 // dH may equal dF: every output block reads its base before its own wave writes it, and the blocks
-// above the diagonal are only written (mirror), never read.
+// above the diagonal are only written (mirror), never read.  epart (Nb doubles, optional): the blocks'
+// shares of the electronic energy 1/2 sum P (H + F) with the context's P and core H.
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF, const int *skip,
-                      bool f32) {
+                      bool f32, double *epart) {
   if (f32 && !ctx->eri32) return zq_fail(ctx, ZQ_ERR_STATE, "zq_fock_jk_device: no FP32 ERI copy");
   zq_prof_begin(ctx, f32 ? "jk_records32" : "jk_records");
   if (f32)
@@ -388,7 +462,8 @@ int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_records");
   zq_prof_begin(ctx, "jk_reduce");
   hipLaunchKernelGGL(zq_jk_reduce<false>, dim3(unsigned(ctx->Nb)), dim3(256), 0, ctx->stream, ctx->d_records,
-                     ctx->d_csr_ptr, ctx->d_csr_idx, dH, ctx->n, dF, skip, ctx->n, nullptr, nullptr, nullptr);
+                     ctx->d_csr_ptr, ctx->d_csr_idx, dH, ctx->n, dF, skip, ctx->n, nullptr, nullptr, nullptr, 0LL, 0LL,
+                     ctx->d_P, ctx->npad, ctx->d_H, epart);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_reduce");
   return ZQ_OK;

@@ -209,27 +209,35 @@ int sygv(int n, double *A, double *B, double *w) {
 }
 
 int chol_inv_t(int n, const double *S, double *X) {
-  std::vector<double> L(size_t(n) * n, 0.0), Li(size_t(n) * n, 0.0);
+  // S = U^T U with U = L^T upper (row-major), right-looking: row j of U, then the trailing update
+  // A[i][i..] -= U[j][i] U[j][i..], all on contiguous row segments (vectorised axpys)
+  std::vector<double> U(S, S + size_t(n) * n);
   for (int j = 0; j < n; ++j) {
-    double s = S[size_t(j) * n + j];
-    for (int k = 0; k < j; ++k) s -= L[size_t(j) * n + k] * L[size_t(j) * n + k];
-    if (s <= 0.0) return 1;
-    L[size_t(j) * n + j] = std::sqrt(s);
+    double *Uj = U.data() + size_t(j) * n;
+    if (Uj[j] <= 0.0) return 1;
+    const double d = std::sqrt(Uj[j]), inv = 1.0 / d;
+    Uj[j] = d;
+    for (int k = j + 1; k < n; ++k) Uj[k] *= inv;
     for (int i = j + 1; i < n; ++i) {
-      double t = S[size_t(i) * n + j];
-      for (int k = 0; k < j; ++k) t -= L[size_t(i) * n + k] * L[size_t(j) * n + k];
-      L[size_t(i) * n + j] = t / L[size_t(j) * n + j];
+      double *__restrict__ Ai = U.data() + size_t(i) * n;
+      const double *__restrict__ uj = Uj;
+      const double f = uj[i];
+      for (int k = i; k < n; ++k) Ai[k] -= f * uj[k];
     }
   }
-  // Li = L^-1 (lower), column by column forward substitution
-  for (int c = 0; c < n; ++c)
-    for (int i = c; i < n; ++i) {
-      double t = (i == c) ? 1.0 : 0.0;
-      for (int k = c; k < i; ++k) t -= L[size_t(i) * n + k] * Li[size_t(k) * n + c];
-      Li[size_t(i) * n + c] = t / L[size_t(i) * n + i];
+  // X = L^-T = U^-1 (upper), by rows from the bottom: X[i][i..] = (e_i - sum_{k>i} U[i][k] X[k][i..]) / U[i][i]
+  for (int i = n - 1; i >= 0; --i) {
+    double *__restrict__ Xi = X + size_t(i) * n;
+    const double *Ui = U.data() + size_t(i) * n;
+    for (int c = 0; c < n; ++c) Xi[c] = c == i ? 1.0 : 0.0;
+    for (int k = i + 1; k < n; ++k) {
+      const double f = Ui[k];
+      const double *__restrict__ Xk = X + size_t(k) * n;
+      for (int c = k; c < n; ++c) Xi[c] -= f * Xk[c];
     }
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) X[size_t(i) * n + j] = Li[size_t(j) * n + i];
+    const double inv = 1.0 / Ui[i];
+    for (int c = i; c < n; ++c) Xi[c] *= inv;
+  }
   return 0;
 }
 

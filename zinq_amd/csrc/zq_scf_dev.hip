@@ -2,10 +2,11 @@
 //
 //   per iteration (all on the context's stream, FP64):
 //     F      = H + G(P)                           J/K kernels (getFock, :686-735)
-//     E      = 1/2 sum P (H + F) + V_NN           deterministic block reduction (getEnergy, :643-655)
+//     E      = 1/2 sum P (H + F) + V_NN           per-block partials from the J/K reduce, summed in a
+//                                                 fixed order (getEnergy, :643-655)
 //     err    = F P S - (F P S)^T                   MFMA GEMMs (getError, :658-683)
 //     B row  = <err_new, err_i>                   deterministic reductions; the <= 9 x 9 DIIS system
-//                                                 is solved on the host (diis, :176-227) and the
+//                                                 is solved by one wave (diis, :176-227) and the
 //     F_ext  = sum c_i F_i, symmetrised           combination runs on the GPU
 //     P      = f X D X^T, D = theta(mu - F')      F' = X^T F_ext X in the orthonormal basis X = L^-T
 //                                                 (S = L L^T); D, the projector onto the nocc lowest
@@ -164,7 +165,8 @@ struct Dev {
   int n;
   size_t nn;
   double *S, *X, *XT, *C, *T, *Fp, *W, *FP, *FPS, *Pn, *Fh, *Eh, *part, *scal, *coef, *eig, *offd;
-  double *dP = nullptr;  // P_k - P_(k-1), npad x npad zero padded (mixed Fock path only)
+  double *dP = nullptr;     // P_k - P_(k-1), npad x npad zero padded (mixed Fock path only)
+  double *epart = nullptr;  // energy partials of the J/K reduce, one per output block
   int *slots, *info;
   int sp2_hint = 24;  // SP2 steps of the previous density, the first convergence check comes after it
   int sp2_steps = 0, sp2_calls = 0, sp2_fallbacks = 0;
@@ -261,16 +263,9 @@ struct ScfState {
   double B[kMaxHist * kMaxHist];  // B[slot][slot] = <err_slot, err_slot'>
 };
 
-__global__ void k_state_init(ScfState *st, double VN) {
-  if (threadIdx.x != 0) return;
-  st->done = 0, st->it = 0, st->sp2_steps = 0, st->sp2_max = 0;
-  st->hist_n = 0, st->cur = 0, st->combine = 0;
-  st->e_old = VN, st->e_new = VN, st->rms = 0.0, st->dE = 0.0;
-}
-
-// Slot of this iteration's error/Fock pair: the oldest when the history is full, else the lowest free.
-__global__ void k_diis_slot(ScfState *st, int diis) {
-  if (threadIdx.x != 0 || st->done) return;
+// Slot of the next iteration's error/Fock pair: the oldest when the history is full, else the lowest
+// free.  Taken at the end of the previous iteration (k_converge) or by k_state_init for the first.
+__device__ void diis_next_slot(ScfState *st, int diis) {
   int slot;
   if (st->hist_n >= diis) {
     slot = st->hist[0];
@@ -286,6 +281,14 @@ __global__ void k_diis_slot(ScfState *st, int diis) {
   }
   st->hist[st->hist_n++] = slot;
   st->cur = slot;
+}
+
+__global__ void k_state_init(ScfState *st, double VN, int diis) {
+  if (threadIdx.x != 0) return;
+  st->done = 0, st->it = 0, st->sp2_steps = 0, st->sp2_max = 0;
+  st->hist_n = 0, st->cur = 0, st->combine = 0;
+  st->e_old = VN, st->e_new = VN, st->rms = 0.0, st->dE = 0.0;
+  if (diis > 0) diis_next_slot(st, diis);
 }
 
 // err = FPS - (FPS)^T into Eh[cur], F into Fh[cur].
@@ -361,8 +364,9 @@ __device__ bool lu_solve(int n, double *A, double *b) {
 // (src/hartree_fock.zig:176-227), or a reset to the current entry when it is singular (:821-845).
 // One wave: lane b holds partial b of every sum (kRedBlocks == 64), each sum a fixed-order butterfly;
 // lane 0 solves the <= 17 x 17 system.
-__global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ part, int diis, double VN,
-                                                   ScfState *__restrict__ st) {
+// epart: the nep energy partials the J/K reduce wrote (one per output block).
+__global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ part, const double *__restrict__ epart,
+                                                   int nep, int diis, double VN, ScfState *__restrict__ st) {
   static_assert(kRedBlocks == 64, "one partial per lane");
   __shared__ double Bs[kMaxHist * kMaxHist];
   if (st->done) return;
@@ -371,7 +375,9 @@ __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ pa
   int hist[kMaxHist];  // the history in registers: every lane needs it, and st-> reads serialise
 #pragma unroll
   for (int h = 0; h < kMaxHist; ++h) hist[h] = h < m ? st->hist[h] : -1;
-  const double e = zqt::wave_sum(part[lane]);
+  double e = 0.0;
+  for (int b = lane; b < nep; b += 64) e += epart[b];
+  e = zqt::wave_sum(e);
   double dots[kMaxHist];
 #pragma unroll
   for (int k = 0; k < kMaxHist; ++k) {
@@ -466,12 +472,23 @@ __global__ void k_sp2_check(int parity, ScfState *__restrict__ st) {
   if (fin[0] < 2 || st->sp2[5] != 0) st->done = 3;
 }
 
-// rms(dP) and the convergence test of src/hartree_fock.zig:853-870 (one wave, butterfly sum).
+// The end of an iteration: the SP2 check of k_sp2_check (parity of its step budget), then rms(dP) and
+// the convergence test of src/hartree_fock.zig:853-870 (one wave, butterfly sum).  The state's done
+// word also goes to *host_done (host memory the device writes), which the host reads one iteration late.
 __global__ __launch_bounds__(64) void k_converge(const double *__restrict__ part, int n, double thr, int maxit,
-                                                 ScfState *__restrict__ st) {
+                                                 ScfState *__restrict__ st, int parity, int *__restrict__ host_done,
+                                                 int diis) {
   if (st->done) return;
   const double s = zqt::wave_sum(part[threadIdx.x]);
   if (threadIdx.x != 0) return;
+  const int *fin = st->sp2 + 2 * parity;
+  st->sp2_steps = fin[1];
+  st->sp2_max = max(st->sp2_max, fin[1]);
+  if (fin[0] < 2 || st->sp2[5] != 0) {  // SP2 did not settle: the P of this iteration is not used
+    st->done = 3;
+    *host_done = 3;
+    return;
+  }
   const double rms = sqrt(s / (double(n) * n));
   const double dE = fabs(st->e_new - st->e_old);
   st->e_old = st->e_new;
@@ -482,11 +499,15 @@ __global__ __launch_bounds__(64) void k_converge(const double *__restrict__ part
     st->done = 1;
   else if (st->it >= maxit)
     st->done = 2;
+  else if (diis > 0)
+    diis_next_slot(st, diis);
+  *host_done = st->done;
 }
 
 // P from the symmetric Fock matrix A by SP2 with a fixed step budget, without host synchronisation.
+// check: k_sp2_check follows SP2 (else the iteration's k_converge makes the check).
 int density_async(Dev &d, const double *A, int nocc, double factor, int nsteps, ScfState *st, double *part_rms,
-                  double *dP) {
+                  double *dP, bool check) {
   zq_ctx *ctx = d.ctx;
   const int n = d.n, np = ctx->npad;
   const int *skip = &st->done;
@@ -496,7 +517,7 @@ int density_async(Dev &d, const double *A, int nocc, double factor, int nsteps, 
   long long stride = 0;
   int ld = 0;
   if (int rc = zq_sp2_enqueue(ctx, d.Fp, n, nocc, 1e-9, nsteps, skip, st->sp2, &D, &stride, &ld)) return rc;
-  hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, ctx->stream, nsteps & 1, st);
+  if (check) hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, ctx->stream, nsteps & 1, st);
   if (int rc = zq_gemm_nt(ctx, d.X, n, D, ld, d.T, n, n, n, n, 1.0, "scf_gemm", skip, st->sp2 + 4, stride)) return rc;
   if (int rc = zq_gemm_nt(ctx, d.T, n, d.X, n, d.Pn, n, n, n, n, factor, "scf_gemm", skip)) return rc;
   hipLaunchKernelGGL(k_store_density, dim3(kRedBlocks), dim3(256), 0, ctx->stream, d.Pn, n, ctx->d_P, np, part_rms, skip,
@@ -531,32 +552,32 @@ int setup(Dev &d, const double *H, const double *S) {
 }
 
 // The kernels of one SCF iteration; every one of them returns at once while state.done != 0.
-int enqueue_iteration(Dev &d, const Run &r, int budget, ScfState *st, double *part_rms) {
+int enqueue_iteration(Dev &d, const Run &r, int budget, ScfState *st, double *part_rms, int *host_done) {
   zq_ctx *ctx = d.ctx;
   const int n = d.n, maxh = r.opt->diis > 0 ? r.opt->diis : 1;
   const size_t nn = d.nn;
   const int gblocks = int(std::min<size_t>(1024, (nn + 255) / 256));
   const int *skip = &st->done;
+  // the reduce also leaves the energy partials (getEnergy) in d.epart; the DIIS slot of this iteration
+  // was taken by the previous one's k_converge (or k_state_init)
   if (r.mixed) {
     // F_k = F_(k-1) + G(P_k - P_(k-1)), in place on d_F, from the FP32 ERI copy
-    if (int rc = zq_fock_jk_device(ctx, d.dP, ctx->d_F, r.x_exch, ctx->d_F, skip, true)) return rc;
-  } else if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, r.x_exch, ctx->d_F, skip)) {
+    if (int rc = zq_fock_jk_device(ctx, d.dP, ctx->d_F, r.x_exch, ctx->d_F, skip, true, d.epart)) return rc;
+  } else if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, r.x_exch, ctx->d_F, skip, false, d.epart)) {
     return rc;
   }
-  hipLaunchKernelGGL(k_energy_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, ctx->d_P, r.np, ctx->d_H, ctx->d_F, n,
-                     d.part, skip);
   if (r.opt->diis > 0) {
-    hipLaunchKernelGGL(k_diis_slot, dim3(1), dim3(64), 0, ctx->stream, st, r.opt->diis);
     // F P (P symmetric) and (F P) S (S symmetric) as A B^T products
     if (int rc = zq_gemm_nt(ctx, ctx->d_F, n, ctx->d_P, r.np, d.FP, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
     if (int rc = zq_gemm_nt(ctx, d.FP, n, d.S, n, d.FPS, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
     hipLaunchKernelGGL(k_antisym_store, dim3(gblocks), dim3(256), 0, ctx->stream, d.FPS, ctx->d_F, n, d.Eh, d.Fh, st);
     hipLaunchKernelGGL(k_dots_state, dim3(kRedBlocks, maxh), dim3(256), 0, ctx->stream, d.Eh, nn, d.part + kRedBlocks, st);
   }
-  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, ctx->stream, d.part, r.opt->diis, r.VN, st);
+  hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, ctx->stream, d.part, d.epart, ctx->Nb, r.opt->diis, r.VN, st);
   hipLaunchKernelGGL(k_combine_state, dim3(gblocks), dim3(256), 0, ctx->stream, d.Fh, ctx->d_F, n, d.W, st);
-  if (int rc = density_async(d, d.W, r.nocc, r.factor, budget, st, part_rms, r.mixed ? d.dP : nullptr)) return rc;
-  hipLaunchKernelGGL(k_converge, dim3(1), dim3(64), 0, ctx->stream, part_rms, n, r.opt->threshold, r.opt->iterations, st);
+  if (int rc = density_async(d, d.W, r.nocc, r.factor, budget, st, part_rms, r.mixed ? d.dP : nullptr, false)) return rc;
+  hipLaunchKernelGGL(k_converge, dim3(1), dim3(64), 0, ctx->stream, part_rms, n, r.opt->threshold, r.opt->iterations, st,
+                     budget & 1, host_done, r.opt->diis);
   ZQ_LAUNCH_CHECK(ctx, "scf iteration");
   return ZQ_OK;
 }
@@ -570,9 +591,13 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   const size_t nn = d.nn;
   std::unique_ptr<ZqGate> gate(new ZqGate);  // set-up allocates; released before the replay loop
   auto *st = (ScfState *)zq_scratch(ctx, "scf_state", sizeof(ScfState));
-  auto *hst = (int *)zq_pinned(ctx, "scf_state_host", 2 * 4 * sizeof(int));
-  if (!st || !hst || !zq_pinned(ctx, "scf_state_full", sizeof(ScfState))) return zq_fail(ctx, ZQ_ERR_OOM, "SCF state");
+  auto *hdone = (int *)zq_pinned(ctx, "scf_done", sizeof(int));
+  if (!st || !hdone || !zq_pinned(ctx, "scf_state_full", sizeof(ScfState))) return zq_fail(ctx, ZQ_ERR_OOM, "SCF state");
+  int *ddone = nullptr;  // the same host word as the kernels address it
+  ZQ_TRY(ctx, hipHostGetDevicePointer((void **)&ddone, hdone, 0));
   double *part_rms = d.part + (1 + kMaxHist) * kRedBlocks;
+  d.epart = (double *)zq_scratch(ctx, "scf_epart", size_t(ctx->Nb) * sizeof(double));
+  if (!d.epart) return zq_fail(ctx, ZQ_ERR_OOM, "SCF energy partials");
   hipEvent_t *ev = ctx->scf_events;
   if (!ev[0]) ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   if (!ev[1]) ZQ_TRY(ctx, hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -583,10 +608,10 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
     if (!d.dP) return zq_fail(ctx, ZQ_ERR_OOM, "SCF dP");
     ZQ_TRY(ctx, hipMemsetAsync(d.dP, 0, size_t(r.np) * r.np * sizeof(double), ctx->stream));
   }
-  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, ctx->stream, st, r.VN);
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, ctx->stream, st, r.VN, r.opt->diis);
   // core-Hamiltonian guess, src/hartree_fock.zig:425-431; W = H until the first iteration replaces it
   ZQ_TRY(ctx, hipMemcpyAsync(d.W, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, kSp2MaxSteps, st, part_rms, r.mixed ? d.dP : nullptr))
+  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, kSp2MaxSteps, st, part_rms, r.mixed ? d.dP : nullptr, true))
     return rc;
   // the incremental builds start from F = H (dP = P_0 after the guess)
   if (r.mixed) ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_F, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
@@ -596,14 +621,15 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   // one iteration as a hipGraph, captured once per context and key, replayed with one launch
   const bool use_graph = !ctx->profile;
   char key[256];
-  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%u/%u/%p/%d", n, nocc, budget, r.opt->diis, r.opt->iterations,
-           r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->scratch_gen, ctx->eri_gen, (void *)st, int(r.mixed));
+  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%u/%u/%p/%p/%d", n, nocc, budget, r.opt->diis,
+           r.opt->iterations, r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->scratch_gen, ctx->eri_gen, (void *)st,
+           (void *)ddone, int(r.mixed));
   if (use_graph && (!ctx->scf_graph || ctx->scf_graph_key != key)) {
     if (ctx->scf_graph) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
     ctx->scf_graph = nullptr;
     ZqCaptureWindow window;  // no allocation or synchronous call in any thread while capturing
     ZQ_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_iteration(d, r, budget, st, part_rms);
+    const int rc = enqueue_iteration(d, r, budget, st, part_rms, ddone);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (rc) {
@@ -618,19 +644,21 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
     ctx->scf_graph = exec;
     ctx->scf_graph_key = key;
   }
+  // the stream is idle here (the previous run ended with a synchronisation), so no kernel writes it now
+  *(volatile int *)hdone = 0;
   int k = 0;
   for (; k < r.opt->iterations; ++k) {
     if (use_graph) {
       ZQ_TRY(ctx, hipGraphLaunch((hipGraphExec_t)ctx->scf_graph, ctx->stream));
-    } else if (int rc = enqueue_iteration(d, r, budget, st, part_rms)) {
+    } else if (int rc = enqueue_iteration(d, r, budget, st, part_rms, ddone)) {
       return rc;
     }
-    ZQ_TRY(ctx, hipMemcpyAsync(hst + 4 * (k & 1), st, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ZQ_TRY(ctx, hipEventRecord(ev[k & 1], ctx->stream));
     if (k >= 1) {
-      // state after iteration k-1; iteration k is already queued behind it
+      // done after iteration k-1 (or a later one: done never goes back to 0); iteration k is already
+      // queued behind it
       ZQ_TRY(ctx, hipEventSynchronize(ev[(k - 1) & 1]));
-      if (hst[4 * ((k - 1) & 1)] != 0) break;
+      if (*(volatile int *)hdone != 0) break;
     }
   }
   auto *hs_p = (ScfState *)zq_pinned(ctx, "scf_state_full", sizeof(ScfState));

@@ -21,26 +21,6 @@
 
 namespace {
 
-// Reduce 8 values over the 8 lanes that differ in the bits MA|MB|MC; afterwards the lane holds the
-// total of value index (lane&MA ? 4 : 0) + (lane&MB ? 2 : 0) + (lane&MC ? 1 : 0).
-template <int MA, int MB, int MC>
-__device__ inline double reduce_scatter8(const double (&v)[8], int lane) {
-  const bool ha = lane & MA, hb = lane & MB, hc = lane & MC;
-  double a[4], b[2];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double send = ha ? v[i] : v[i + 4], keep = ha ? v[i + 4] : v[i];
-    a[i] = keep + __shfl_xor(send, MA);
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const double send = hb ? a[i] : a[i + 2], keep = hb ? a[i + 2] : a[i];
-    b[i] = keep + __shfl_xor(send, MB);
-  }
-  const double send = hc ? b[0] : b[1], keep = hc ? b[1] : b[0];
-  return keep + __shfl_xor(send, MC);
-}
-
 // Lane exchanges without LDS: DPP moves inside a 16-lane row and the gfx950 permlane swaps across rows.
 template <int CTRL>
 __device__ inline double dpp_move(double v) {
@@ -105,6 +85,38 @@ __device__ inline double sum2_over_p(const double (&a)[2]) {
   return t;
 }
 
+// Reduce-scatter of 8 values over lane bits 2, 1, 0: lane l ends with the sum of v[l & 7] over the 8
+// lanes that share l >> 3.
+__device__ inline double sum8_scatter_low(const double (&v)[8], int lane) {
+  const bool b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
+  double a[4], b[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double send = b2 ? v[i] : v[i + 4], keep = b2 ? v[i + 4] : v[i];
+    a[i] = keep + dpp_xor4(send);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = b1 ? a[i] : a[i + 2], keep = b1 ? a[i + 2] : a[i];
+    b[i] = keep + dpp_xor2(send);
+  }
+  const double send = b0 ? b[0] : b[1], keep = b0 ? b[1] : b[0];
+  return keep + dpp_xor1(send);
+}
+
+// Reduce-scatter of 8 values over lane bits 5, 4, 3: lane l ends with the sum of v[l >> 3] over the 8
+// lanes that share l & 7.
+__device__ inline double sum8_scatter_high(const double (&v)[8], int lane) {
+  double a[4], b[2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a[i] = perm32_sum(v[i], v[i + 4]);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) b[i] = perm16_sum(a[i], a[i + 2]);
+  const bool b3 = lane & 8;
+  const double send = b3 ? b[0] : b[1], keep = b3 ? b[1] : b[0];
+  return keep + dpp_xor8(send);
+}
+
 // One workgroup (4 waves) per work item; wave w handles ket rows r = 2w, 2w+1 of every quartet for all
 // 64 bra pairs (lane = bra pair (p,q)), loading its two 512-byte rows straight into registers.  While a
 // quartet is being contracted, the next quartet's rows and the per-lane density values it needs are
@@ -122,7 +134,7 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
                                                      double *__restrict__ rec, const int *__restrict__ skip,
                                                      const JKMol *__restrict__ mols, const int *__restrict__ done,
                                                      long long pstride = 0, long long rstride = 0) {
-  __shared__ double red[4][2][64];
+  __shared__ double red[2][4][2][64];  // two buffers, so each cross-wave sum needs one barrier
   const int item_id = MULTI ? blockIdx.y : blockIdx.x;
   if (MULTI) {
     P += blockIdx.x * pstride;
@@ -143,9 +155,12 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
 
   const int p = lane >> 3, q = lane & 7;
   const int Pg = bp * 8 + p, Qg = bq * 8 + q;
+  // fI halves the bra pairs counted twice (p == q on a diagonal bra block).  It is a power of two and
+  // constant per lane, so it is folded into this lane's density values (exactly) rather than into every
+  // integral; Jbra, which takes no such value, is scaled once at the end.
   const double fI = (bp == bq && p == q) ? 0.5 : 1.0;
   const double *Prow = P + size_t(Pg) * npad, *Qrow = P + size_t(Qg) * npad;
-  const double Ppq = Prow[Qg];
+  const double Ppq = Prow[Qg] * fI;
 
   int bs = 0, kk = k0;
   for (;; ++bs) {
@@ -156,6 +171,7 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
   int br = bs + kk;
 
   double *out = rec + size_t(rec0) * 64;
+  int rb = 0;  // buffer of red for the next cross-wave sum
   // this wave's two rows of the item's first quartet: element (r*8+s)*64 + lane, r = 2w + rr
   const E *src = eri + (size_t(Ib) * (Ib + 1) / 2 + size_t(k0)) * ZQ_QUARTET + (2 * wave) * 512 + lane;
   double Jbra = 0.0;
@@ -167,13 +183,13 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     Kps[s] = Kqs[s] = 0.0;
-    Pp_s[s] = Prow[bs * 8 + s];
-    Pq_s[s] = Qrow[bs * 8 + s];
+    Pp_s[s] = Prow[bs * 8 + s] * fI;
+    Pq_s[s] = Qrow[bs * 8 + s] * fI;
   }
 #pragma unroll
   for (int rr = 0; rr < 2; ++rr) {
-    Pr_cur[2 * rr] = Prow[br * 8 + 2 * wave + rr];
-    Pr_cur[2 * rr + 1] = Qrow[br * 8 + 2 * wave + rr];
+    Pr_cur[2 * rr] = Prow[br * 8 + 2 * wave + rr] * fI;
+    Pr_cur[2 * rr + 1] = Qrow[br * 8 + 2 * wave + rr] * fI;
   }
   // the quartet stream is read once per build: non-temporal loads, D quartets ahead of the contraction
 #pragma unroll
@@ -201,13 +217,25 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
     if (k + 1 < k1) {
 #pragma unroll
       for (int rr = 0; rr < 2; ++rr) {
-        Pr_nxt[2 * rr] = Prow[nbr * 8 + 2 * wave + rr];
-        Pr_nxt[2 * rr + 1] = Qrow[nbr * 8 + 2 * wave + rr];
+        Pr_nxt[2 * rr] = Prow[nbr * 8 + 2 * wave + rr] * fI;
+        Pr_nxt[2 * rr + 1] = Qrow[nbr * 8 + 2 * wave + rr] * fI;
       }
       // quartet k + 1 is already in flight (pf[0])
     }
     const int Jb = zq_bpair(br, bs);
     const bool diagJ = (br == bs), diagQ = (Jb == Ib);
+    if (diagJ || diagQ) {  // (uniform, rare) halve the ket pairs r == s and the bra-ket diagonal
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const int r = 2 * wave + rr;
+          double w = 1.0;
+          if (diagJ && r == s) w = 0.5;
+          if (diagQ && lane == r * 8 + s) w *= 0.5;
+          cur[rr * 8 + s] *= w;
+        }
+    }
     double jt[16], accpr[2], accqr[2];
 #pragma unroll
     for (int rr = 0; rr < 2; ++rr) {
@@ -217,11 +245,7 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
       double apr = 0.0, aqr = 0.0;
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
-        const double v = cur[rr * 8 + s];
-        double w = fI;
-        if (diagJ && r == s) w *= 0.5;
-        if (diagQ && lane == r * 8 + s) w *= 0.5;
-        const double a = v * w;
+        const double a = cur[rr * 8 + s];
         Jbra += Prs[s] * a;
         jt[rr * 8 + s] = Ppq * a;
         Kqs[s] += Pp_r * a;
@@ -242,24 +266,25 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
     out += 192;
 
     if (run_end || k == k1 - 1) {
-      red[wave][0][lane] = reduce_scatter8<4, 2, 1>(Kps, lane);    // K[p, bs*8+s], s = q
-      red[wave][1][lane] = reduce_scatter8<32, 16, 8>(Kqs, lane);  // K[q, bs*8+s], s = p
+      // buffer rb was last read by wave 0 before the previous cross-wave barrier, so it is free
+      red[rb][wave][0][lane] = sum8_scatter_low(Kps, lane);   // K[p, bs*8+s], s = q
+      red[rb][wave][1][lane] = sum8_scatter_high(Kqs, lane);  // K[q, bs*8+s], s = p
       __syncthreads();
       if (wave == 0) {
-        const double a = ((red[0][0][lane] + red[1][0][lane]) + red[2][0][lane]) + red[3][0][lane];
-        const double b = ((red[0][1][lane] + red[1][1][lane]) + red[2][1][lane]) + red[3][1][lane];
+        const double a = ((red[rb][0][0][lane] + red[rb][1][0][lane]) + red[rb][2][0][lane]) + red[rb][3][0][lane];
+        const double b = ((red[rb][0][1][lane] + red[rb][1][1][lane]) + red[rb][2][1][lane]) + red[rb][3][1][lane];
         out[p * 8 + q] = -xexch * a;
         out[64 + q * 8 + p] = -xexch * b;
       }
-      __syncthreads();
+      rb ^= 1;
       out += 128;
 #pragma unroll
       for (int s = 0; s < 8; ++s) Kps[s] = Kqs[s] = 0.0;
       if (run_end && k + 1 < k1) {
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-          Pp_s[s] = Prow[nbs * 8 + s];
-          Pq_s[s] = Qrow[nbs * 8 + s];
+          Pp_s[s] = Prow[nbs * 8 + s] * fI;
+          Pq_s[s] = Qrow[nbs * 8 + s] * fI;
         }
       }
     }
@@ -273,9 +298,10 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
 #pragma unroll
     for (int j = 0; j < 4; ++j) Pr_cur[j] = Pr_nxt[j];
   }
-  red[wave][0][lane] = Jbra;
+  red[rb][wave][0][lane] = Jbra * fI;
   __syncthreads();
-  if (wave == 0) out[lane] = 2.0 * (((red[0][0][lane] + red[1][0][lane]) + red[2][0][lane]) + red[3][0][lane]);
+  if (wave == 0)
+    out[lane] = 2.0 * (((red[rb][0][0][lane] + red[rb][1][0][lane]) + red[rb][2][0][lane]) + red[rb][3][0][lane]);
 }
 
 // F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).  Four waves

@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per zq_jk_records launch measured with FETCH_SIZE / WRITE_SIZE counters, scaled by a 1 GiB
 # copy (profiles/zq_jk_records_water8_hbm_traffic.md), by basis size n
-JK_TRAFFIC = {192: 1561123276}
+JK_TRAFFIC = {192: 1561121731}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3}
 METRIC = "MO-transform TFLOP/s + molecules/s, mixed vs FP64 at 1µEh; p50/p99 latency"
 

@@ -64,18 +64,21 @@ __device__ inline void tile_of(int t, int T, int &ti, int &tj) {
 }
 
 // Gershgorin bounds of the symmetric n x n F (ld n): scal[0] = min_i (F_ii - R_i), scal[1] = max_i (F_ii + R_i).
-// One thread per row; F is symmetric, so thread i sums column i, and the loads of a wave are coalesced.
+// One wave per row (wave w takes rows w, w + nw, ...): the lanes read the row coalesced and sum it with
+// a butterfly.
 __global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal,
                                                      const int *__restrict__ skip) {
   __shared__ double lo_s[16], hi_s[16];
   if (skip && *skip) return;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = int(blockDim.x) >> 6;
   double lo = 1e300, hi = -1e300;
-  for (int i = threadIdx.x; i < n; i += 1024) {
+#pragma unroll 2
+  for (int i = w; i < n; i += nw) {
+    const double *row = F + size_t(i) * n;
     double rsum = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < n; ++j) rsum += fabs(F[size_t(j) * n + i]);
-    const double d = F[size_t(i) * n + i];
+    for (int j = lane; j < n; j += 64) rsum += fabs(row[j]);
+    rsum = wave_sum(rsum);
+    const double d = row[i];
     rsum -= fabs(d);
     lo = fmin(lo, d - rsum);
     hi = fmax(hi, d + rsum);
@@ -88,7 +91,6 @@ __global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ 
   if (lane == 0) lo_s[w] = lo, hi_s[w] = hi;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = int(blockDim.x) >> 6;
     for (int k = 1; k < nw; ++k) lo_s[0] = fmin(lo_s[0], lo_s[k]), hi_s[0] = fmax(hi_s[0], hi_s[k]);
     // widen by a relative margin: the row sums above carry rounding errors
     const double pad = 1e-12 * fmax(fabs(lo_s[0]), fabs(hi_s[0]));
@@ -140,42 +142,68 @@ __global__ __launch_bounds__(64) void k_sp2_init(const double *__restrict__ F, i
   }
 }
 
-// The operands of one SP2 tile (ti <= tj): this lane's X_in elements and, for m <= 256 (PRE), every
-// operand row block of the product, so that the tile costs one memory round trip; larger m walk K with
-// the double-buffered tile_nt instead.
+// The operands of one SP2 tile (ti <= tj).  For m <= 256 (PRE) the workgroup's four waves split K by
+// 16-wide chunks, wave w taking chunks w, w + 4, w + 8, w + 12 below m, and each loads all of its chunks
+// at once: the tile costs one memory round trip and a quarter of the MFMA chain, and no MFMA multiplies
+// the zero padding above m.  Larger m: one wave walks K with the double-buffered tile_nt.  Wave 0 also
+// holds its own X_in elements, for 2 X - X^2.
 template <bool PRE>
 struct Sp2Ops {
   double x[4];
-  Frag f[PRE ? 4 : 1];
+  Frag f;
 };
+
+constexpr int sp2_threads(bool pre) { return pre ? 256 : 64; }
 
 template <bool PRE>
 __device__ inline void sp2_load(Sp2Ops<PRE> &o, const double *__restrict__ Xin, int m, int i0, int j0) {
-  const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
+  if (w == 0) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) o.x[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
+    for (int j = 0; j < 4; ++j) o.x[j] = Xin[size_t(i0 + 4 * j + q) * m + j0 + r];
+  }
   if (PRE) {
+    // m % 16 == 0 and m is even: a chunk lies wholly inside or wholly outside, and rows are 16-byte aligned
 #pragma unroll
-    for (int b = 0; b < 4; ++b) load_block<true>(o.f[b], Xin, m, Xin, m, i0 + r, j0 + r, m, m, m, 64 * b, q);
+    for (int i = 0; i < 4; ++i) {
+      const int c = w + 4 * i;
+      if (16 * c < m) {
+        const double2 *pa = reinterpret_cast<const double2 *>(Xin + size_t(i0 + r) * m + 16 * c + 4 * q);
+        const double2 *pb = reinterpret_cast<const double2 *>(Xin + size_t(j0 + r) * m + 16 * c + 4 * q);
+        const double2 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+        o.f.a[i][0] = a0.x, o.f.a[i][1] = a0.y, o.f.a[i][2] = a1.x, o.f.a[i][3] = a1.y;
+        o.f.b[i][0] = b0.x, o.f.b[i][1] = b0.y, o.f.b[i][2] = b1.x, o.f.b[i][3] = b1.y;
+      }
+    }
   }
 }
 
-// X_out tile = X_in^2 (square) or 2 X_in - X_in^2, stored with its mirror, and its partial traces in pout[t].
+// X_out tile = X_in^2 (square) or 2 X_in - X_in^2, stored with its mirror, and its partial traces in
+// pout[t].  Every wave of the workgroup calls it; PRE sums the waves' partial products in LDS, in wave
+// order, and wave 0 finishes the tile.
 template <bool PRE>
 __device__ inline void sp2_finish(const Sp2Ops<PRE> &o, const double *__restrict__ Xin, double *__restrict__ Xout, int m,
                                   int ti, int tj, bool square, double2 *__restrict__ pout, int t) {
-  const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
   d4 acc = {0.0, 0.0, 0.0, 0.0};
   if (PRE) {
+    __shared__ d4 red[3][64];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
+    for (int i = 0; i < 4; ++i)
+      if (16 * (w + 4 * i) < m)
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(o.f[b].a[c][u], o.f[b].b[c][u], acc, 0, 0, 0);
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(o.f.a[i][u], o.f.b[i][u], acc, 0, 0, 0);
+    if (w > 0) red[w - 1][lane] = acc;
+    __syncthreads();
+    if (w == 0) {
+      acc += red[0][lane];
+      acc += red[1][lane];
+      acc += red[2][lane];
+    }
   } else {
     acc = tile_nt<true>(Xin, m, Xin, m, i0, j0, m, m, m);
   }
+  if (w != 0) return;
   double v[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) v[j] = square ? acc[j] : 2.0 * o.x[j] - acc[j];
@@ -200,7 +228,7 @@ __device__ inline bool sp2_decide(double t1, double t2, double tol, int nocc, in
 
 __device__ inline void sum_partials(const double2 *__restrict__ pin, int ntiles, double &t1, double &t2) {
   t1 = 0.0, t2 = 0.0;
-  for (int u = threadIdx.x; u < ntiles; u += 64) t1 += pin[u].x, t2 += pin[u].y;
+  for (int u = threadIdx.x & 63; u < ntiles; u += 64) t1 += pin[u].x, t2 += pin[u].y;
   t1 = wave_sum(t1);
   t2 = wave_sum(t2);
 }
@@ -209,7 +237,7 @@ __device__ inline void sum_partials(const double2 *__restrict__ pin, int ntiles,
 // state = {count, steps}; count >= 2 means settled.  The step that brings count to 2 stores the parity
 // of its output buffer in *final; later steps only carry the state forward and return.
 template <bool PRE>
-__global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin, double *__restrict__ Xout, int m, int nocc,
+__global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_step(const double *__restrict__ Xin, double *__restrict__ Xout, int m, int nocc,
                                                  const double2 *__restrict__ pin, double2 *__restrict__ pout,
                                                  const int *__restrict__ sin, int *__restrict__ sout, int *__restrict__ final,
                                                  int pout_parity, double tol, const int *__restrict__ skip) {
@@ -240,31 +268,61 @@ __global__ __launch_bounds__(64) void k_sp2_step(const double *__restrict__ Xin,
 // result count as unsettled; a wait that sees *abort raised gives up too.
 constexpr unsigned kSpinLimit = 1u << 22;
 
-__device__ inline unsigned claim(unsigned *ticket) {
-  unsigned t = 0;
-  if (threadIdx.x == 0) t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __shfl(t, 0, 64);
-}
-
-__device__ inline void publish(unsigned *done) {
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline bool wait_done(const unsigned *done, unsigned target, int *abort) {
-  unsigned spins = 0;
-  while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-    if ((spins & 63) == 63 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
-    if (++spins > kSpinLimit) {
-      __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
+// With several waves a workgroup (NW > 1) takes its tiles through wave 0: lane 0 of wave 0 draws the
+// ticket and hands it over in LDS, wave 0 alone publishes (it alone writes the tile) and waits, and a
+// barrier hands its acquire on to the other waves.  The ticket slot alternates: slot s is written again
+// two claims later, after the barrier of the claim in between, which every wave passes only after it
+// has read slot s.
+template <int NW>
+__device__ inline unsigned claim(unsigned *ticket, int &slot) {
+  if (NW == 1) {
+    unsigned t = 0;
+    if (threadIdx.x == 0) t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __shfl(t, 0, 64);
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  __shared__ unsigned s_ticket[2];
+  if (threadIdx.x == 0) s_ticket[slot] = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  return true;
+  const unsigned t = s_ticket[slot];
+  slot ^= 1;
+  return t;
+}
+
+// Counts a finished tile.  Every wave has read X_in before the tile's reduction barrier, so only the
+// writer, wave 0, orders its stores before the count.
+template <int NW>
+__device__ inline void publish(unsigned *done) {
+  if (NW == 1) __syncthreads();
+  if (threadIdx.x < 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int NW>
+__device__ inline bool wait_done(const unsigned *done, unsigned target, int *abort) {
+  __shared__ int s_ok;
+  bool ok = true;
+  if (threadIdx.x < 64) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if ((spins & 63) == 63 && __hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        ok = false;
+        break;
+      }
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = false;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (NW > 1 && threadIdx.x == 0) s_ok = ok;
+  }
+  __syncthreads();
+  if (NW > 1) ok = s_ok;
+  return ok;
 }
 
 // Every step of SP2 in one launch.  The tiles of step k are handed out by the ticket counter sync[k];
@@ -277,7 +335,7 @@ __device__ inline bool wait_done(const unsigned *done, unsigned target, int *abo
 // the step-per-launch form.  state gets {count, steps} in the pair 2 * (nsteps & 1) and the settled
 // buffer's parity in [4], as k_sp2_step leaves them; [5] is the abort flag.
 template <bool PRE>
-__global__ __launch_bounds__(64) void k_sp2_run(double *__restrict__ X, size_t mm, int m, int nocc,
+__global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_run(double *__restrict__ X, size_t mm, int m, int nocc,
                                                 double2 *__restrict__ part, unsigned *__restrict__ sync,
                                                 int *__restrict__ state, int nsteps, double tol,
                                                 const int *__restrict__ skip) {
@@ -285,8 +343,9 @@ __global__ __launch_bounds__(64) void k_sp2_run(double *__restrict__ X, size_t m
   const int T = m / 16, ntiles = T * (T + 1) / 2;
   unsigned *ticket = sync, *done = sync + kSp2MaxSteps;
   int *abort = state + 5;
-  int count = -1, steps = 0, fin = 0;
-  unsigned t = claim(&ticket[0]);
+  constexpr int NW = sp2_threads(PRE) / 64;
+  int count = -1, steps = 0, fin = 0, slot = 0;
+  unsigned t = claim<NW>(&ticket[0], slot);
   for (int k = 0; k < nsteps; ++k) {
     const double *Xin = X + (k & 1) * mm;
     double *Xout = X + ((k + 1) & 1) * mm;
@@ -305,16 +364,16 @@ __global__ __launch_bounds__(64) void k_sp2_run(double *__restrict__ X, size_t m
     double2 *pout = part + size_t(k + 1) * ntiles;
     while (t < unsigned(ntiles)) {
       sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, square, pout, int(t));
-      publish(&done[k]);
-      t = claim(&ticket[k]);
+      publish<NW>(&done[k]);
+      t = claim<NW>(&ticket[k], slot);
       if (t < unsigned(ntiles)) {
         tile_of(int(t), T, ti, tj);
         sp2_load<PRE>(o, Xin, m, ti * 16, tj * 16);
       }
     }
     if (count >= 2 || k + 1 == nsteps) break;  // the launch's end publishes the last step
-    t = claim(&ticket[k + 1]);
-    if (!wait_done(&done[k], unsigned(ntiles), abort)) break;
+    t = claim<NW>(&ticket[k + 1], slot);
+    if (!wait_done<NW>(&done[k], unsigned(ntiles), abort)) break;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     int *pair = state + 2 * (nsteps & 1);
@@ -368,7 +427,7 @@ int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
 }
 
 void sp2_start(zq_ctx *ctx, const Sp2Bufs &b, const double *dF, int n, int *state, const int *skip) {
-  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(std::min(1024, (n + 63) / 64 * 64)), 0, ctx->stream, dF, n, b.scal, skip);
+  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(1024), 0, ctx->stream, dF, n, b.scal, skip);
   hipLaunchKernelGGL(k_sp2_init, dim3(b.ntiles), dim3(64), 0, ctx->stream, dF, n, b.m, b.scal, b.X, b.part, state, b.sync,
                      skip);
 }
@@ -379,7 +438,7 @@ void sp2_step(zq_ctx *ctx, const Sp2Bufs &b, int k, int nocc, double tol, int *s
   const double2 *pi = b.part + p * b.ntiles;
   double2 *po = b.part + (p ^ 1) * b.ntiles;
   if (b.m <= 256)
-    hipLaunchKernelGGL(k_sp2_step<true>, dim3(b.ntiles), dim3(64), 0, ctx->stream, Xi, Xo, b.m, nocc, pi, po,
+    hipLaunchKernelGGL(k_sp2_step<true>, dim3(b.ntiles), dim3(sp2_threads(true)), 0, ctx->stream, Xi, Xo, b.m, nocc, pi, po,
                        state + 2 * p, state + 2 * (p ^ 1), state + 4, p ^ 1, tol, skip);
   else
     hipLaunchKernelGGL(k_sp2_step<false>, dim3(b.ntiles), dim3(64), 0, ctx->stream, Xi, Xo, b.m, nocc, pi, po,
@@ -397,7 +456,7 @@ int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, i
   sp2_start(ctx, b, dF, n, state6, skip);
   // one workgroup per tile: each normally takes one tile a step
   if (b.m <= 256)
-    hipLaunchKernelGGL(k_sp2_run<true>, dim3(b.ntiles), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc, b.part, b.sync,
+    hipLaunchKernelGGL(k_sp2_run<true>, dim3(b.ntiles), dim3(sp2_threads(true)), 0, ctx->stream, b.X, b.mm, b.m, nocc, b.part, b.sync,
                        state6, nsteps, tol, skip);
   else
     hipLaunchKernelGGL(k_sp2_run<false>, dim3(std::min(b.ntiles, 1024)), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc,

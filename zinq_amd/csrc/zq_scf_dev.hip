@@ -325,37 +325,75 @@ __global__ __launch_bounds__(256) void k_dots_state(const double *__restrict__ h
   if (threadIdx.x == 0) part[blockIdx.y * kRedBlocks + blockIdx.x] = red[0];
 }
 
-// LU with partial pivoting and the triangular solves, the same steps as the host zq::getrf/getrs.
-__device__ bool lu_solve(int n, double *A, double *b) {
-  int piv[kMaxHist + 1];
-  for (int k = 0; k < n; ++k) {
-    int p = k;
-    double best = fabs(A[k * n + k]);
-    for (int i = k + 1; i < n; ++i)
-      if (fabs(A[i * n + k]) > best) best = fabs(A[i * n + k]), p = i;
-    piv[k] = p;
-    if (best == 0.0) return false;
-    if (p != k)
-      for (int j = 0; j < n; ++j) {
-        const double t = A[k * n + j];
-        A[k * n + j] = A[p * n + j], A[p * n + j] = t;
+// LU with partial pivoting and the triangular solves of the host zq::getrf/getrs, on one wave: lane i
+// holds row i of A (n <= kLuMax) and b[i] in registers.  Every element sees the same operations in the
+// same order as in the serial steps: the same pivot (the first row with the largest |A[i][k]|), the same
+// multipliers and updates, the forward sums in ascending j, the backward sums in ascending j.  The rows
+// of b are swapped as the pivots are found, which permutes b exactly as applying the swaps afterwards
+// does.  Returns false, on every lane, for a zero pivot; otherwise lane i < n holds x[i] in b.
+constexpr int kLuMax = kMaxHist + 1;
+
+__device__ __forceinline__ bool lu_solve_wave(int n, double (&a)[kLuMax], double &b) {
+  // every loop has a constant trip count and guards its body, so that it unrolls fully and a[] stays in
+  // registers
+  const int lane = threadIdx.x & 63;
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < kLuMax; ++k) {
+    if (k < n && ok) {
+      double v = (lane >= k && lane < n) ? fabs(a[k]) : -1.0;
+      int p = lane;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const double vo = __shfl_xor(v, o, 64);
+        const int po = __shfl_xor(p, o, 64);
+        if (vo > v || (vo == v && po < p)) v = vo, p = po;
       }
-    for (int i = k + 1; i < n; ++i) {
-      const double f = A[i * n + k] / A[k * n + k];
-      A[i * n + k] = f;
-      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      ok = v != 0.0;
+      if (ok && p != k) {
+        const int src = lane == k ? p : (lane == p ? k : lane);
+#pragma unroll
+        for (int j = 0; j < kLuMax; ++j)
+          if (j < n) a[j] = __shfl(a[j], src, 64);
+        b = __shfl(b, src, 64);
+      }
+      const double akk = __shfl(a[k], k, 64);
+      const bool below = ok && lane > k && lane < n;
+      double f = 0.0;
+      if (below) {
+        f = a[k] / akk;
+        a[k] = f;
+      }
+#pragma unroll
+      for (int j = k + 1; j < kLuMax; ++j) {
+        if (j < n) {
+          const double akj = __shfl(a[j], k, 64);
+          if (below) a[j] -= f * akj;
+        }
+      }
     }
   }
-  for (int k = 0; k < n; ++k)
-    if (piv[k] != k) {
-      const double t = b[k];
-      b[k] = b[piv[k]], b[piv[k]] = t;
+  if (!ok) return false;
+#pragma unroll
+  for (int j = 0; j < kLuMax; ++j) {
+    if (j < n) {
+      const double bj = __shfl(b, j, 64);
+      if (lane > j && lane < n) b -= a[j] * bj;
     }
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < i; ++j) b[i] -= A[i * n + j] * b[j];
-  for (int i = n - 1; i >= 0; --i) {
-    for (int j = i + 1; j < n; ++j) b[i] -= A[i * n + j] * b[j];
-    b[i] /= A[i * n + i];
+  }
+#pragma unroll
+  for (int i = kLuMax - 1; i >= 0; --i) {
+    if (i < n) {
+      double acc = b;
+#pragma unroll
+      for (int j = i + 1; j < kLuMax; ++j) {
+        if (j < n) {
+          const double bj = __shfl(b, j, 64);
+          acc -= a[j] * bj;
+        }
+      }
+      if (lane == i) b = acc / a[i];
+    }
   }
   return true;
 }
@@ -363,7 +401,7 @@ __device__ bool lu_solve(int n, double *A, double *b) {
 // E = sum of the energy partials + V_NN; DIIS: the new row of B, the bordered system, its LU solve
 // (src/hartree_fock.zig:176-227), or a reset to the current entry when it is singular (:821-845).
 // One wave: lane b holds partial b of every sum (kRedBlocks == 64), each sum a fixed-order butterfly;
-// lane 0 solves the <= 17 x 17 system.
+// the wave solves the <= 17 x 17 system, one row per lane.
 // epart: the nep energy partials the J/K reduce wrote (one per output block).
 __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ part, const double *__restrict__ epart,
                                                    int nep, int diis, double VN, ScfState *__restrict__ st) {
@@ -372,11 +410,17 @@ __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ pa
   if (st->done) return;
   const int lane = threadIdx.x;
   const int m = st->hist_n, slot = st->cur;
-  int hist[kMaxHist];  // the history in registers: every lane needs it, and st-> reads serialise
+  // every load is issued before any of them is used: the history, all the dot partials, the energy
+  int hist[kMaxHist];  // the history in registers: every lane needs it
+  double dp[kMaxHist];
 #pragma unroll
-  for (int h = 0; h < kMaxHist; ++h) hist[h] = h < m ? st->hist[h] : -1;
+  for (int h = 0; h < kMaxHist; ++h) hist[h] = st->hist[h];
+#pragma unroll
+  for (int k = 0; k < kMaxHist; ++k) dp[k] = part[(1 + k) * kRedBlocks + lane];
   double e = 0.0;
   for (int b = lane; b < nep; b += 64) e += epart[b];
+#pragma unroll
+  for (int h = 0; h < kMaxHist; ++h) hist[h] = h < m ? hist[h] : -1;
   e = zqt::wave_sum(e);
   double dots[kMaxHist];
 #pragma unroll
@@ -384,7 +428,8 @@ __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ pa
     bool live = false;
 #pragma unroll
     for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
-    dots[k] = (live && diis > 0) ? zqt::wave_sum(part[(1 + k) * kRedBlocks + lane]) : 0.0;
+    const double d = zqt::wave_sum(dp[k]);
+    dots[k] = (live && diis > 0) ? d : 0.0;
   }
   // the m x m block of B over the history, one entry per lane, with the new row and column from dots
   for (int idx = lane; diis > 0 && m >= 2 && idx < m * m; idx += 64) {
@@ -408,30 +453,33 @@ __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ pa
     Bs[i * kMaxHist + j] = b;
   }
   __syncthreads();
-  if (lane != 0) return;
-  st->e_new = e + VN;
-  st->combine = 0;
-  if (diis <= 0) return;
+  if (lane == 0) {
+    st->e_new = e + VN;
+    st->combine = 0;
+    if (diis > 0) {
 #pragma unroll
-  for (int k = 0; k < kMaxHist; ++k) {
-    bool live = false;
+      for (int k = 0; k < kMaxHist; ++k) {
+        bool live = false;
 #pragma unroll
-    for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
-    if (live) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
+        for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
+        if (live) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
+      }
+    }
   }
-  if (m < 2) return;
-  const int dd = m + 1;
-  __shared__ double A[(kMaxHist + 1) * (kMaxHist + 1)], c[kMaxHist + 1];  // LDS, not per-lane scratch
-  for (int i = 0; i < dd * dd; ++i) A[i] = 0.0;
-  for (int i = 0; i < dd; ++i) c[i] = 0.0;
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < m; ++j) A[i * dd + j] = Bs[i * kMaxHist + j];
-  for (int i = 0; i < m; ++i) A[i * dd + m] = A[m * dd + i] = -1.0;
-  c[m] = -1.0;
-  if (lu_solve(dd, A, c)) {
-    for (int i = 0; i < m; ++i) st->coef[i] = c[i];
-    st->combine = 1;
-  } else {
+  if (diis <= 0 || m < 2) return;
+  // the bordered system [B -1; -1 0] c = [0; -1], row i on lane i
+  double a[kLuMax], c = lane == m ? -1.0 : 0.0;
+#pragma unroll
+  for (int j = 0; j < kLuMax; ++j) {
+    double x = 0.0;
+    if (lane < m) x = j < m ? Bs[lane * kMaxHist + j] : (j == m ? -1.0 : 0.0);
+    else if (lane == m) x = j < m ? -1.0 : 0.0;
+    a[j] = x;
+  }
+  if (lu_solve_wave(m + 1, a, c)) {
+    if (lane < m) st->coef[lane] = c;
+    if (lane == 0) st->combine = 1;
+  } else if (lane == 0) {
     st->hist[0] = slot;
     st->hist_n = 1;
   }

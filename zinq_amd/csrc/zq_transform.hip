@@ -136,19 +136,42 @@ __device__ inline E q1_val(const E *__restrict__ eri, const Q1Src &s, int K, int
   return eri[s.base + idx];
 }
 
+// Off the diagonal (bmu != bnu, br != bs, Ib != Jb) the element (mu = bmu*8+u, nu = bnu*8+w | K) of a
+// quartet sits at base + K sK + u su + w sw, with strides fixed per block; the diagonal blocks take
+// q1_val's general path.
+struct Q1Lin {
+  int sK, su, sw;
+  bool general;
+};
+
+__device__ inline Q1Lin q1_lin(const Q1Src &s) {
+  Q1Lin l;
+  l.general = s.diagK || s.diagB || s.Ib == s.Jb;
+  const bool bra_fast = s.Ib > s.Jb;  // [ket][bra] storage: the bra pair (pl, ql) is the fast index
+  const int sp = bra_fast ? 8 : 512, sq = bra_fast ? 1 : 64;
+  l.sK = bra_fast ? 64 : 1;
+  l.su = s.swp ? sq : sp;
+  l.sw = s.swp ? sp : sq;
+  return l;
+}
+
 // ---------------------------------------------------------------- Q1, FP64 / FP32
-// Workgroup (512 threads) owns ket block pair Jb (64 slots) x AO block bnu (8 nu) x a group of 64 i.
-// Loop over mu blocks: stage T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then
-// X[K][i][nu] += C1[mu,i] T via MFMA.  Output tiles: MT (<= 4) tiles of 16 i by 32 tiles of 16 (K,nu)
-// columns, split evenly over the 8 waves.
+// Workgroup (256 threads) owns half kh of ket block pair Jb's 64 slots (K = 32 kh .. 32 kh + 31) x AO
+// block bnu (8 nu) x a group of 16 MT occupied orbitals.  Loop over mu blocks: stage
+// T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then X[K][i][nu] += C1[mu,i] T via MFMA.
+// Output tiles: MT tiles of 16 i by 16 tiles of 16 (K,nu) columns, split evenly over the 4 waves.
+// Two such workgroups fit a CU, so one's staging and barriers overlap the other's MFMAs.  The staging
+// loads follow the quartet's storage order: bra-fastest quartets (Ib > Jb) are read along (u,w),
+// ket-fastest ones (Ib < Jb) along K.
 template <class MF, int MT, class E>
-__global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
+__global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
                                              int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
   typedef typename MF::T T;
   constexpr int TPW = MT * 4;  // tiles per wave
-  __shared__ T Ts[64][65];
+  __shared__ T Ts[32][65];
   const int Jb = blockIdx.x / nb, bnu = blockIdx.x % nb;
   const int i0 = blockIdx.y * (MT * 16);  // MT tiles of 16 occupied orbitals per workgroup
+  const int K0 = 32 * blockIdx.z;
   int br, bs;
   zq_unpair(Jb, br, bs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -158,7 +181,7 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
 
   // A wave's tiles span at most two 16-row i tiles (it_lo, it_lo + 1): their C1 operands for a mu block
   // are four values per lane, loaded with the block instead of once per MFMA.
-  const int it_lo = (wave * TPW) >> 5;
+  const int it_lo = (wave * TPW) >> 4;
   auto load_a = [&](int bm, T (&a)[2][2]) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
@@ -170,33 +193,52 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
       }
     }
   };
+  // element j of this thread's share of a staged block: (K local, u*8+w)
+  auto slot = [&](int j, bool kfast, int &kl, int &uw) {
+    const int e = tid + 256 * j;
+    if (kfast)
+      kl = e & 31, uw = e >> 5;
+    else
+      kl = e >> 6, uw = e & 63;
+  };
+  auto fetch = [&](int bm, T (&v)[8], bool &kfast) {
+    const Q1Src src = q1_src(bm, bnu, br, bs, Jb);
+    const Q1Lin lin = q1_lin(src);
+    kfast = src.Ib < src.Jb;
+    if (!lin.general) {
+      const E *q = eri + src.base;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int kl, uw;
+        slot(j, kfast, kl, uw);
+        v[j] = T(q[(K0 + kl) * lin.sK + (uw >> 3) * lin.su + (uw & 7) * lin.sw]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int kl, uw;
+        slot(j, kfast, kl, uw);
+        v[j] = T(q1_val(eri, src, K0 + kl, uw >> 3, uw & 7));
+      }
+    }
+  };
   T a_cur[2][2], a_nxt[2][2];
   load_a(0, a_cur);
   // the next mu block's values are loaded into registers while the current one is multiplied
   T rv[8];
-  {
-    const Q1Src src = q1_src(0, bnu, br, bs, Jb);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = tid + 512 * j;
-      rv[j] = T(q1_val(eri, src, e >> 6, (e & 63) >> 3, e & 7));
-    }
-  }
+  bool kf;
+  fetch(0, rv, kf);
   for (int bmu = 0; bmu < nb; ++bmu) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int e = tid + 512 * j;
-      Ts[e >> 6][e & 63] = rv[j];
+      int kl, uw;
+      slot(j, kf, kl, uw);
+      Ts[kl][uw] = rv[j];
     }
     __syncthreads();
     if (bmu + 1 < nb) {
-      const Q1Src nsrc = q1_src(bmu + 1, bnu, br, bs, Jb);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int e = tid + 512 * j;
-        rv[j] = T(q1_val(eri, nsrc, e >> 6, (e & 63) >> 3, e & 7));
-      }
+      fetch(bmu + 1, rv, kf);
       load_a(bmu + 1, a_nxt);
     }
 #pragma unroll
@@ -204,10 +246,10 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
       const int uu = ks * 4 + (lane >> 4);
 #pragma unroll
       for (int t = 0; t < TPW; ++t) {
-        const int tile = wave * TPW + t, it = tile >> 5, nt = tile & 31;
+        const int tile = wave * TPW + t, it = tile >> 4, nt = tile & 15;
         const T a = (it == it_lo) ? a_cur[ks][0] : a_cur[ks][1];
-        const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
-        acc[t] = MF::mma(a, Ts[K][uu * 8 + wl], acc[t]);
+        const int col = nt * 16 + (lane & 15), kl = col >> 3, wl = col & 7;
+        acc[t] = MF::mma(a, Ts[kl][uu * 8 + wl], acc[t]);
       }
     }
 #pragma unroll
@@ -219,8 +261,8 @@ __global__ __launch_bounds__(512) void zq_q1(const E *__restrict__ eri, const ty
   }
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
-    const int tile = wave * TPW + t, it = tile >> 5, nt = tile & 31;
-    const int col = nt * 16 + (lane & 15), K = col >> 3, wl = col & 7;
+    const int tile = wave * TPW + t, it = tile >> 4, nt = tile & 15;
+    const int col = nt * 16 + (lane & 15), K = K0 + (col >> 3), wl = col & 7;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = i0 + it * 16 + MF::row(lane, j);
@@ -402,6 +444,13 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
     for (int y = 0; y < 2; ++y) acc[x][y] = MF::zero();
   const bool a_kfast = (g.sAk == 1);
   const bool b_nfast = (g.mode == 1) || (g.sBn == 1);
+  // 16 x 16 sub-tiles wholly outside M or N (o = 40 rows in a 64-row tile, v = 152 columns in three
+  // 64-column tiles) take no MFMA: the test is uniform over the wave, and the SIMD time goes to others
+  bool live[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) live[x][y] = m0 + wm + 16 * x < g.M && n0 + wn + 16 * y < g.N;
   T ra[4], rb[4];
   gemm_load<MF>(g, A, B, m0, n0, 0, a_kfast, b_nfast, ra, rb);
   gemm_stage<T>(As[0], Bs[0], ra, rb, a_kfast, b_nfast);
@@ -421,7 +470,8 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = MF::mma(a[x], bb[y], acc[x][y]);
+        for (int y = 0; y < 2; ++y)
+          if (live[x][y]) acc[x][y] = MF::mma(a[x], bb[y], acc[x][y]);
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -642,6 +692,7 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
     const int mt = std::min(cap, (std::min(m1, 16 * cap) + 15) / 16);
     const int groups = precision == ZQ_PREC_BF16X3 ? (m1 + 63) / 64 : (m1 + 16 * mt - 1) / (16 * mt);
     dim3 grid(unsigned(Nb * nb), unsigned(groups));
+    const dim3 grid2(unsigned(Nb * nb), unsigned(groups), 2u);  // FP64 / FP32: two halves of the ket slots
     zq_prof_begin(ctx, "ao2mo_q1");
     if (precision == ZQ_PREC_BF16X3) {
       float *Xf = (float *)X;
@@ -656,18 +707,18 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
         // FP32 forms: read the FP32 copy of the quartets (the same rounding as converting on the fly)
         const float *e32 = ctx->eri32;
         switch (mt) {
-          case 1: hipLaunchKernelGGL((zq_q1<MF, 1, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
-          case 2: hipLaunchKernelGGL((zq_q1<MF, 2, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
-          case 3: hipLaunchKernelGGL((zq_q1<MF, 3, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
-          default: hipLaunchKernelGGL((zq_q1<MF, 4, float>), grid, dim3(512), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          case 1: hipLaunchKernelGGL((zq_q1<MF, 1, float>), grid2, dim3(256), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          case 2: hipLaunchKernelGGL((zq_q1<MF, 2, float>), grid2, dim3(256), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          case 3: hipLaunchKernelGGL((zq_q1<MF, 3, float>), grid2, dim3(256), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
+          default: hipLaunchKernelGGL((zq_q1<MF, 4, float>), grid2, dim3(256), 0, ctx->stream, e32, d1, m1, m1, nb, np, X); break;
         }
       } else {
         const double *e64 = ctx->eri;
         switch (mt) {
-          case 1: hipLaunchKernelGGL((zq_q1<MF, 1, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
-          case 2: hipLaunchKernelGGL((zq_q1<MF, 2, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
-          case 3: hipLaunchKernelGGL((zq_q1<MF, 3, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
-          default: hipLaunchKernelGGL((zq_q1<MF, 4, double>), grid, dim3(512), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          case 1: hipLaunchKernelGGL((zq_q1<MF, 1, double>), grid2, dim3(256), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          case 2: hipLaunchKernelGGL((zq_q1<MF, 2, double>), grid2, dim3(256), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          case 3: hipLaunchKernelGGL((zq_q1<MF, 3, double>), grid2, dim3(256), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
+          default: hipLaunchKernelGGL((zq_q1<MF, 4, double>), grid2, dim3(256), 0, ctx->stream, e64, d1, m1, m1, nb, np, X); break;
         }
       }
     }

@@ -62,6 +62,12 @@ int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
  * on ctx -- 1 the device-decided iteration, 2 the host-synchronised restart -- or 0 before any run. */
 int zq_scf_last_driver(const zq_ctx *ctx);
 
+/* Diagnostics, with no counterpart in the reference: the SP2 densities of the last zq_scf_run_device on
+ * ctx that the device-decided iteration finished (0s otherwise): how many, how many of them took
+ * certified scale-and-fold steps (HOMO/LUMO bounds from an earlier plain density of the same SCF,
+ * shifted by ||F' - F'_ref||_F), and the SP2 steps of all of them.  Returns ZQ_OK or ZQ_ERR_INVALID. */
+int zq_scf_last_sp2(const zq_ctx *ctx, int *densities, int *folded, int *steps);
+
 /* GPU symmetric eigensolver, eigh of src/linear_algebra.zig:45-71 (LAPACKE_dsyevd, row-major, 'V'):
  * A = U diag(W) U^T for the symmetric n x n A (row-major), W ascending, U (n x n row-major) with the
  * eigenvectors in its columns.  For n <= 192 the tridiagonal reduction is one workgroup holding A in

@@ -170,6 +170,10 @@ def test_device_scf_matches_oracle(ctx, kind):
     o = s.nel // 2
     r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
     assert ctx.last_scf_driver() == "device"  # decided on the device, no host-synchronised restart
+    # the later densities fold with certified HOMO/LUMO bounds, and still give the aufbau density
+    sp = ctx.last_scf_sp2()
+    print(kind, r["iterations"], sp)
+    assert sp["densities"] >= r["iterations"] and 1 <= sp["folded"] < sp["densities"]
     ro = O.scf(H, S, g, VN, o, threshold=1e-10)
     assert r["E"] == pytest.approx(ro["E"], abs=1e-10)
     # geigh contract: F C = S C e, C^T S C = I, e ascending; MP2 from these orbitals agrees

@@ -22,9 +22,24 @@
 //   writes per-tile partial traces (tr X, ||X||_F^2 = tr X^2), which every workgroup of the next step
 //   sums in the same fixed order, so all of them take the same branch.  X is ping-ponged between two
 //   buffers (input [p], output [p^1]).
-//   Two forms, bitwise equal: k_sp2_run takes every step in one launch (tiles handed out by ticket, a
-//   step's tiles counted before the next reads X), which the device SCF uses; k_sp2_step is one launch
-//   per step, used by the host-checked driver.
+//   Two forms, bitwise equal on plain runs: k_sp2_run takes every step in one launch (tiles handed out
+//   by ticket, a step's tiles counted before the next reads X), which the device SCF uses; k_sp2_step
+//   is one launch per step, used by the host-checked driver.
+//
+// Certified scale-and-fold steps (one-launch form in the device SCF).  With bounds a_k >= every
+// virtual eigenvalue of X_k and b_k <= every occupied one (a_k < b_k), the down step may fold at
+// s = a_k / 2:  X <- ((X - s I) / (1 - s))^2, which maps [0, a_k] into [0, f(a_k)] and keeps [b_k, 1]
+// above it, and the up step at s = (1 - b_k) / 2:  X <- 2 X / (1 - s) - X^2 / (1 - s)^2, its mirror
+// image.  Each step folds the side whose bound lags (down when a_k > 1 - b_k); the gap widens about
+// twice as fast per step as with X^2 / 2X - X^2, both bounds follow the same maps, and as they approach
+// {0, 1} the fold points vanish.  The bounds come from an earlier
+// plain run of the same SCF: its traces at the first step with tr X - tr X^2 < 1e-3 give
+// sum_virt x <= ((idem + tr X - N) / 2 + 4 idem^2) and sum_occ (1 - x) <= ((idem - tr X + N) / 2 + 4 idem^2),
+// hence the LUMO's image from above and the HOMO's from below, which the inverted branch maps carry back
+// to energies (a lower bound of the LUMO, an upper bound of the HOMO of that run's F').  By Weyl's
+// inequality no eigenvalue of the new F' lies further than D = ||F' - F'_ref||_F from the old one, so
+// LUMO - D and HOMO + D bound the new run; it folds only when the gap left, (LUMO - D) - (HOMO + D),
+// exceeds half the estimated gap, and runs plain otherwise (becoming the new reference).
 
 #include <algorithm>
 
@@ -34,6 +49,26 @@
 namespace {
 
 using namespace zqt;
+
+// Sp2Bufs::est, in doubles: the fold bookkeeping kept across the SP2 runs of one device SCF.
+enum : int {
+  kEstValid = 0,      // 1: ref holds F' of a settled plain run, and kEstHomo / kEstLumo bound its HOMO / LUMO
+  kEstHomo = 1,       // upper bound of that run's HOMO energy
+  kEstLumo = 2,       // lower bound of its LUMO energy
+  kEstLastPlain = 3,  // 1: the last run was plain: ref holds its F', its traces are in the history
+  kEstLastLo = 4,     // and its Gershgorin interval
+  kEstLastHi = 5,
+  kEstFold = 6,       // this run folds, with the bounds kEstA0 (virtual) and kEstB0 (occupied) in X_0's scale
+  kEstA0 = 7,
+  kEstB0 = 8,
+  kEstSteps = 9,      // steps the last run recorded, and whether it settled
+  kEstSettled = 10,
+  kEstRuns = 11,      // since the SCF's first density: densities, folded ones, SP2 steps in all
+  kEstFolds = 12,
+  kEstStepSum = 13,
+  kEstHist = 16,      // history[k] = {tr X_k, tr X_k^2, 1 for a down step, fold point}
+  kEstSize = kEstHist + 4 * kSp2MaxSteps
+};
 
 template <bool VEC>
 __global__ __launch_bounds__(64) void k_gemm_nt(const double *__restrict__ A, int lda, const double *__restrict__ B,
@@ -63,20 +98,55 @@ __device__ inline void tile_of(int t, int T, int &ti, int &tj) {
   tj = ti + t;
 }
 
+// HOMO (upper bound) and LUMO (lower bound) energies of the last plain run from its trace history.
+__device__ bool sp2_estimate(const double *est, int nocc, double &eh, double &el) {
+  const int steps = int(est[kEstSteps]);
+  const double *h = est + kEstHist;
+  int ks = -1;
+  for (int k = 0; k < steps && ks < 0; ++k)
+    if (h[4 * k] - h[4 * k + 1] < 1e-3) ks = k;
+  if (ks < 0) return false;
+  const double idem = h[4 * ks] - h[4 * ks + 1], d = h[4 * ks] - double(nocc), r = 4.0 * idem * idem + 1e-12;
+  double xl = fmin(fmax(0.5 * (idem + d) + r, 0.0), 1.0), xh = fmin(fmax(1.0 - (0.5 * (idem - d) + r), 0.0), 1.0);
+  for (int k = ks - 1; k >= 0; --k) {
+    if (h[4 * k + 2] != 0.0) {
+      xl = sqrt(xl), xh = sqrt(xh);
+    } else {
+      xl = 1.0 - sqrt(fmax(1.0 - xl, 0.0)), xh = 1.0 - sqrt(fmax(1.0 - xh, 0.0));
+    }
+  }
+  const double lo = est[kEstLastLo], hi = est[kEstLastHi];
+  const double inv = 1.0 / fmax(hi - lo, 1e-10 * fmax(1.0, fabs(hi)));
+  eh = hi - xh / inv;
+  el = hi - xl / inv;
+  return el > eh;
+}
+
 // Gershgorin bounds of the symmetric n x n F (ld n): scal[0] = min_i (F_ii - R_i), scal[1] = max_i (F_ii + R_i).
 // One wave per row (wave w takes rows w, w + nw, ...): the lanes read the row coalesced and sum it with
-// a butterfly.
+// a butterfly.  track != 0 (one-launch SP2 of the device SCF; 1 starts an SCF) also decides whether
+// this run folds: it turns the last plain run's history into bounds, measures ||F - ref||_F, and
+// writes kEstFold / kEstA0 / kEstB0; a plain run copies F into ref.
 __global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal,
-                                                     const int *__restrict__ skip) {
-  __shared__ double lo_s[16], hi_s[16];
+                                                     const int *__restrict__ skip, int track, int nocc,
+                                                     double *__restrict__ est, double *__restrict__ ref) {
+  __shared__ double lo_s[16], hi_s[16], dd_s[16];
+  __shared__ int fold_s;
   if (skip && *skip) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = int(blockDim.x) >> 6;
-  double lo = 1e300, hi = -1e300;
+  double lo = 1e300, hi = -1e300, dd = 0.0;
 #pragma unroll 2
   for (int i = w; i < n; i += nw) {
     const double *row = F + size_t(i) * n;
     double rsum = 0.0;
-    for (int j = lane; j < n; j += 64) rsum += fabs(row[j]);
+    for (int j = lane; j < n; j += 64) {
+      const double f = row[j];
+      rsum += fabs(f);
+      if (track == 2) {
+        const double df = f - ref[size_t(i) * n + j];
+        dd += df * df;
+      }
+    }
     rsum = wave_sum(rsum);
     const double d = row[i];
     rsum -= fabs(d);
@@ -88,15 +158,52 @@ __global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ 
     lo = fmin(lo, __shfl_xor(lo, o, 64));
     hi = fmax(hi, __shfl_xor(hi, o, 64));
   }
-  if (lane == 0) lo_s[w] = lo, hi_s[w] = hi;
+  dd = wave_sum(dd);
+  if (lane == 0) lo_s[w] = lo, hi_s[w] = hi, dd_s[w] = dd;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < nw; ++k) lo_s[0] = fmin(lo_s[0], lo_s[k]), hi_s[0] = fmax(hi_s[0], hi_s[k]);
+    double dsum = dd_s[0];
+    for (int k = 1; k < nw; ++k) lo_s[0] = fmin(lo_s[0], lo_s[k]), hi_s[0] = fmax(hi_s[0], hi_s[k]), dsum += dd_s[k];
     // widen by a relative margin: the row sums above carry rounding errors
     const double pad = 1e-12 * fmax(fabs(lo_s[0]), fabs(hi_s[0]));
-    scal[0] = lo_s[0] - pad;
-    scal[1] = hi_s[0] + pad;
+    const double emin = lo_s[0] - pad, emax = hi_s[0] + pad;
+    scal[0] = emin;
+    scal[1] = emax;
+    int fold = 0;
+    if (track) {
+      double valid = track == 1 ? 0.0 : est[kEstValid];
+      if (track == 2 && est[kEstLastPlain] != 0.0) {
+        // ref holds the last run's F': its history gives the bounds, when it settled
+        double eh, el;
+        valid = 0.0;
+        if (est[kEstSettled] != 0.0 && sp2_estimate(est, nocc, eh, el)) {
+          est[kEstHomo] = eh, est[kEstLumo] = el;
+          valid = 1.0;
+        }
+      }
+      est[kEstValid] = valid;
+      if (valid != 0.0) {
+        const double D = sqrt(dsum), eh = est[kEstHomo], el = est[kEstLumo];
+        const double lumo = el - D, homo = eh + D;
+        if (lumo - homo > 0.5 * (el - eh)) {
+          const double inv = 1.0 / fmax(emax - emin, 1e-10 * fmax(1.0, fabs(emax)));
+          fold = 1;
+          est[kEstA0] = fmin(fmax((emax - lumo) * inv, 0.0), 1.0);
+          est[kEstB0] = fmin(fmax((emax - homo) * inv, 0.0), 1.0);
+        }
+      }
+      est[kEstFold] = fold;
+      est[kEstLastPlain] = fold ? 0.0 : 1.0;
+      if (track == 1) est[kEstRuns] = 0.0, est[kEstFolds] = 0.0, est[kEstStepSum] = 0.0;
+      est[kEstRuns] += 1.0;
+      est[kEstFolds] += fold;
+      if (!fold) est[kEstLastLo] = emin, est[kEstLastHi] = emax;
+    }
+    fold_s = fold;
   }
+  __syncthreads();
+  if (track && !fold_s)
+    for (size_t e = threadIdx.x; e < size_t(n) * n; e += blockDim.x) ref[e] = F[e];
 }
 
 // Writes the partial traces of the tile this wave holds (vals[j] at rows i0+4j+q, column j0+r).
@@ -178,12 +285,18 @@ __device__ inline void sp2_load(Sp2Ops<PRE> &o, const double *__restrict__ Xin, 
   }
 }
 
-// X_out tile = X_in^2 (square) or 2 X_in - X_in^2, stored with its mirror, and its partial traces in
+// X_out tile = alpha X_in^2 + beta X_in + gamma I, stored with its mirror, and its partial traces in
 // pout[t].  Every wave of the workgroup calls it; PRE sums the waves' partial products in LDS, in wave
 // order, and wave 0 finishes the tile.
+// The polynomial of one step, X_(k+1) = alpha X_k^2 + beta X_k + gamma I; s is the fold point (0 plain).
+struct Sp2Map {
+  double alpha, beta, gamma, s;
+  bool down;
+};
+
 template <bool PRE>
 __device__ inline void sp2_finish(const Sp2Ops<PRE> &o, const double *__restrict__ Xin, double *__restrict__ Xout, int m,
-                                  int ti, int tj, bool square, double2 *__restrict__ pout, int t) {
+                                  int ti, int tj, const Sp2Map &p, double2 *__restrict__ pout, int t) {
   const int i0 = ti * 16, j0 = tj * 16, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4, w = threadIdx.x >> 6;
   d4 acc = {0.0, 0.0, 0.0, 0.0};
   if (PRE) {
@@ -206,7 +319,11 @@ __device__ inline void sp2_finish(const Sp2Ops<PRE> &o, const double *__restrict
   if (w != 0) return;
   double v[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) v[j] = square ? acc[j] : 2.0 * o.x[j] - acc[j];
+  for (int j = 0; j < 4; ++j) {
+    // plain steps: exactly X^2 (alpha 1, beta 0) or 2X - X^2 (alpha -1, beta 2), one rounding
+    v[j] = fma(p.alpha, acc[j], p.beta * o.x[j]);
+    if (i0 + 4 * j + q == j0 + r) v[j] += p.gamma;
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int row = i0 + 4 * j + q, col = j0 + r;
@@ -216,14 +333,34 @@ __device__ inline void sp2_finish(const Sp2Ops<PRE> &o, const double *__restrict
   tile_partials(v, ti, tj, i0, j0, pout, t);
 }
 
-// The stop rule and branch of one step from the summed partials (t1 = tr X_k, t2 = tr X_k^2): count < 0
-// before idem < tol was first seen, then the number of steps taken since; the step that brings count to
-// 2 produces the settled projector.
-__device__ inline bool sp2_decide(double t1, double t2, double tol, int nocc, int &count) {
+// The stop rule and the step's map from the summed partials (t1 = tr X_k, t2 = tr X_k^2 over the m x m
+// grid): count < 0 before idem < tol was first seen, then the number of steps taken since; the step that
+// brings count to 2 produces the settled projector.  Plain: X^2 or 2X - X^2, whichever keeps the trace
+// closer to N.  Folding (bounds a < b carried in a, b): the folded map of the side whose bound lags,
+// down when a > 1 - b, else up; both bounds go through it.  (The trace rule cannot choose between
+// folded maps: a folded down step lifts every eigenvalue near 0, the zero padding included, to
+// s^2/(1-s)^2, which makes its trace look worse than it is and keeps the iteration going up.)
+__device__ inline Sp2Map sp2_map(double t1, double t2, double tol, int nocc, int &count, bool fold, double &a,
+                                 double &b) {
   const double N = double(nocc);
   if (count < 0 && t1 - t2 < tol) count = 0;
   if (count >= 0) ++count;
-  return fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
+  Sp2Map p;
+  if (!fold || !(b > a)) {
+    p.down = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
+    p.alpha = p.down ? 1.0 : -1.0, p.beta = p.down ? 0.0 : 2.0, p.gamma = 0.0, p.s = 0.0;
+    return p;
+  }
+  const double sd = 0.5 * a, su = 0.5 * (1.0 - b);
+  const double cd = 1.0 / (1.0 - sd), cu = 1.0 / (1.0 - su);
+  p.down = a > 1.0 - b;
+  if (p.down)
+    p.alpha = cd * cd, p.beta = -2.0 * sd * cd * cd, p.gamma = sd * sd * cd * cd, p.s = sd;
+  else
+    p.alpha = -cu * cu, p.beta = 2.0 * cu, p.gamma = 0.0, p.s = su;
+  a = p.alpha * a * a + p.beta * a + p.gamma;
+  b = p.alpha * b * b + p.beta * b + p.gamma;
+  return p;
 }
 
 __device__ inline void sum_partials(const double2 *__restrict__ pin, int ntiles, double &t1, double &t2) {
@@ -254,14 +391,14 @@ __global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_step(const double *__r
   const int steps = sin[1];
   Sp2Ops<PRE> o;
   sp2_load<PRE>(o, Xin, m, ti * 16, tj * 16);
-  double t1, t2;
+  double t1, t2, a = 0.0, b = 1.0;
   sum_partials(pin, ntiles, t1, t2);
-  const bool square = sp2_decide(t1, t2, tol, nocc, count);
+  const Sp2Map mp = sp2_map(t1, t2, tol, nocc, count, false, a, b);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sout[0] = count, sout[1] = steps + 1;
     if (count == 2) *final = pout_parity;  // X_out is the settled projector
   }
-  sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, square, pout, blockIdx.x);
+  sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, mp, pout, blockIdx.x);
 }
 
 // Bounded waits of the one-launch SP2: a wait that runs out raises *abort, and so makes the launch's
@@ -338,10 +475,15 @@ template <bool PRE>
 __global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_run(double *__restrict__ X, size_t mm, int m, int nocc,
                                                 double2 *__restrict__ part, unsigned *__restrict__ sync,
                                                 int *__restrict__ state, int nsteps, double tol,
-                                                const int *__restrict__ skip) {
+                                                const int *__restrict__ skip, double *__restrict__ est) {
   if (skip && *skip) return;
   const int T = m / 16, ntiles = T * (T + 1) / 2;
   unsigned *ticket = sync, *done = sync + kSp2MaxSteps;
+  // folding bounds, carried identically by every wave; the history is kept by block 0
+  bool fold = false;
+  double fa = 0.0, fb = 1.0;
+  if (est && est[kEstFold] != 0.0) fold = true, fa = est[kEstA0], fb = est[kEstB0];
+  double *hist = (est && blockIdx.x == 0 && threadIdx.x == 0) ? est + kEstHist : nullptr;
   int *abort = state + 5;
   constexpr int NW = sp2_threads(PRE) / 64;
   int count = -1, steps = 0, fin = 0, slot = 0;
@@ -358,12 +500,13 @@ __global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_run(double *__restrict
     }
     double t1, t2;
     sum_partials(part + size_t(k) * ntiles, ntiles, t1, t2);
-    const bool square = sp2_decide(t1, t2, tol, nocc, count);
+    const Sp2Map mp = sp2_map(t1, t2, tol, nocc, count, fold, fa, fb);
+    if (hist) hist[4 * k] = t1, hist[4 * k + 1] = t2, hist[4 * k + 2] = mp.down ? 1.0 : 0.0, hist[4 * k + 3] = mp.s;
     steps = k + 1;
     if (count == 2) fin = (k + 1) & 1;
     double2 *pout = part + size_t(k + 1) * ntiles;
     while (t < unsigned(ntiles)) {
-      sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, square, pout, int(t));
+      sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, mp, pout, int(t));
       publish<NW>(&done[k]);
       t = claim<NW>(&ticket[k], slot);
       if (t < unsigned(ntiles)) {
@@ -379,6 +522,7 @@ __global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_run(double *__restrict
     int *pair = state + 2 * (nsteps & 1);
     pair[0] = count, pair[1] = steps;
     state[4] = fin;
+    if (est) est[kEstSteps] = steps, est[kEstSettled] = count >= 2 ? 1.0 : 0.0, est[kEstStepSum] += steps;
   }
 }
 
@@ -410,7 +554,7 @@ namespace {
 struct Sp2Bufs {
   int m, T, ntiles;
   size_t mm;
-  double *X, *scal;
+  double *X, *scal, *est, *ref;
   double2 *part;
   unsigned *sync;
 };
@@ -422,12 +566,16 @@ int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
   b.part = (double2 *)zq_scratch(ctx, "sp2_part", size_t(kSp2MaxSteps + 1) * b.ntiles * sizeof(double2));
   b.scal = (double *)zq_scratch(ctx, "sp2_scal", 2 * sizeof(double));
   b.sync = (unsigned *)zq_scratch(ctx, "sp2_sync", 2 * kSp2MaxSteps * sizeof(unsigned));
-  if (!b.X || !b.part || !b.scal || !b.sync) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  b.est = (double *)zq_scratch(ctx, "sp2_est", kEstSize * sizeof(double));
+  b.ref = (double *)zq_scratch(ctx, "sp2_ref", size_t(n) * n * sizeof(double));
+  if (!b.X || !b.part || !b.scal || !b.sync || !b.est || !b.ref) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
   return ZQ_OK;
 }
 
-void sp2_start(zq_ctx *ctx, const Sp2Bufs &b, const double *dF, int n, int *state, const int *skip) {
-  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(1024), 0, ctx->stream, dF, n, b.scal, skip);
+// track: 0 plain SP2 outside the fold bookkeeping, 1 the first run of a device SCF, 2 a later run.
+void sp2_start(zq_ctx *ctx, const Sp2Bufs &b, const double *dF, int n, int nocc, int *state, const int *skip,
+               int track) {
+  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(1024), 0, ctx->stream, dF, n, b.scal, skip, track, nocc, b.est, b.ref);
   hipLaunchKernelGGL(k_sp2_init, dim3(b.ntiles), dim3(64), 0, ctx->stream, dF, n, b.m, b.scal, b.X, b.part, state, b.sync,
                      skip);
 }
@@ -448,24 +596,36 @@ void sp2_step(zq_ctx *ctx, const Sp2Bufs &b, int k, int nocc, double tol, int *s
 }  // namespace
 
 int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state6,
-                   const double **dX, long long *stride, int *ld) {
+                   const double **dX, long long *stride, int *ld, int track) {
   if (nsteps < 1 || nsteps > kSp2MaxSteps) return zq_fail(ctx, ZQ_ERR_INVALID, "SP2 step budget out of range");
+  if (track < 0 || track > 2) return zq_fail(ctx, ZQ_ERR_INVALID, "SP2 fold tracking mode out of range");
   Sp2Bufs b;
   if (int rc = sp2_bufs(ctx, n, b)) return rc;
   zq_prof_begin(ctx, "scf_sp2");
-  sp2_start(ctx, b, dF, n, state6, skip);
+  sp2_start(ctx, b, dF, n, nocc, state6, skip, track);
+  double *est = track ? b.est : nullptr;
   // one workgroup per tile: each normally takes one tile a step
   if (b.m <= 256)
     hipLaunchKernelGGL(k_sp2_run<true>, dim3(b.ntiles), dim3(sp2_threads(true)), 0, ctx->stream, b.X, b.mm, b.m, nocc, b.part, b.sync,
-                       state6, nsteps, tol, skip);
+                       state6, nsteps, tol, skip, est);
   else
     hipLaunchKernelGGL(k_sp2_run<false>, dim3(std::min(b.ntiles, 1024)), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc,
-                       b.part, b.sync, state6, nsteps, tol, skip);
+                       b.part, b.sync, state6, nsteps, tol, skip, est);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "k_sp2_run");
   *dX = b.X;
   *stride = (long long)b.mm;
   *ld = b.m;
+  return ZQ_OK;
+}
+
+int zq_sp2_tracked_stats(zq_ctx *ctx, int out[3]) {
+  auto *est = (const double *)zq_scratch(ctx, "sp2_est", kEstSize * sizeof(double));
+  if (!est) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  double h[3] = {0.0, 0.0, 0.0};
+  ZQ_TRY(ctx, hipMemcpyAsync(h, est + kEstRuns, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; ++k) out[k] = int(h[k]);
   return ZQ_OK;
 }
 
@@ -476,7 +636,7 @@ int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, i
   auto *state = (int *)zq_scratch(ctx, "sp2_state", 6 * sizeof(int));
   if (!state) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
   zq_prof_begin(ctx, "scf_sp2");
-  sp2_start(ctx, b, dF, n, state, nullptr);
+  sp2_start(ctx, b, dF, n, nocc, state, nullptr, 0);
   int launched = 0, h[2] = {0, 0}, fin = 0;
   int next_check = std::max(4, std::min(max_steps, *steps_hint + 3));
   while (true) {

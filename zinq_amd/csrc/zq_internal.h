@@ -102,6 +102,7 @@ struct zq_ctx {
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
   int scf_driver = 0;    // driver that finished the last device SCF: 1 device-decided, 2 host-synchronised
+  int scf_sp2[3] = {0, 0, 0};  // its SP2 densities, folded ones, steps (device-decided driver only)
   hipEvent_t scf_events[2] = {nullptr, nullptr};  // the asynchronous SCF's lagged state reads
 
   // named scratch buffers (grow-only)
@@ -187,14 +188,18 @@ int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, i
 // host synchronisation.  state6 = two {count, steps} pairs, the parity of the buffer that holds the
 // settled projector, and an abort flag: the projector is *dX + state6[4] * *stride (m x m, ld m), which
 // a kernel selects at run time.  The final pair is state6 + 2 * (nsteps & 1); the result is settled when
-// count >= 2 there and state6[5] == 0.  All kernels return at once while *skip != 0.
+// count >= 2 there and state6[5] == 0.  All kernels return at once while *skip != 0.  track = 1 (the
+// first density of a device SCF) or 2 (a later one of the same SCF) lets the run fold with HOMO/LUMO
+// bounds certified from an earlier plain run (zq_dense.hip); 0 runs plain SP2 only.
 constexpr int kSp2MaxSteps = 128;
 // Eigen-decomposition of the symmetric n x n A (ld n, overwritten): w = eigenvalues ascending, row k of
 // A = eigenvector k.  offd: n doubles of workspace; info: device int (0 on success).  n <= 192 by the
 // one-workgroup tridiagonal reduction of zq_eigen.hip and rocSOLVER dstedc, larger n by dsyevd.
 int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *info);  // zq_eigen.hip
 int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, int nsteps, const int *skip, int *state6,
-                   const double **dX, long long *stride, int *ld);  // zq_dense.hip
+                   const double **dX, long long *stride, int *ld, int track);  // zq_dense.hip
+// {densities, folded densities, SP2 steps} of the tracked runs since the last track = 1 run (synchronous).
+int zq_sp2_tracked_stats(zq_ctx *ctx, int out[3]);  // zq_dense.hip
 
 #define ZQ_TRY(ctx, call)                                \
   do {                                                   \

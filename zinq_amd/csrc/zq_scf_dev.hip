@@ -564,7 +564,8 @@ int density_async(Dev &d, const double *A, int nocc, double factor, int nsteps, 
   const double *D = nullptr;
   long long stride = 0;
   int ld = 0;
-  if (int rc = zq_sp2_enqueue(ctx, d.Fp, n, nocc, 1e-9, nsteps, skip, st->sp2, &D, &stride, &ld)) return rc;
+  // the first density of the SCF starts the fold bookkeeping; later ones may fold
+  if (int rc = zq_sp2_enqueue(ctx, d.Fp, n, nocc, 1e-9, nsteps, skip, st->sp2, &D, &stride, &ld, check ? 1 : 2)) return rc;
   if (check) hipLaunchKernelGGL(k_sp2_check, dim3(1), dim3(64), 0, ctx->stream, nsteps & 1, st);
   if (int rc = zq_gemm_nt(ctx, d.X, n, D, ld, d.T, n, n, n, n, 1.0, "scf_gemm", skip, st->sp2 + 4, stride)) return rc;
   if (int rc = zq_gemm_nt(ctx, d.T, n, d.X, n, d.Pn, n, n, n, n, factor, "scf_gemm", skip)) return rc;
@@ -858,6 +859,9 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   }
   if (status == 3) ctx->sp2_budget = std::min(kSp2MaxSteps, ctx->sp2_budget + 16);
   ctx->scf_driver = (status == 0 || status == 3) ? 2 : 1;
+  ctx->scf_sp2[0] = ctx->scf_sp2[1] = ctx->scf_sp2[2] = 0;
+  if (ctx->scf_driver == 1)
+    if (int rc = zq_sp2_tracked_stats(ctx, ctx->scf_sp2)) return rc;
   if (status == 0 || status == 3) {
     // DIIS history beyond the device bookkeeping, or an SP2 budget too small: synchronised driver
     // (FP64 Fock builds throughout)
@@ -901,3 +905,9 @@ extern "C" int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps) {
 }
 
 extern "C" int zq_scf_last_driver(const zq_ctx *ctx) { return ctx ? ctx->scf_driver : 0; }
+
+extern "C" int zq_scf_last_sp2(const zq_ctx *ctx, int *densities, int *folded, int *steps) {
+  if (!ctx || !densities || !folded || !steps) return ZQ_ERR_INVALID;
+  *densities = ctx->scf_sp2[0], *folded = ctx->scf_sp2[1], *steps = ctx->scf_sp2[2];
+  return ZQ_OK;
+}

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const ty
                                              int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
   typedef typename MF::T T;
   constexpr int TPW = MT * 4;  // tiles per wave
-  __shared__ T Ts[32][65];
+  __shared__ T Ts[2][32][65];
   const int Jb = blockIdx.x / nb, bnu = blockIdx.x % nb;
   const int i0 = blockIdx.y * (MT * 16);  // MT tiles of 16 occupied orbitals per workgroup
   const int K0 = 32 * blockIdx.z;
@@ -222,22 +222,27 @@ __global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const ty
       }
     }
   };
-  T a_cur[2][2], a_nxt[2][2];
-  load_a(0, a_cur);
-  // the next mu block's values are loaded into registers while the current one is multiplied
-  T rv[8];
-  bool kf;
-  fetch(0, rv, kf);
-  for (int bmu = 0; bmu < nb; ++bmu) {
-    __syncthreads();
+  auto stage = [&](const T (&v)[8], bool kfast, T (*dst)[65]) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       int kl, uw;
-      slot(j, kf, kl, uw);
-      Ts[kl][uw] = rv[j];
+      slot(j, kfast, kl, uw);
+      dst[kl][uw] = v[j];
     }
-    __syncthreads();
-    if (bmu + 1 < nb) {
+  };
+  T a_cur[2][2], a_nxt[2][2];
+  load_a(0, a_cur);
+  // double-buffered LDS: block bmu + 1 is fetched before block bmu is multiplied out of Ts[cur] and
+  // staged into Ts[cur ^ 1] after it, so each mu block waits on one barrier
+  T rv[8];
+  bool kf;
+  fetch(0, rv, kf);
+  stage(rv, kf, Ts[0]);
+  __syncthreads();
+  int cur = 0;
+  for (int bmu = 0; bmu < nb; ++bmu) {
+    const bool more = bmu + 1 < nb;
+    if (more) {
       fetch(bmu + 1, rv, kf);
       load_a(bmu + 1, a_nxt);
     }
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const ty
         const int tile = wave * TPW + t, it = tile >> 4, nt = tile & 15;
         const T a = (it == it_lo) ? a_cur[ks][0] : a_cur[ks][1];
         const int col = nt * 16 + (lane & 15), kl = col >> 3, wl = col & 7;
-        acc[t] = MF::mma(a, Ts[kl][uu * 8 + wl], acc[t]);
+        acc[t] = MF::mma(a, Ts[cur][kl][uu * 8 + wl], acc[t]);
       }
     }
 #pragma unroll
@@ -258,6 +263,9 @@ __global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const ty
       for (int h = 0; h < 2; ++h) a_cur[ks][h] = a_nxt[ks][h];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) MF::flush(acc[t]);
+    if (more) stage(rv, kf, Ts[cur ^ 1]);
+    __syncthreads();
+    cur ^= 1;
   }
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {

@@ -252,15 +252,23 @@ void launch_tridiag(hipStream_t s, int count, const double *A, int lda, long lon
   hipLaunchKernelGGL((k_tridiag<RB, TR>), dim3(count), dim3(16 * TR), 0, s, A, lda, sA, n, d, e, sDE, Q, sQ);
 }
 
-rocblas_handle eig_handle(zq_ctx *ctx) {
+}  // namespace
+
+// The context's rocBLAS handle, on its stream, with atomics off so that every product is reproducible.
+void *zq_blas(zq_ctx *ctx) {
   if (!ctx->solver) {
     rocblas_handle h = nullptr;
     if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
     rocblas_set_stream(h, ctx->stream);
+    rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed);
     ctx->solver = h;
   }
-  return (rocblas_handle)ctx->solver;
+  return ctx->solver;
 }
+
+namespace {
+
+rocblas_handle eig_handle(zq_ctx *ctx) { return (rocblas_handle)zq_blas(ctx); }
 
 }  // namespace
 

@@ -157,6 +157,8 @@ void zq_free_eri(zq_ctx *ctx);
 void zq_prof_begin(zq_ctx *ctx, const char *name);
 void zq_prof_end(zq_ctx *ctx);
 void zq_release_solver(zq_ctx *ctx);  // zq_scf_dev.hip, also drops the captured SCF graph
+// The context's rocBLAS handle (rocblas_handle), bound to its stream, atomics off; null on failure.
+void *zq_blas(zq_ctx *ctx);  // zq_eigen.hip
 int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_exch, double *dF,
                       const int *skip = nullptr, bool f32 = false, double *epart = nullptr);  // zq_jk.hip
 int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, made once per load

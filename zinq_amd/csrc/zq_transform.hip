@@ -25,6 +25,8 @@
 #include <cmath>
 #include <type_traits>
 
+#include <rocblas/rocblas.h>
+
 #include "zq_internal.h"
 
 namespace {
@@ -760,6 +762,22 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
   }
   if (int rc = launch_gemm<MF>(ctx, g2, m1, "ao2mo_q2")) return rc;
   if (int rc = launch_gemm<MF>(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
+  if constexpr (std::is_same<MF, MF64>::value) {
+    // FP64 Q4 is a plain strided-batched product with W_ij read transposed: rocBLAS DGEMM (column-major,
+    // so each row-major matrix is read as its transpose), atomics off, measured faster than the tile
+    // kernel above for this shape.
+    auto h = (rocblas_handle)zq_blas(ctx);
+    if (!h) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocblas handle");
+    const double one = 1.0, zero = 0.0;
+    zq_prof_begin(ctx, "ao2mo_q4");
+    const rocblas_status st =
+        rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m4, m3, np, &one, d3, m4, 0, W,
+                                      m3, rocblas_stride(np) * m3, &zero, O, m4, rocblas_stride(m3) * m4, m1 * m2);
+    zq_prof_end(ctx);
+    if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAUNCH, "ao2mo_q4: rocblas_dgemm_strided_batched");
+    *d_out = O;
+    return ZQ_OK;
+  }
   if (int rc = launch_gemm<MF>(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
   *d_out = O;
   return ZQ_OK;

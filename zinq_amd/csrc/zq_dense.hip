@@ -66,6 +66,7 @@ enum : int {
   kEstRuns = 11,      // since the SCF's first density: densities, folded ones, SP2 steps in all
   kEstFolds = 12,
   kEstStepSum = 13,
+  kEstRefSel = 14,    // which of the two n x n reference buffers holds F'_ref
   kEstHist = 16,      // history[k] = {tr X_k, tr X_k^2, 1 for a down step, fold point}
   kEstSize = kEstHist + 4 * kSp2MaxSteps
 };
@@ -98,10 +99,9 @@ __device__ inline void tile_of(int t, int T, int &ti, int &tj) {
   tj = ti + t;
 }
 
-// HOMO (upper bound) and LUMO (lower bound) energies of the last plain run from its trace history.
-__device__ bool sp2_estimate(const double *est, int nocc, double &eh, double &el) {
-  const int steps = int(est[kEstSteps]);
-  const double *h = est + kEstHist;
+// HOMO (upper bound) and LUMO (lower bound) energies of the last plain run (steps steps, Gershgorin
+// interval [lo, hi]) from its trace history h (a copy in LDS of est + kEstHist).
+__device__ bool sp2_estimate(int steps, double lo, double hi, const double *h, int nocc, double &eh, double &el) {
   int ks = -1;
   for (int k = 0; k < steps && ks < 0; ++k)
     if (h[4 * k] - h[4 * k + 1] < 1e-3) ks = k;
@@ -115,7 +115,6 @@ __device__ bool sp2_estimate(const double *est, int nocc, double &eh, double &el
       xl = 1.0 - sqrt(fmax(1.0 - xl, 0.0)), xh = 1.0 - sqrt(fmax(1.0 - xh, 0.0));
     }
   }
-  const double lo = est[kEstLastLo], hi = est[kEstLastHi];
   const double inv = 1.0 / fmax(hi - lo, 1e-10 * fmax(1.0, fabs(hi)));
   eh = hi - xh / inv;
   el = hi - xl / inv;
@@ -123,67 +122,106 @@ __device__ bool sp2_estimate(const double *est, int nocc, double &eh, double &el
 }
 
 // Gershgorin bounds of the symmetric n x n F (ld n): scal[0] = min_i (F_ii - R_i), scal[1] = max_i (F_ii + R_i).
-// One wave per row (wave w takes rows w, w + nw, ...): the lanes read the row coalesced and sum it with
-// a butterfly.  track != 0 (one-launch SP2 of the device SCF; 1 starts an SCF) also decides whether
-// this run folds: it turns the last plain run's history into bounds, measures ||F - ref||_F, and
-// writes kEstFold / kEstA0 / kEstB0; a plain run copies F into ref.
-__global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal,
-                                                     const int *__restrict__ skip, int track, int nocc,
-                                                     double *__restrict__ est, double *__restrict__ ref) {
-  __shared__ double lo_s[16], hi_s[16], dd_s[16];
-  __shared__ int fold_s;
+// kGershBlocks workgroups of 4 waves, one wave per row (global wave g takes rows g, g + 4 kGershBlocks,
+// ...): the lanes read the row coalesced and sum it with a butterfly.  Each workgroup leaves its
+// partials in gpart and counts itself in *gcount after a release fence; the last one, after an acquire
+// fence, combines them in workgroup order and clears the count.  track != 0 (one-launch SP2 of the
+// device SCF; 1 starts an SCF) also decides whether this run folds: the last workgroup turns the last
+// plain run's history into bounds, takes ||F - ref||_F against the reference buffer kEstRefSel names,
+// and writes kEstFold / kEstA0 / kEstB0.  The same pass stores F into the other buffer; a plain run
+// makes that one the reference.
+constexpr int kGershBlocks = 16;
+
+__global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F, int n, double *__restrict__ scal,
+                                                    const int *__restrict__ skip, int track, int nocc,
+                                                    double *__restrict__ est, double *__restrict__ ref,
+                                                    double *__restrict__ gpart, unsigned *__restrict__ gcount) {
+  __shared__ double lo_s[4], hi_s[4], dd_s[4], hist_s[4 * kSp2MaxSteps];
+  __shared__ int last_s;
   if (skip && *skip) return;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = int(blockDim.x) >> 6;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the bookkeeping scalars are read before the rows, so that their latency hides behind the sums
+  double e_valid = 0.0, e_homo = 0.0, e_lumo = 0.0, e_lo = 0.0, e_hi = 0.0, e_runs = 0.0, e_folds = 0.0;
+  int e_steps = 0, sel = 0;
+  bool last_plain = false, pending = false;
+  if (track) {
+    e_valid = est[kEstValid], e_homo = est[kEstHomo], e_lumo = est[kEstLumo];
+    e_lo = est[kEstLastLo], e_hi = est[kEstLastHi], e_steps = int(est[kEstSteps]);
+    sel = int(est[kEstRefSel]) & 1;
+    e_runs = est[kEstRuns], e_folds = est[kEstFolds];
+    last_plain = est[kEstLastPlain] != 0.0;
+    pending = track == 2 && last_plain && est[kEstSettled] != 0.0;
+  }
+  const double *rold = ref + size_t(sel) * n * n;
+  double *rnew = ref + size_t(sel ^ 1) * n * n;
   double lo = 1e300, hi = -1e300, dd = 0.0;
-#pragma unroll 2
-  for (int i = w; i < n; i += nw) {
+  const int gw = blockIdx.x * 4 + w, ngw = gridDim.x * 4;
+  for (int i = gw; i < n; i += ngw) {
     const double *row = F + size_t(i) * n;
+    const double d = row[i];
     double rsum = 0.0;
-    for (int j = lane; j < n; j += 64) {
-      const double f = row[j];
-      rsum += fabs(f);
-      if (track == 2) {
-        const double df = f - ref[size_t(i) * n + j];
-        dd += df * df;
+    for (int j0 = 0; j0 < n; j0 += 256) {
+      double f[4], r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + 64 * u + lane;
+        f[u] = j < n ? row[j] : 0.0;
+        r[u] = (track == 2 && j < n) ? rold[size_t(i) * n + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = j0 + 64 * u + lane;
+        rsum += fabs(f[u]);
+        const double df = f[u] - r[u];
+        if (track == 2) dd += df * df;
+        if (track && j < n) rnew[size_t(i) * n + j] = f[u];
       }
     }
-    rsum = wave_sum(rsum);
-    const double d = row[i];
-    rsum -= fabs(d);
+    rsum = wave_sum(rsum) - fabs(d);
     lo = fmin(lo, d - rsum);
     hi = fmax(hi, d + rsum);
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    lo = fmin(lo, __shfl_xor(lo, o, 64));
-    hi = fmax(hi, __shfl_xor(hi, o, 64));
   }
   dd = wave_sum(dd);
   if (lane == 0) lo_s[w] = lo, hi_s[w] = hi, dd_s[w] = dd;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double dsum = dd_s[0];
-    for (int k = 1; k < nw; ++k) lo_s[0] = fmin(lo_s[0], lo_s[k]), hi_s[0] = fmax(hi_s[0], hi_s[k]), dsum += dd_s[k];
+    gpart[3 * blockIdx.x] = fmin(fmin(lo_s[0], lo_s[1]), fmin(lo_s[2], lo_s[3]));
+    gpart[3 * blockIdx.x + 1] = fmax(fmax(hi_s[0], hi_s[1]), fmax(hi_s[2], hi_s[3]));
+    gpart[3 * blockIdx.x + 2] = ((dd_s[0] + dd_s[1]) + dd_s[2]) + dd_s[3];
+    __threadfence();
+    last_s = atomicAdd(gcount, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last_s) return;
+  __threadfence();
+  // the last plain run's history into LDS for the estimate
+  if (pending)
+    for (int e = threadIdx.x; e < 4 * e_steps; e += blockDim.x) hist_s[e] = est[kEstHist + e];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *gcount = 0u;
+    double glo = gpart[0], ghi = gpart[1], dsum = gpart[2];
+    for (int k = 1; k < int(gridDim.x); ++k)
+      glo = fmin(glo, gpart[3 * k]), ghi = fmax(ghi, gpart[3 * k + 1]), dsum += gpart[3 * k + 2];
     // widen by a relative margin: the row sums above carry rounding errors
-    const double pad = 1e-12 * fmax(fabs(lo_s[0]), fabs(hi_s[0]));
-    const double emin = lo_s[0] - pad, emax = hi_s[0] + pad;
+    const double pad = 1e-12 * fmax(fabs(glo), fabs(ghi));
+    const double emin = glo - pad, emax = ghi + pad;
     scal[0] = emin;
     scal[1] = emax;
     int fold = 0;
     if (track) {
-      double valid = track == 1 ? 0.0 : est[kEstValid];
-      if (track == 2 && est[kEstLastPlain] != 0.0) {
+      double valid = track == 1 ? 0.0 : e_valid, eh = e_homo, el = e_lumo;
+      if (track == 2 && last_plain) {
         // ref holds the last run's F': its history gives the bounds, when it settled
-        double eh, el;
         valid = 0.0;
-        if (est[kEstSettled] != 0.0 && sp2_estimate(est, nocc, eh, el)) {
+        if (pending && sp2_estimate(e_steps, e_lo, e_hi, hist_s, nocc, eh, el)) {
           est[kEstHomo] = eh, est[kEstLumo] = el;
           valid = 1.0;
         }
       }
       est[kEstValid] = valid;
       if (valid != 0.0) {
-        const double D = sqrt(dsum), eh = est[kEstHomo], el = est[kEstLumo];
+        const double D = sqrt(dsum);
         const double lumo = el - D, homo = eh + D;
         if (lumo - homo > 0.5 * (el - eh)) {
           const double inv = 1.0 / fmax(emax - emin, 1e-10 * fmax(1.0, fabs(emax)));
@@ -194,16 +232,13 @@ __global__ __launch_bounds__(1024) void k_gershgorin(const double *__restrict__ 
       }
       est[kEstFold] = fold;
       est[kEstLastPlain] = fold ? 0.0 : 1.0;
-      if (track == 1) est[kEstRuns] = 0.0, est[kEstFolds] = 0.0, est[kEstStepSum] = 0.0;
-      est[kEstRuns] += 1.0;
-      est[kEstFolds] += fold;
+      if (!fold) est[kEstRefSel] = double(sel ^ 1);
+      if (track == 1) e_runs = 0.0, e_folds = 0.0, est[kEstStepSum] = 0.0;
+      est[kEstRuns] = e_runs + 1.0;
+      est[kEstFolds] = e_folds + fold;
       if (!fold) est[kEstLastLo] = emin, est[kEstLastHi] = emax;
     }
-    fold_s = fold;
   }
-  __syncthreads();
-  if (track && !fold_s)
-    for (size_t e = threadIdx.x; e < size_t(n) * n; e += blockDim.x) ref[e] = F[e];
 }
 
 // Writes the partial traces of the tile this wave holds (vals[j] at rows i0+4j+q, column j0+r).
@@ -554,9 +589,9 @@ namespace {
 struct Sp2Bufs {
   int m, T, ntiles;
   size_t mm;
-  double *X, *scal, *est, *ref;
+  double *X, *scal, *est, *ref, *gpart;
   double2 *part;
-  unsigned *sync;
+  unsigned *sync, *gcount;
 };
 
 int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
@@ -567,15 +602,21 @@ int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
   b.scal = (double *)zq_scratch(ctx, "sp2_scal", 2 * sizeof(double));
   b.sync = (unsigned *)zq_scratch(ctx, "sp2_sync", 2 * kSp2MaxSteps * sizeof(unsigned));
   b.est = (double *)zq_scratch(ctx, "sp2_est", kEstSize * sizeof(double));
-  b.ref = (double *)zq_scratch(ctx, "sp2_ref", size_t(n) * n * sizeof(double));
-  if (!b.X || !b.part || !b.scal || !b.sync || !b.est || !b.ref) return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  b.ref = (double *)zq_scratch(ctx, "sp2_ref", 2 * size_t(n) * n * sizeof(double));
+  b.gpart = (double *)zq_scratch(ctx, "sp2_gpart", 3 * kGershBlocks * sizeof(double));
+  bool fresh = false;  // the Gershgorin arrival count starts at zero and its last workgroup clears it again
+  b.gcount = (unsigned *)zq_scratch(ctx, "sp2_gcount", sizeof(unsigned), &fresh);
+  if (!b.X || !b.part || !b.scal || !b.sync || !b.est || !b.ref || !b.gpart || !b.gcount)
+    return zq_fail(ctx, ZQ_ERR_OOM, "SP2 buffers");
+  if (fresh) ZQ_TRY(ctx, hipMemsetAsync(b.gcount, 0, sizeof(unsigned), ctx->stream));
   return ZQ_OK;
 }
 
 // track: 0 plain SP2 outside the fold bookkeeping, 1 the first run of a device SCF, 2 a later run.
 void sp2_start(zq_ctx *ctx, const Sp2Bufs &b, const double *dF, int n, int nocc, int *state, const int *skip,
                int track) {
-  hipLaunchKernelGGL(k_gershgorin, dim3(1), dim3(1024), 0, ctx->stream, dF, n, b.scal, skip, track, nocc, b.est, b.ref);
+  hipLaunchKernelGGL(k_gershgorin, dim3(kGershBlocks), dim3(256), 0, ctx->stream, dF, n, b.scal, skip, track, nocc, b.est,
+                     b.ref, b.gpart, b.gcount);
   hipLaunchKernelGGL(k_sp2_init, dim3(b.ntiles), dim3(64), 0, ctx->stream, dF, n, b.m, b.scal, b.X, b.part, state, b.sync,
                      skip);
 }

@@ -72,10 +72,37 @@ __device__ inline d4 tile_nt(const double *__restrict__ A, int lda, const double
   return acc;
 }
 
+// Lane exchanges of a double without LDS: a DPP move inside a 16-lane row, and the gfx950 permlane
+// swaps across rows.  A swap of v with itself leaves, in its two results, v and the partner's v (in
+// either order, by lane), so their sum is v + v[lane ^ 16] (or ^ 32) on every lane.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double add_swap16(double v) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(unsigned(__double2loint(v)), unsigned(__double2loint(v)), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(unsigned(__double2hiint(v)), unsigned(__double2hiint(v)), false, false);
+  return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+}
+__device__ inline double add_swap32(double v) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(unsigned(__double2loint(v)), unsigned(__double2loint(v)), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(unsigned(__double2hiint(v)), unsigned(__double2hiint(v)), false, false);
+  return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+}
+
+// Sum over the 64 lanes, the same value on every lane: partners at lane distance 1 and 2 (quad
+// permutes), then the other quad of the half-row (half-row mirror: the quads are uniform by then), the
+// other half of the row (rotate by 8), the other row of the pair (lane ^ 16) and the other half of the
+// wave (lane ^ 32).  Every lane adds the same pairs, so every lane holds the same bits.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x128>(v);
+  v = add_swap16(v);
+  return add_swap32(v);
 }
 
 

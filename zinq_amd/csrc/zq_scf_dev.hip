@@ -333,31 +333,41 @@ __global__ __launch_bounds__(256) void k_dots_state(const double *__restrict__ h
 // does.  Returns false, on every lane, for a zero pivot; otherwise lane i < n holds x[i] in b.
 constexpr int kLuMax = kMaxHist + 1;
 
+// lane src's v on every lane; src is wave-uniform, so this is two v_readlane, not an LDS permute
+__device__ inline double lane_bcast(double v, int src) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), src),
+                          __builtin_amdgcn_readlane(__double2loint(v), src));
+}
+
 __device__ __forceinline__ bool lu_solve_wave(int n, double (&a)[kLuMax], double &b) {
   // every loop has a constant trip count and guards its body, so that it unrolls fully and a[] stays in
-  // registers
+  // registers; every row or value that moves between lanes comes from a wave-uniform lane (readlane)
   const int lane = threadIdx.x & 63;
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < kLuMax; ++k) {
     if (k < n && ok) {
-      double v = (lane >= k && lane < n) ? fabs(a[k]) : -1.0;
-      int p = lane;
+      // pivot: the first row i >= k with the largest |A[i][k]|, scanned in row order as the serial steps do
+      int p = k;
+      double best = fabs(lane_bcast(a[k], k));
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double vo = __shfl_xor(v, o, 64);
-        const int po = __shfl_xor(p, o, 64);
-        if (vo > v || (vo == v && po < p)) v = vo, p = po;
-      }
-      ok = v != 0.0;
+      for (int i = k + 1; i < kLuMax; ++i)
+        if (i < n) {
+          const double v = fabs(lane_bcast(a[k], i));
+          if (v > best) best = v, p = i;
+        }
+      ok = best != 0.0;
       if (ok && p != k) {
-        const int src = lane == k ? p : (lane == p ? k : lane);
 #pragma unroll
         for (int j = 0; j < kLuMax; ++j)
-          if (j < n) a[j] = __shfl(a[j], src, 64);
-        b = __shfl(b, src, 64);
+          if (j < n) {
+            const double rk = lane_bcast(a[j], k), rp = lane_bcast(a[j], p);
+            a[j] = lane == k ? rp : (lane == p ? rk : a[j]);
+          }
+        const double bk = lane_bcast(b, k), bp = lane_bcast(b, p);
+        b = lane == k ? bp : (lane == p ? bk : b);
       }
-      const double akk = __shfl(a[k], k, 64);
+      const double akk = lane_bcast(a[k], k);
       const bool below = ok && lane > k && lane < n;
       double f = 0.0;
       if (below) {
@@ -367,7 +377,7 @@ __device__ __forceinline__ bool lu_solve_wave(int n, double (&a)[kLuMax], double
 #pragma unroll
       for (int j = k + 1; j < kLuMax; ++j) {
         if (j < n) {
-          const double akj = __shfl(a[j], k, 64);
+          const double akj = lane_bcast(a[j], k);
           if (below) a[j] -= f * akj;
         }
       }
@@ -377,7 +387,7 @@ __device__ __forceinline__ bool lu_solve_wave(int n, double (&a)[kLuMax], double
 #pragma unroll
   for (int j = 0; j < kLuMax; ++j) {
     if (j < n) {
-      const double bj = __shfl(b, j, 64);
+      const double bj = lane_bcast(b, j);
       if (lane > j && lane < n) b -= a[j] * bj;
     }
   }
@@ -388,7 +398,7 @@ __device__ __forceinline__ bool lu_solve_wave(int n, double (&a)[kLuMax], double
 #pragma unroll
       for (int j = i + 1; j < kLuMax; ++j) {
         if (j < n) {
-          const double bj = __shfl(b, j, 64);
+          const double bj = lane_bcast(b, j);
           acc -= a[j] * bj;
         }
       }

@@ -304,24 +304,26 @@ __global__ __launch_bounds__(256) void zq_jk_records(const E *__restrict__ eri, 
     out[lane] = 2.0 * (((red[rb][0][0][lane] + red[rb][1][0][lane]) + red[rb][2][0][lane]) + red[rb][3][0][lane]);
 }
 
-// F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).  Four waves
-// take contiguous quarters of the block's record list with 8 loads in flight; the quarters are added in
-// a fixed order, so the result is bitwise reproducible.
+// F block (x >= y) = H + sum of records aimed at (x,y) + transposed records aimed at (y,x).  kRedWaves
+// waves take contiguous shares of the block's record list with 8 loads in flight each; the shares are
+// added in wave order, so the result is bitwise reproducible.
 // GROUP: blk[blockIdx.x] = (molecule, its output block); H, F, n and the leading dimension come from
 // mols[].  Otherwise H and F are n x n with leading dimension ldf and blockIdx.x is the output block.
 // MULTI: blockIdx.y is the density (records + d * rstride, F + d * fstride); H may be null (base 0).
 // epart (optional, plain form): epart[block] = this block's share of 1/2 sum P (Hc + F), the electronic
 // energy of getEnergy (src/hartree_fock.zig:643-655), from the padded density Pe (ld pld) and the core
 // Hamiltonian Hc (ld ldf); blocks off the diagonal count twice, as their mirrors are not visited.
+constexpr int kRedWaves = 16;
+
 template <bool GROUP, bool MULTI = false>
-__global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
+__global__ __launch_bounds__(64 * kRedWaves) void zq_jk_reduce(const double *__restrict__ rec, const int *__restrict__ csr_ptr,
                                                     const int *__restrict__ csr_idx, const double *H, int n, double *F,
                                                     const int *__restrict__ skip, int ldf, const JKMol *__restrict__ mols,
                                                     const int2 *__restrict__ blk, const int *__restrict__ done,
                                                     long long rstride = 0, long long fstride = 0,
                                                     const double *__restrict__ Pe = nullptr, int pld = 0,
                                                     const double *__restrict__ Hc = nullptr, double *__restrict__ epart = nullptr) {
-  __shared__ double part[4][64];
+  __shared__ double part[kRedWaves][64];
   if (skip && *skip) return;
   if (MULTI) {
     rec += blockIdx.y * rstride;
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ r
   zq_unpair(o, x, y);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i = lane >> 3, j = lane & 7;
   const int e0 = csr_ptr[g], e1 = csr_ptr[g + 1], len = e1 - e0;
-  const int a0 = e0 + (len * wave) / 4, a1 = e0 + (len * (wave + 1)) / 4;
+  const int a0 = e0 + (len * wave) / kRedWaves, a1 = e0 + (len * (wave + 1)) / kRedWaves;
   double acc = 0.0;
   int e = a0;
   for (; e + 8 <= a1; e += 8) {
@@ -366,7 +368,10 @@ __global__ __launch_bounds__(256) void zq_jk_reduce(const double *__restrict__ r
   if (wave != 0) return;
   const int gi = x * 8 + i, gj = y * 8 + j;
   const double h = (H && gi < n && gj < n) ? H[size_t(gi) * ldf + gj] : 0.0;
-  const double f = h + (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
+  double sum = part[0][lane];
+#pragma unroll
+  for (int k = 1; k < kRedWaves; ++k) sum += part[k][lane];
+  const double f = h + sum;
   // on diagonal blocks lane (i,j) and lane (j,i) sum the same terms in different orders: let only
   // i >= j write, so both triangles hold one value
   if (gi < n && gj < n && (x != y || i >= j)) {
@@ -487,7 +492,7 @@ int zq_fock_jk_device(zq_ctx *ctx, const double *dP, const double *dH, double x_
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_records");
   zq_prof_begin(ctx, "jk_reduce");
-  hipLaunchKernelGGL(zq_jk_reduce<false>, dim3(unsigned(ctx->Nb)), dim3(256), 0, ctx->stream, ctx->d_records,
+  hipLaunchKernelGGL(zq_jk_reduce<false>, dim3(unsigned(ctx->Nb)), dim3(64 * kRedWaves), 0, ctx->stream, ctx->d_records,
                      ctx->d_csr_ptr, ctx->d_csr_idx, dH, ctx->n, dF, skip, ctx->n, nullptr, nullptr, nullptr, 0LL, 0LL,
                      ctx->d_P, ctx->npad, ctx->d_H, epart);
   zq_prof_end(ctx);
@@ -500,7 +505,7 @@ hipError_t zq_jk_group(hipStream_t stream, const double *eri, const JKItem *item
                        const int *csr_idx, const int2 *blk, int nblk, const double *H, double *F) {
   hipLaunchKernelGGL((zq_jk_records<double, true>), dim3(unsigned(nitems)), dim3(256), 0, stream, eri, items, nitems, P,
                      0, x_exch, rec, nullptr, mols, done);
-  hipLaunchKernelGGL(zq_jk_reduce<true>, dim3(unsigned(nblk)), dim3(256), 0, stream, rec, csr_ptr, csr_idx, H, 0, F,
+  hipLaunchKernelGGL(zq_jk_reduce<true>, dim3(unsigned(nblk)), dim3(64 * kRedWaves), 0, stream, rec, csr_ptr, csr_idx, H, 0, F,
                      nullptr, 0, mols, blk, done);
   return hipGetLastError();
 }
@@ -554,7 +559,7 @@ extern "C" int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, s
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_jk_records (multi)");
   zq_prof_begin(ctx, "jk_reduce");
-  hipLaunchKernelGGL((zq_jk_reduce<false, true>), dim3(unsigned(ctx->Nb), unsigned(ndens)), dim3(256), 0, ctx->stream, dR,
+  hipLaunchKernelGGL((zq_jk_reduce<false, true>), dim3(unsigned(ctx->Nb), unsigned(ndens)), dim3(64 * kRedWaves), 0, ctx->stream, dR,
                      ctx->d_csr_ptr, ctx->d_csr_idx, H ? ctx->d_H : nullptr, n, dF, nullptr, n, nullptr, nullptr, nullptr,
                      (long long)rstride, (long long)nn);
   zq_prof_end(ctx);

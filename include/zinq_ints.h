@@ -2,8 +2,8 @@
  * zinq_ints.h -- host Gaussian integral engine with the C ABI of the reference's libint shim.
  *
  * The reference reaches libint2 through src/libint.h (declared at src/libint.h:1-30, defined in
- * src/libint.cpp).  libint2 is not available in this build environment, so this library provides the
- * same entry points, computed by an in-house McMurchie-Davidson engine (s..g shells, pure spherical
+ * src/libint.cpp).  This library provides the same entry points without libint2, computed by an
+ * in-house McMurchie-Davidson engine (s..g shells, pure spherical
  * d and higher, contracted shells renormalised).  The dense ERI layout is exactly the one
  * src/libint.cpp:143-150 writes: I[((p*n + q)*n + r)*n + s] = <pq|rs> = (pr|qs).
  *

@@ -698,7 +698,7 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
   // Q1: MT tiles of 16 occupied orbitals per workgroup (at most 3 for FP32C, whose FP32 + FP64
   // accumulator pairs would otherwise spill), ceil(m1 / (16 MT)) workgroups along i
   {
-    const int cap = std::is_same<MF, MF32C>::value ? 3 : 4;
+    const int cap = 4;
     const int mt = std::min(cap, (std::min(m1, 16 * cap) + 15) / 16);
     const int groups = precision == ZQ_PREC_BF16X3 ? (m1 + 63) / 64 : (m1 + 16 * mt - 1) / (16 * mt);
     dim3 grid(unsigned(Nb * nb), unsigned(groups));

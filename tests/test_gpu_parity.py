@@ -373,6 +373,34 @@ def test_serve_same_alone_and_in_the_mix():
         assert r["E_mp2"] == alone[r["key"]]["E_mp2"]
 
 
+def test_serve_burst_reattach_eager():
+    """A burst of 64 requests on 8 slots whose contexts switch molecules between requests (each attach
+    rebuilds the J/K plan and zeroes P), launched eagerly (profiling on: no captured graph), so that a
+    context's first kernels run while other slots keep the device busy.  Every result equals bitwise the
+    molecule's result served alone.  Regression: the plan's P fill once went to the null stream, which
+    is not ordered before the non-blocking context stream; landing after the first density, it made the
+    incremental mixed-path Fock builds add G(P_0) twice, and the SCF converged to a wrong state."""
+    from zinq_amd.serve import Server
+    mols = _serve_molecules()
+    keys = [m.key for m in mols]
+    alone = {}
+    srv = Server(mols, slots=1)
+    for m in mols:
+        rec, _ = srv.serve([(0.0, m.key)])
+        assert "error" not in rec[0], rec[0]
+        alone[m.key] = rec[0]["E_mp2"]
+    srv.close()
+    pick = np.random.default_rng(11).integers(0, len(keys), size=64)
+    srv = Server(mols, slots=8)
+    for c in srv.ctx:
+        c.profile(True)
+    rec, _ = srv.serve([(0.0, keys[int(j)]) for j in pick])
+    srv.close()
+    bad = [(r["key"], r["slot"], r.get("iterations"), r.get("E_mp2", r.get("error"))) for r in rec
+           if "error" in r or r["E_mp2"] != alone[r["key"]]]
+    assert not bad, bad
+
+
 def test_serve_mixed_results_within_bar():
     """Served mixed-path energies against the plain FP64 device path: |dE_MP2| <= 1e-6 Eh."""
     from zinq_amd import hip

@@ -688,7 +688,8 @@ int zq_sp2_density(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, i
     if (h[0] >= 2 || launched >= max_steps) break;
     next_check = std::min(max_steps, launched + 4);
   }
-  ZQ_TRY(ctx, hipMemcpy(&fin, state + 4, sizeof(int), hipMemcpyDeviceToHost));
+  ZQ_TRY(ctx, hipMemcpyAsync(&fin, state + 4, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
   zq_prof_end(ctx);
   *dD = b.X + (fin & 1) * b.mm;
   *ld = b.m;

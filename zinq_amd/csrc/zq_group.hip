@@ -486,10 +486,13 @@ int build_plan(zq_batch *b, int chunk) {
   BTRY(b, hipMalloc(&b->d_ptr, ptr.size() * sizeof(int)));
   BTRY(b, hipMalloc(&b->d_idx, std::max<size_t>(1, idx.size()) * sizeof(int)));
   BTRY(b, hipMalloc(&b->d_blk, blk.size() * sizeof(int2)));
-  BTRY(b, hipMemcpy(b->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice));
-  BTRY(b, hipMemcpy(b->d_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!idx.empty()) BTRY(b, hipMemcpy(b->d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
-  BTRY(b, hipMemcpy(b->d_blk, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice));
+  // the batch stream is non-blocking: uploads go on it, so its kernels see them
+  BTRY(b, hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice, b->stream));
+  BTRY(b, hipMemcpyAsync(b->d_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  if (!idx.empty())
+    BTRY(b, hipMemcpyAsync(b->d_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  BTRY(b, hipMemcpyAsync(b->d_blk, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream));
+  BTRY(b, hipStreamSynchronize(b->stream));  // the plan's host vectors go out of scope
   b->chunk = chunk;
   return ZQ_OK;
 }
@@ -525,17 +528,18 @@ int upload(zq_batch *b) {
   BTRY(b, hipMalloc(&b->d_done, M * sizeof(int)));
   BTRY(b, hipMalloc(&b->d_count, sizeof(int)));
   BTRY(b, hipMalloc(&b->d_jkm, M * sizeof(JKMol)));
-  BTRY(b, hipMemset(b->ws, 0, ws_total * sizeof(double)));
-  BTRY(b, hipMemset(b->d_done, 0, M * sizeof(int)));
-  BTRY(b, hipMemcpy(b->d_mols, bm.data(), M * sizeof(BMol), hipMemcpyHostToDevice));
-  BTRY(b, hipMemcpy(b->d_jkm, jm.data(), M * sizeof(JKMol), hipMemcpyHostToDevice));
+  BTRY(b, hipMemsetAsync(b->ws, 0, ws_total * sizeof(double), b->stream));
+  BTRY(b, hipMemsetAsync(b->d_done, 0, M * sizeof(int), b->stream));
+  BTRY(b, hipMemcpyAsync(b->d_mols, bm.data(), M * sizeof(BMol), hipMemcpyHostToDevice, b->stream));
+  BTRY(b, hipMemcpyAsync(b->d_jkm, jm.data(), M * sizeof(JKMol), hipMemcpyHostToDevice, b->stream));
+  BTRY(b, hipStreamSynchronize(b->stream));
   double *canon = nullptr;
   BTRY(b, hipMalloc(&canon, canon_max * sizeof(double)));
   for (size_t m = 0; m < M; ++m) {
     const HostMol &h = b->mols[m];
     const int nb = nb_of(h.n), Nb = nb * (nb + 1) / 2, mp = mp_of(h.n);
     const size_t nq = size_t(Nb) * (Nb + 1) / 2, mm = size_t(mp) * mp;
-    hipError_t e = hipMemcpy(canon, h.eri.data(), h.eri.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(canon, h.eri.data(), h.eri.size() * sizeof(double), hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(zq_pack_quartets, dim3(unsigned(nq)), dim3(256), 0, b->stream, canon, b->eri + b->eri_off[m], h.n,
                          nq);
@@ -555,10 +559,11 @@ int upload(zq_batch *b) {
         stage[3 * mm + size_t(j) * mp + i] = hX[size_t(i) * h.n + j];
       }
     double *w = b->ws + b->ws_off[m];
-    if (e == hipSuccess) e = hipMemcpy(w + M_H * mm, stage.data(), mm * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(w + M_S * mm, stage.data() + mm, mm * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(w + M_X * mm, stage.data() + 2 * mm, mm * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(w + M_XT * mm, stage.data() + 3 * mm, mm * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpyAsync(w + M_H * mm, stage.data(), mm * 8, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w + M_S * mm, stage.data() + mm, mm * 8, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w + M_X * mm, stage.data() + 2 * mm, mm * 8, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w + M_XT * mm, stage.data() + 3 * mm, mm * 8, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);  // stage goes out of scope
     if (e != hipSuccess) {
       (void)hipFree(canon);
       return bfail(b, ZQ_ERR_LAUNCH, std::string("batch upload: ") + hipGetErrorString(e));
@@ -681,7 +686,7 @@ int zq_batch_autotune(zq_batch *b, int chosen[2]) {
   };
   int best_chunk = 1;
   float best = 1e30f;
-  BTRY(b, hipMemset(b->d_done, 0, b->mols.size() * sizeof(int)));
+  BTRY(b, hipMemsetAsync(b->d_done, 0, b->mols.size() * sizeof(int), b->stream));
   for (int ch : {1, 2, 4, 8, 16}) {
     if (int rc = build_plan(b, ch)) return rc;
     const float ms = time_it([&] {
@@ -755,7 +760,8 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   std::vector<BState> hs(M);
-  BTRY(b, hipMemcpy(hs.data(), b->d_state, M * sizeof(BState), hipMemcpyDeviceToHost));
+  BTRY(b, hipMemcpyAsync(hs.data(), b->d_state, M * sizeof(BState), hipMemcpyDeviceToHost, b->stream));
+  BTRY(b, hipStreamSynchronize(b->stream));
 
   // eigenpairs of X^T F_ext X: molecules ordered by n so that each size is one strided batch
   std::vector<int> order(M);
@@ -784,7 +790,7 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
       for (void *x : p) (void)hipFree(x);
     }
   } guard{{d_eig, d_c, d_w, d_e, d_eoff, d_info}};
-  BTRY(b, hipMemcpy(d_eoff, eoff.data(), M * sizeof(long long), hipMemcpyHostToDevice));
+  BTRY(b, hipMemcpyAsync(d_eoff, eoff.data(), M * sizeof(long long), hipMemcpyHostToDevice, b->stream));
   if (b->nt == 512)
     hipLaunchKernelGGL(k_batch_fprime<512>, dim3(M), dim3(512), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff);
   else

@@ -464,10 +464,15 @@ int zq_build_jk_plan(zq_ctx *ctx) {
   ZQ_TRY(ctx, hipMalloc(&ctx->d_P, npad2 * sizeof(double)));
   ZQ_TRY(ctx, hipMalloc(&ctx->d_H, size_t(ctx->n) * ctx->n * sizeof(double)));
   ZQ_TRY(ctx, hipMalloc(&ctx->d_F, size_t(ctx->n) * ctx->n * sizeof(double)));
-  ZQ_TRY(ctx, hipMemcpy(ctx->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice));
-  ZQ_TRY(ctx, hipMemcpy(ctx->d_csr_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!idx.empty()) ZQ_TRY(ctx, hipMemcpy(ctx->d_csr_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
-  ZQ_TRY(ctx, hipMemset(ctx->d_P, 0, npad2 * sizeof(double)));
+  // on the context's stream: it is non-blocking, so null-stream copies and fills are not ordered before
+  // its kernels, and a fill of P that lands after the first density corrupts the incremental builds
+  hipStream_t st = ctx->stream;
+  ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice, st));
+  ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_csr_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  if (!idx.empty())
+    ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_csr_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  ZQ_TRY(ctx, hipMemsetAsync(ctx->d_P, 0, npad2 * sizeof(double), st));
+  ZQ_TRY(ctx, hipStreamSynchronize(st));  // the plan's host vectors go out of scope
   return ZQ_OK;
 }
 

@@ -5,6 +5,7 @@
 // re-laid into 8x8 AO-block quartets (see zq_internal.h) on the device and stay resident in HBM.
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <thread>
 
@@ -13,6 +14,7 @@
 #include "zq_internal.h"
 
 int zq_build_jk_plan(zq_ctx *ctx);  // zq_jk.hip
+int zq_jk_buffers(zq_ctx *ctx);     // zq_jk.hip
 
 int zq_fail(zq_ctx *ctx, int code, const std::string &msg) {
   if (ctx) ctx->err = msg;
@@ -123,13 +125,14 @@ static void prof_collect(zq_ctx *ctx) {
 void zq_free_eri(zq_ctx *ctx) {
   if (ctx->eri && ctx->eri_owned) (void)hipFree(ctx->eri);
   if (ctx->eri32 && ctx->eri32_owned) (void)hipFree(ctx->eri32);
-  ctx->eri_owned = ctx->eri32_owned = true;
-  for (void *p : {(void *)ctx->d_items, (void *)ctx->d_records, (void *)ctx->d_csr_ptr, (void *)ctx->d_csr_idx,
-                  (void *)ctx->d_P, (void *)ctx->d_H, (void *)ctx->d_F})
-    if (p) (void)hipFree(p);
-  ctx->eri = nullptr, ctx->eri32 = nullptr, ctx->d_items = nullptr, ctx->d_records = nullptr;
+  if (ctx->plan_owned)
+    for (void *p : {(void *)ctx->d_items, (void *)ctx->d_csr_ptr, (void *)ctx->d_csr_idx})
+      if (p) (void)hipFree(p);
+  ctx->eri_owned = ctx->eri32_owned = ctx->plan_owned = true;
+  ctx->set_id = 0;
+  // the records, P, H and F buffers stay (grow-only); zq_destroy frees them
+  ctx->eri = nullptr, ctx->eri32 = nullptr, ctx->d_items = nullptr;
   ctx->d_csr_ptr = nullptr, ctx->d_csr_idx = nullptr;
-  ctx->d_P = ctx->d_H = ctx->d_F = nullptr;
   ctx->n = ctx->npad = ctx->nb = ctx->Nb = 0;
   ctx->nquartets = 0;
   ctx->nitems = 0;
@@ -226,6 +229,8 @@ void zq_destroy(zq_ctx *ctx) {
   prof_collect(ctx);
   zq_release_solver(ctx);
   zq_free_eri(ctx);
+  for (double *p : {ctx->d_records, ctx->d_P, ctx->d_H, ctx->d_F})
+    if (p) (void)hipFree(p);
   for (auto &kv : ctx->scratch)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto &kv : ctx->pinned)
@@ -303,7 +308,9 @@ zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int 
   if (!g_packed || n == 0 || n > 4096 || hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
     return nullptr;
   if (hipSetDevice(device) != hipSuccess) return nullptr;
+  static std::atomic<unsigned long long> next_id{1};
   auto *s = new zq_eri_set;
+  s->id = next_id.fetch_add(1);
   s->device = device;
   s->n = int(n);
   s->nb = int((n + ZQ_BLK - 1) / ZQ_BLK);
@@ -328,6 +335,8 @@ zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int 
       ok = hipGetLastError() == hipSuccess;
     }
   }
+  // the J/K plan of this size, once for every context that attaches
+  if (ok) ok = zq_jk_plan_upload(s->Nb, s->nquartets, st, s->plan, s->err) == ZQ_OK;
   if (st) ok = (hipStreamSynchronize(st) == hipSuccess) && ok;
   if (d_canon) (void)hipFree(d_canon);
   if (st) (void)hipStreamDestroy(st);
@@ -352,6 +361,7 @@ void zq_eri_set_destroy(zq_eri_set *s) {
   (void)hipSetDevice(s->device);
   if (s->eri) (void)hipFree(s->eri);
   if (s->eri32) (void)hipFree(s->eri32);
+  zq_jk_plan_free(s->plan);
   delete s;
 }
 
@@ -366,19 +376,26 @@ int zq_attach(zq_ctx *ctx, zq_eri_set *s) {
   if (s->device != ctx->device) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_attach: set and context on different devices");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (ctx->eri == s->eri && ctx->n == s->n) return ZQ_OK;  // already attached: plan and buffers stay
+  if (ctx->set_id == s->id) return ZQ_OK;  // already attached: plan and buffers stay
   zq_free_eri(ctx);
   ++ctx->eri_gen;
   ctx->n = s->n, ctx->nb = s->nb, ctx->npad = s->npad, ctx->Nb = s->Nb, ctx->nquartets = s->nquartets;
   ctx->eri = s->eri, ctx->eri_owned = false;
   ctx->eri32 = s->eri32, ctx->eri32_owned = s->eri32 == nullptr;
-  return zq_build_jk_plan(ctx);
+  ctx->d_items = s->plan.items, ctx->d_csr_ptr = s->plan.ptr, ctx->d_csr_idx = s->plan.idx;
+  ctx->nitems = s->plan.nitems, ctx->nrecords = s->plan.nrecords, ctx->plan_owned = false;
+  if (int rc = zq_jk_buffers(ctx)) {
+    zq_free_eri(ctx);
+    return rc;
+  }
+  ctx->set_id = s->id;
+  return ZQ_OK;
 }
 
 size_t zq_device_bytes(const zq_ctx *ctx) {
   if (!ctx) return 0;
   size_t b = ctx->eri_owned ? ctx->nquartets * ZQ_QUARTET * sizeof(double) : 0;
-  b += ctx->nrecords * 64 * sizeof(double);
+  b += (ctx->cap_records + ctx->cap_P + ctx->cap_H + ctx->cap_F) * sizeof(double);
   if (ctx->eri32 && ctx->eri32_owned) b += ctx->nquartets * ZQ_QUARTET * sizeof(float);
   for (auto &kv : ctx->scratch) b += kv.second.bytes;
   return b;

@@ -66,6 +66,18 @@ struct JKPlan {
 };
 // chunk = quartets per item (<= 0: the default for this size)
 void zq_jk_plan(int Nb, size_t nquartets, int chunk, JKPlan &plan);  // zq_jk.hip
+// The same plan on the device: it depends on the molecule's size only, so a resident ERI set keeps one
+// that every context attached to it reads.
+struct JKPlanDev {
+  JKItem *items = nullptr;
+  int *ptr = nullptr, *idx = nullptr;
+  int nitems = 0;
+  size_t nrecords = 0;
+};
+// Builds the default plan for Nb block pairs and uploads it on st (synchronised before returning);
+// ZQ_OK or an error code with its message in err.  zq_jk_plan_free releases the three arrays.
+int zq_jk_plan_upload(int Nb, size_t nquartets, hipStream_t st, JKPlanDev &d, std::string &err);  // zq_jk.hip
+void zq_jk_plan_free(JKPlanDev &d);                                                             // zq_jk.hip
 
 struct DevBuf {
   void *ptr = nullptr;
@@ -84,13 +96,18 @@ struct zq_ctx {
   float *eri32 = nullptr;  // FP32 copy for the mixed-precision Fock builds (zq_eri32), same layout
   bool eri_owned = true, eri32_owned = true;  // false while attached to a shared zq_eri_set
 
-  // J/K work decomposition
+  // J/K work decomposition: the plan (items, CSR) is the context's own after zq_eri_load and the set's
+  // while attached; records, P, H and F are the context's, grow-only (capacities in doubles), so that
+  // re-attaching frees nothing once they fit the largest molecule
   JKItem *d_items = nullptr;
   int nitems = 0;
   size_t nrecords = 0;
   double *d_records = nullptr;
   int *d_csr_ptr = nullptr, *d_csr_idx = nullptr;
+  bool plan_owned = true;
+  unsigned long long set_id = 0;  // id of the attached zq_eri_set, 0 when none
   double *d_P = nullptr, *d_H = nullptr, *d_F = nullptr;
+  size_t cap_records = 0, cap_P = 0, cap_H = 0, cap_F = 0;
 
   // rocBLAS handle used by the device SCF's eigensolver (created on first use)
   void *solver = nullptr;
@@ -127,6 +144,8 @@ struct zq_eri_set {
   size_t nquartets = 0;
   double *eri = nullptr;
   float *eri32 = nullptr;
+  JKPlanDev plan;  // the J/K plan of this size, shared by the attached contexts
+  unsigned long long id = 0;  // unique per process: a new set may reuse a freed set's addresses
   std::string err;
 };
 

@@ -448,32 +448,74 @@ void zq_jk_plan(int Nb, size_t nq, int chunk, JKPlan &plan) {
   plan.nrecords = tgt.size();
 }
 
-int zq_build_jk_plan(zq_ctx *ctx) {
+void zq_jk_plan_free(JKPlanDev &d) {
+  for (void *p : {(void *)d.items, (void *)d.ptr, (void *)d.idx})
+    if (p) (void)hipFree(p);
+  d = JKPlanDev{};
+}
+
+int zq_jk_plan_upload(int Nb, size_t nquartets, hipStream_t st, JKPlanDev &d, std::string &err) {
   JKPlan plan;
-  zq_jk_plan(ctx->Nb, ctx->nquartets, 0, plan);
-  if (plan.nrecords >= (size_t(1) << 30)) return zq_fail(ctx, ZQ_ERR_INVALID, "too many J/K records");
+  zq_jk_plan(Nb, nquartets, 0, plan);
+  if (plan.nrecords >= (size_t(1) << 30)) {
+    err = "too many J/K records";
+    return ZQ_ERR_INVALID;
+  }
   const std::vector<JKItem> &items = plan.items;
   const std::vector<int> &ptr = plan.ptr, &idx = plan.idx;
-  ctx->nitems = int(items.size());
-  ctx->nrecords = plan.nrecords;
-  const size_t npad2 = size_t(ctx->npad) * ctx->npad;
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_items, items.size() * sizeof(JKItem)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_records, plan.nrecords * 64 * sizeof(double)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_csr_ptr, ptr.size() * sizeof(int)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_csr_idx, std::max<size_t>(1, idx.size()) * sizeof(int)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_P, npad2 * sizeof(double)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_H, size_t(ctx->n) * ctx->n * sizeof(double)));
-  ZQ_TRY(ctx, hipMalloc(&ctx->d_F, size_t(ctx->n) * ctx->n * sizeof(double)));
-  // on the context's stream: it is non-blocking, so null-stream copies and fills are not ordered before
-  // its kernels, and a fill of P that lands after the first density corrupts the incremental builds
-  hipStream_t st = ctx->stream;
-  ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice, st));
-  ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_csr_ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  if (!idx.empty())
-    ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_csr_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  ZQ_TRY(ctx, hipMemsetAsync(ctx->d_P, 0, npad2 * sizeof(double), st));
-  ZQ_TRY(ctx, hipStreamSynchronize(st));  // the plan's host vectors go out of scope
+  JKPlanDev out;
+  out.nitems = int(items.size());
+  out.nrecords = plan.nrecords;
+  // uploads on st, never the null stream: st is non-blocking, and null-stream work is not ordered
+  // before its kernels
+  hipError_t e = hipMalloc(&out.items, items.size() * sizeof(JKItem));
+  if (e == hipSuccess) e = hipMalloc(&out.ptr, ptr.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&out.idx, std::max<size_t>(1, idx.size()) * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpyAsync(out.items, items.data(), items.size() * sizeof(JKItem), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out.ptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !idx.empty())
+    e = hipMemcpyAsync(out.idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // the plan's host vectors go out of scope
+  if (e != hipSuccess) {
+    err = std::string("J/K plan upload: ") + hipGetErrorString(e);
+    zq_jk_plan_free(out);
+    return e == hipErrorOutOfMemory ? ZQ_ERR_OOM : ZQ_ERR_LAUNCH;
+  }
+  d = out;
   return ZQ_OK;
+}
+
+// The context's records, P (npad x npad), H and F (n x n) for its current size.  Each grows when too
+// small and is otherwise kept: a hipFree waits for every stream on the device, which under concurrent
+// requests would stall an attach behind the other slots' work.  P is zeroed on the context's stream.
+int zq_jk_buffers(zq_ctx *ctx) {
+  const size_t npad2 = size_t(ctx->npad) * ctx->npad, nn = size_t(ctx->n) * ctx->n;
+  struct Want {
+    double **p;
+    size_t *cap, need;
+  };
+  for (const Want &w : {Want{&ctx->d_records, &ctx->cap_records, ctx->nrecords * 64}, Want{&ctx->d_P, &ctx->cap_P, npad2},
+                        Want{&ctx->d_H, &ctx->cap_H, nn}, Want{&ctx->d_F, &ctx->cap_F, nn}}) {
+    if (*w.cap >= w.need && *w.p) continue;
+    if (*w.p) (void)hipFree(*w.p);
+    *w.p = nullptr;
+    *w.cap = 0;
+    ZQ_TRY(ctx, hipMalloc(w.p, std::max<size_t>(1, w.need) * sizeof(double)));
+    *w.cap = w.need;
+  }
+  ZQ_TRY(ctx, hipMemsetAsync(ctx->d_P, 0, npad2 * sizeof(double), ctx->stream));
+  return ZQ_OK;
+}
+
+// zq_eri_load: the context's own plan for its ERIs, then its buffers.
+int zq_build_jk_plan(zq_ctx *ctx) {
+  JKPlanDev d;
+  std::string err;
+  if (int rc = zq_jk_plan_upload(ctx->Nb, ctx->nquartets, ctx->stream, d, err)) return zq_fail(ctx, rc, err);
+  ctx->d_items = d.items, ctx->d_csr_ptr = d.ptr, ctx->d_csr_idx = d.idx;
+  ctx->nitems = d.nitems, ctx->nrecords = d.nrecords;
+  ctx->plan_owned = true;
+  return zq_jk_buffers(ctx);
 }
 
 // Device-resident J/K: P (npad x npad, zero padded) and H (n x n) already on the device; F (n x n) out.

@@ -112,10 +112,12 @@ struct zq_ctx {
   // rocBLAS handle used by the device SCF's eigensolver (created on first use)
   void *solver = nullptr;
 
-  // captured SCF iteration (hipGraphExec_t) and the key it was captured for; the generations change
-  // whenever a scratch buffer or the ERI residency is (re)allocated, which invalidates the capture
-  void *scf_graph = nullptr;
-  std::string scf_graph_key;
+  // captured SCF iterations (hipGraphExec_t) by the key they were captured for: sizes, options, and
+  // the residency (the attached set's id, else the ERI load's generation), so that a context serving
+  // several molecules keeps one per molecule.  scratch_gen changes whenever a buffer a capture holds
+  // is re-allocated; the cache is emptied then (scf_graphs_gen records the generation it holds).
+  std::map<std::string, void *> scf_graphs;
+  unsigned scf_graphs_gen = 0;
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
   int scf_driver = 0;    // driver that finished the last device SCF: 1 device-decided, 2 host-synchronised

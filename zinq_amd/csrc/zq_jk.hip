@@ -500,6 +500,7 @@ int zq_jk_buffers(zq_ctx *ctx) {
     if (*w.p) (void)hipFree(*w.p);
     *w.p = nullptr;
     *w.cap = 0;
+    ++ctx->scratch_gen;  // captured SCF iterations hold the old address
     ZQ_TRY(ctx, hipMalloc(w.p, std::max<size_t>(1, w.need) * sizeof(double)));
     *w.cap = w.need;
   }
@@ -575,6 +576,7 @@ int zq_eri32(zq_ctx *ctx) {
   const size_t cnt = ctx->nquartets * ZQ_QUARTET;
   ZQ_TRY(ctx, hipMalloc(&ctx->eri32, cnt * sizeof(float)));
   ctx->eri32_owned = true;
+  ++ctx->scratch_gen;  // a new address for captured SCF iterations (the residency key stays)
   zq_narrow_eri(ctx->stream, ctx->eri, cnt, ctx->eri32);
   ZQ_LAUNCH_CHECK(ctx, "zq_narrow");
   return ZQ_OK;

@@ -585,6 +585,13 @@ int density_async(Dev &d, const double *A, int nocc, double factor, int nsteps, 
   return ZQ_OK;
 }
 
+// Destroys the context's captured iterations.
+void drop_graphs(zq_ctx *ctx) {
+  for (auto &kv : ctx->scf_graphs)
+    if (kv.second) (void)hipGraphExecDestroy((hipGraphExec_t)kv.second);
+  ctx->scf_graphs.clear();
+}
+
 struct Run {
   int n, nocc, np;
   double x_exch, factor, VN;
@@ -680,12 +687,16 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   // one iteration as a hipGraph, captured once per context and key, replayed with one launch
   const bool use_graph = !ctx->profile;
   char key[256];
-  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%u/%u/%p/%p/%d", n, nocc, budget, r.opt->diis,
-           r.opt->iterations, r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->scratch_gen, ctx->eri_gen, (void *)st,
-           (void *)ddone, int(r.mixed));
-  if (use_graph && (!ctx->scf_graph || ctx->scf_graph_key != key)) {
-    if (ctx->scf_graph) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
-    ctx->scf_graph = nullptr;
+  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%c%llu/%p/%p/%d", n, nocc, budget, r.opt->diis,
+           r.opt->iterations, r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->set_id ? 's' : 'g',
+           ctx->set_id ? ctx->set_id : (unsigned long long)ctx->eri_gen, (void *)st, (void *)ddone, int(r.mixed));
+  if (ctx->scf_graphs_gen != ctx->scratch_gen || ctx->scf_graphs.size() >= 64) {
+    drop_graphs(ctx);
+    ctx->scf_graphs_gen = ctx->scratch_gen;
+  }
+  auto it = ctx->scf_graphs.find(key);
+  hipGraphExec_t graph_exec = it == ctx->scf_graphs.end() ? nullptr : (hipGraphExec_t)it->second;
+  if (use_graph && !graph_exec) {
     ZqCaptureWindow window;  // no allocation or synchronous call in any thread while capturing
     ZQ_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_iteration(d, r, budget, st, part_rms, ddone);
@@ -700,15 +711,15 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
     const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e2 != hipSuccess) return zq_check(ctx, e2, "hipGraphInstantiate");
-    ctx->scf_graph = exec;
-    ctx->scf_graph_key = key;
+    ctx->scf_graphs[key] = exec;
+    graph_exec = exec;
   }
   // the stream is idle here (the previous run ended with a synchronisation), so no kernel writes it now
   *(volatile int *)hdone = 0;
   int k = 0;
   for (; k < r.opt->iterations; ++k) {
     if (use_graph) {
-      ZQ_TRY(ctx, hipGraphLaunch((hipGraphExec_t)ctx->scf_graph, ctx->stream));
+      ZQ_TRY(ctx, hipGraphLaunch(graph_exec, ctx->stream));
     } else if (int rc = enqueue_iteration(d, r, budget, st, part_rms, ddone)) {
       return rc;
     }
@@ -829,10 +840,7 @@ void zq_release_solver(zq_ctx *ctx) {
     rocblas_destroy_handle((rocblas_handle)ctx->solver);
     ctx->solver = nullptr;
   }
-  if (ctx->scf_graph) {
-    (void)hipGraphExecDestroy((hipGraphExec_t)ctx->scf_graph);
-    ctx->scf_graph = nullptr;
-  }
+  drop_graphs(ctx);
   for (hipEvent_t &e : ctx->scf_events)
     if (e) (void)hipEventDestroy(e), e = nullptr;
 }

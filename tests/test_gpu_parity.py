@@ -401,6 +401,33 @@ def test_serve_burst_reattach_eager():
     assert not bad, bad
 
 
+def test_attach_switch_back_replays_cached_iteration():
+    """One context attached to set A, then B, then A again (plain and mixed Fock): the SCF iteration
+    captured for A is kept across the switch and replayed, and A's results are bitwise those of its
+    first run; B's equal those of a fresh context."""
+    from zinq_amd import hip
+    mols = _serve_molecules()
+    a, b = mols[1], mols[2]
+    sa, sb = hip.EriSet(a.packed, a.n, 0, True), hip.EriSet(b.packed, b.n, 0, True)
+    try:
+        for fock in ("fp64", "fp32"):
+            def run(c, m, s):
+                c.attach(s)
+                r = c.scf(m.H, m.S, m.VN, m.o, diis=8, threshold=1e-8, device=True, fock_precision=fock)
+                assert c.last_scf_driver() == "device"
+                return r["E"], r["iterations"], r["C"]
+            with hip.Context(0) as c, hip.Context(0) as fresh:
+                first = run(c, a, sa)
+                mid = run(c, b, sb)
+                again = run(c, a, sa)
+                ref_b = run(fresh, b, sb)
+            assert first[:2] == again[:2] and np.array_equal(first[2], again[2]), (fock, first[:2], again[:2])
+            assert mid[:2] == ref_b[:2], (fock, mid[:2], ref_b[:2])
+    finally:
+        sa.close()
+        sb.close()
+
+
 def test_serve_mixed_results_within_bar():
     """Served mixed-path energies against the plain FP64 device path: |dE_MP2| <= 1e-6 Eh."""
     from zinq_amd import hip

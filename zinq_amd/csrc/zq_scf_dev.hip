@@ -536,7 +536,11 @@ __global__ void k_sp2_check(int parity, ScfState *__restrict__ st) {
 __global__ __launch_bounds__(64) void k_converge(const double *__restrict__ part, int n, double thr, int maxit,
                                                  ScfState *__restrict__ st, int parity, int *__restrict__ host_done,
                                                  int diis) {
-  if (st->done) return;
+  if (st->done) {
+    // set before this iteration (the guess density's SP2 check): the host stops replaying as well
+    if (threadIdx.x == 0) *host_done = st->done;
+    return;
+  }
   const double s = zqt::wave_sum(part[threadIdx.x]);
   if (threadIdx.x != 0) return;
   const int *fin = st->sp2 + 2 * parity;

@@ -98,10 +98,13 @@ int zq_profile_get(zq_ctx *ctx, const char *prefix, double *ms, int *launches);
 size_t zq_device_bytes(const zq_ctx *ctx);
 
 /* Resident ERI sets for serving: the quartets of one molecule (and, with with_fp32, their FP32 copy for
- * the mixed Fock path) stay in HBM independently of any context, and several contexts -- one per
- * concurrent request, each with its own stream, J/K plan and scratch -- compute on them at once.
+ * the mixed Fock path) stay in HBM independently of any context, together with the molecule's J/K
+ * work plan, and several contexts -- one per concurrent request, each with its own stream, J/K
+ * records and scratch -- compute on them at once.  A context's buffers only grow, so re-attaching
+ * allocates nothing once they fit the largest molecule.
  * zq_eri_set_create takes the zq_eri_load_packed layout; NULL on failure.  zq_eri_set_bytes is the
- * set's HBM footprint.  zq_attach points ctx at the set (no copy; the set must outlive the attachment,
+ * set's HBM footprint (the quartets and their FP32 copy; the plan adds under 1%).  zq_attach points
+ * ctx at the set (no copy; the set must outlive the attachment,
  * which ends with the next zq_attach, zq_eri_load* or zq_destroy of ctx). */
 typedef struct zq_eri_set zq_eri_set;
 zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int with_fp32);

@@ -375,11 +375,12 @@ def test_serve_same_alone_and_in_the_mix():
 
 def test_serve_burst_reattach_eager():
     """A burst of 64 requests on 8 slots whose contexts switch molecules between requests (each attach
-    rebuilds the J/K plan and zeroes P), launched eagerly (profiling on: no captured graph), so that a
-    context's first kernels run while other slots keep the device busy.  Every result equals bitwise the
-    molecule's result served alone.  Regression: the plan's P fill once went to the null stream, which
-    is not ordered before the non-blocking context stream; landing after the first density, it made the
-    incremental mixed-path Fock builds add G(P_0) twice, and the SCF converged to a wrong state."""
+    points the context at the set's J/K plan and zeroes its P), launched eagerly (profiling on: no
+    captured graph), so that a context's first kernels run while other slots keep the device busy.
+    Every result equals bitwise the molecule's result served alone.  Regression: the attach's P fill
+    once went to the null stream, which is not ordered before the non-blocking context stream; landing
+    after the first density, it made the incremental mixed-path Fock builds add G(P_0) twice, and the
+    SCF converged to a wrong state."""
     from zinq_amd.serve import Server
     mols = _serve_molecules()
     keys = [m.key for m in mols]

@@ -1,0 +1,106 @@
+"""Host logic of the serving loop (zinq_amd/serve.py) on the CPU: admission by footprint, LRU eviction of
+idle ERI sets, continuous batching over the slots.  The GPU calls are replaced by a stand-in module that
+records what the server does with them; the GPU tests run the same loop on the device."""
+import threading
+import time
+
+import pytest
+
+from zinq_amd import serve as sv
+
+
+class _Mol:
+    def __init__(self, key, n):
+        self.key, self.n, self.o = key, n, n // 8
+        self.H = self.S = self.packed = None
+        self.VN = 0.0
+
+
+def _fake_hip(budget_box, log):
+    lock = threading.Lock()
+    live = set()
+    running = [0]
+
+    class EriSet:
+        def __init__(self, packed, n, device, with_fp32):
+            self.n, self.closed = n, False
+            with lock:
+                used = sum(sv.eri_set_bytes(s.n, with_fp32) for s in live) + sv.eri_set_bytes(n, with_fp32)
+                if used > budget_box[0]:
+                    log.append(f"over budget: {used} > {budget_box[0]}")
+                live.add(self)
+                log.append(f"create {n}")
+
+        def close(self):
+            with lock:
+                live.discard(self)
+                self.closed = True
+                log.append(f"close {self.n}")
+
+    class Context:
+        def __init__(self, device):
+            self.set = None
+
+        def attach(self, s):
+            if s.closed:
+                raise RuntimeError("attached a closed set")
+            self.set = s
+
+        def scf(self, H, S, VN, nocc, **kw):
+            with lock:
+                running[0] += 1
+                log.append(("running", running[0]))
+            time.sleep(0.003)
+            closed = self.set.closed
+            with lock:
+                running[0] -= 1
+            if closed:
+                raise RuntimeError("set freed while in use")
+            return dict(E=-float(self.set.n), C=None, e=None, iterations=3)
+
+        def mp2(self, C, e, nocc, precision):
+            if self.set.closed:
+                raise RuntimeError("set freed while in use")
+            return -1e-3 * self.set.n
+
+        def close(self):
+            pass
+
+    class Hip:
+        pass
+
+    Hip.EriSet, Hip.Context, Hip.GpuOutOfMemory = EriSet, Context, MemoryError
+    return Hip
+
+
+@pytest.mark.parametrize("slots", [1, 3])
+def test_admission_eviction_and_slots(monkeypatch, slots):
+    mols = [_Mol(f"m{n}", n) for n in (48, 56, 64, 72)]  # sets of 11, 20, 33 and 51 MB
+    reserve = max(sv.scratch_bytes(m.n, m.o, 4) for m in mols)
+    big = max(sv.eri_set_bytes(m.n, True) for m in mols)
+    budget_box, log = [0], []
+    monkeypatch.setattr(sv, "hip", _fake_hip(budget_box, log))
+    hbm = slots * reserve + int(1.3 * big)  # the largest set and a small one at most
+    srv = sv.Server(mols, slots=slots, hbm_budget=hbm)
+    budget_box[0] = srv.budget
+    keys = [m.key for m in mols]
+    rec, wall = srv.serve(sv.poisson_arrivals(400.0, 40, keys, seed=9))
+    srv.close()
+    assert len(rec) == 40 and wall > 0
+    for r in rec:
+        assert "error" not in r, r
+        n = int(r["key"][1:])
+        assert r["E_rhf"] == -float(n) and r["E_corr"] == pytest.approx(-1e-3 * n)  # its own set
+        assert r["latency"] >= r["done"] - r["start"] >= 0
+    assert not [x for x in log if isinstance(x, str) and x.startswith("over budget")]
+    assert max(x[1] for x in log if isinstance(x, tuple)) <= slots
+    creates = sum(1 for x in log if isinstance(x, str) and x.startswith("create"))
+    assert creates > len(mols)  # the budget forced evictions and re-creations
+    assert srv.reserved <= srv.budget
+
+
+def test_budget_below_the_slot_reserve_is_refused(monkeypatch):
+    mols = [_Mol("m72", 72)]
+    monkeypatch.setattr(sv, "hip", _fake_hip([0], []))
+    with pytest.raises(MemoryError):
+        sv.Server(mols, slots=4, hbm_budget=sv.scratch_bytes(72, 9, 4))

@@ -45,8 +45,8 @@ int zq_scf_run(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const double
 /* Same contract as zq_scf_run with every n x n step on the GPU.  Each iteration is decided on the
  * device: J/K, energy, DIIS error, the DIIS history and its bordered LU solve, the extrapolation, and
  * the density P = f X D X^T (X = L^-T from S = L L^T), where D, the projector onto the nocc lowest
- * eigenvectors of X^T F X, comes from SP2 purification (GEMMs on MFMA).  The iteration is captured once
- * as a hipGraph and replayed; the host reads the convergence state one iteration late, so the GPU is
+ * eigenvectors of X^T F X, comes from SP2 purification (GEMMs on MFMA).  The iteration is captured as a
+ * hipGraph (one per molecule and options, kept by ctx) and replayed; the host reads the convergence state one iteration late, so the GPU is
  * never idle between iterations.  After convergence one zq_eigh decomposition of the last
  * extrapolated F (in the orthonormal basis) gives C and e.  A density whose SP2 does not settle within the step budget (no HOMO-LUMO gap) makes
  * the run restart on a host-synchronised driver that falls back to dsyevd for that density. */

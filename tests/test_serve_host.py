@@ -99,6 +99,21 @@ def test_admission_eviction_and_slots(monkeypatch, slots):
     assert srv.reserved <= srv.budget
 
 
+def test_molecule_above_the_budget_fails_instead_of_waiting(monkeypatch):
+    """A set larger than the whole budget can never be admitted: its requests fail at once with an
+    out-of-memory record, and the others are served."""
+    mols = [_Mol("m24", 24), _Mol("m72", 72)]
+    reserve = max(sv.scratch_bytes(m.n, m.o, 4) for m in mols)
+    budget_box, log = [0], []
+    monkeypatch.setattr(sv, "hip", _fake_hip(budget_box, log))
+    srv = sv.Server(mols, slots=2, hbm_budget=2 * reserve + 2 * sv.eri_set_bytes(24, True))
+    budget_box[0] = srv.budget
+    rec, _ = srv.serve([(0.0, "m72"), (0.001, "m24"), (0.002, "m72")])
+    srv.close()
+    assert [("error" in r) for r in rec] == [True, False, True]
+    assert "budget" in rec[0]["error"] and rec[1]["E_rhf"] == -24.0
+
+
 def test_budget_below_the_slot_reserve_is_refused(monkeypatch):
     mols = [_Mol("m72", 72)]
     monkeypatch.setattr(sv, "hip", _fake_hip([0], []))

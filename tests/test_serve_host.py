@@ -114,6 +114,28 @@ def test_molecule_above_the_budget_fails_instead_of_waiting(monkeypatch):
     assert "budget" in rec[0]["error"] and rec[1]["E_rhf"] == -24.0
 
 
+def test_unknown_molecule_and_failed_set_are_recorded(monkeypatch):
+    """A request naming no known molecule, and one whose ERI set cannot be created, each get an error
+    record; the loop keeps serving the rest."""
+    mols = [_Mol("m24", 24), _Mol("m32", 32)]
+    hipmod = _fake_hip([10 ** 15], [])
+    make = hipmod.EriSet
+
+    class FailingSet(make):
+        def __init__(self, packed, n, device, with_fp32):
+            if n == 32:
+                raise MemoryError("set allocation failed")
+            super().__init__(packed, n, device, with_fp32)
+
+    hipmod.EriSet = FailingSet
+    monkeypatch.setattr(sv, "hip", hipmod)
+    srv = sv.Server(mols, slots=2, hbm_budget=10 ** 12)
+    rec, _ = srv.serve([(0.0, "nope"), (0.001, "m32"), (0.002, "m24")])
+    srv.close()
+    assert "unknown molecule" in rec[0]["error"] and "allocation failed" in rec[1]["error"]
+    assert "error" not in rec[2] and rec[2]["E_rhf"] == -24.0
+
+
 def test_budget_below_the_slot_reserve_is_refused(monkeypatch):
     mols = [_Mol("m72", 72)]
     monkeypatch.setattr(sv, "hip", _fake_hip([0], []))

@@ -139,13 +139,22 @@ class Server:
                     self.lock.wait(timeout=wait)
                     continue
                 key = requests[i][1]
-                if key not in self.resident and eri_set_bytes(self.mols[key].n, self.with_fp32) > self.budget:
-                    # can never be admitted, even with every other set evicted: fail it now
-                    out[i] = dict(error=repr(hip.GpuOutOfMemory(f"{key}: ERI set above the HBM budget")), key=key,
-                                  arrival=requests[i][0], start=now, done=now, latency=now - requests[i][0], slot=None)
+                fail = None
+                if key not in self.mols:
+                    fail = KeyError(f"unknown molecule {key!r}")
+                elif key not in self.resident and eri_set_bytes(self.mols[key].n, self.with_fp32) > self.budget:
+                    # can never be admitted, even with every other set evicted
+                    fail = hip.GpuOutOfMemory(f"{key}: ERI set above the HBM budget")
+                else:
+                    try:
+                        slot = self._try_admit(key)
+                    except Exception as e:  # the set could not be created: recorded, never silent
+                        fail = e
+                if fail is not None:
+                    out[i] = dict(error=repr(fail), key=key, arrival=requests[i][0], start=now, done=now,
+                                  latency=now - requests[i][0], slot=None)
                     pending.popleft()
                     continue
-                slot = self._try_admit(key)
                 if slot is None:
                     self.lock.wait(timeout=0.05)
                     continue

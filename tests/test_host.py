@@ -176,3 +176,13 @@ def test_matrix_text_format_round_trip(tmp_path):
     r = tmp_path / "b.mat"
     r.write_text("2 2\n1.5 -2.25\n3.0\n4.0\n")
     np.testing.assert_array_equal(matio.read_matrix(r), [[1.5, -2.25], [3.0, 4.0]])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; a request for several exits with a message and a non-zero status,
+    before touching the GPU."""
+    import subprocess
+    import sys
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 2 and "one GPU" in p.stderr and p.stdout == ""

@@ -167,7 +167,8 @@ def sweep(k, basis):
     ctx.close()
     return dict(sweep=f"(H2O)_{k} {basis.split(':')[-1]}", n=n, o=o, E_rhf=r["E"], E_mp2=r["E"] + ref_e2,
                 rows=rows, mixed_path=mixed,
-                reference="FP64 GPU form (MP2 transform + FP64 sums), matched to the CPU oracle at n <= 96")
+                reference="FP64 GPU form (MP2 transform + FP64 sums); tests/test_gpu_sizes.py matches it to the CPU "
+                          "oracle on (H2O)_4 whole and on sampled rows, blocks and energies at n = 144..240")
 
 
 def batch(count, steps, warmup):
@@ -245,9 +246,10 @@ def serve(lams, slots, kmax):
                 max_abs_dE_mp2_vs_fp64=worst, resident_sets_gb=round(resident_gb, 2), integrals_s=round(t_int, 1))
 
 
-def kernel_only(which, k, reps, basis):
+def kernel_only(which, k, reps, basis, precision="fp64"):
     """Single-kernel runs for hardware-counter passes: `jk` repeats the FP64 Fock build of (H2O)_k on a
-    fixed density; `copy` copies a device buffer of 1 GiB (reads 1 GiB, writes 1 GiB), the calibration
+    fixed density; `mp2` repeats the MP2 energy (OVOV transform + sum) of (H2O)_k from converged orbitals
+    in the given transform precision; `copy` copies a device buffer of 1 GiB (reads 1 GiB, writes 1 GiB), the calibration
     for FETCH_SIZE / WRITE_SIZE."""
     if which == "copy":
         import torch
@@ -263,6 +265,14 @@ def kernel_only(which, k, reps, basis):
     ctx = hip.Context(0)
     ctx.eri_load_packed(s.coulomb_packed(), s.nbf)
     H = s.hcore()
+    if which == "mp2":
+        o = s.nel // 2
+        r = ctx.scf(H, s.overlap(), s.nrep(), o, threshold=1e-10, device=True)
+        for _ in range(reps):
+            ctx.mp2(r["C"], r["e"], o, precision)
+        ctx.close()
+        return dict(kernel="zq_q1/zq_q2/zq_gemm", n=s.nbf, o=o, precision=precision,
+                    algorithmic_flops=ovov_flops(s.nbf, o), reps=reps)
     P = np.eye(s.nbf) * (s.nel / s.nbf)
     for _ in range(reps):
         ctx.fock_jk(H, P)
@@ -287,7 +297,7 @@ def main():
     ap.add_argument("--serve", default="", metavar="L1,L2,..", help="serving workload at these arrival rates instead")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--kmax", type=int, default=10)
-    ap.add_argument("--kernel", default="", choices=["", "jk", "copy"], help="single-kernel run for counter passes")
+    ap.add_argument("--kernel", default="", choices=["", "jk", "mp2", "copy"], help="single-kernel run for counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: only the one-GPU path is built; a multi-GPU run is not supported.", file=sys.stderr)
@@ -299,7 +309,8 @@ def main():
         print(json.dumps(batch(args.batch, args.steps, args.warmup)), flush=True)
         return
     if args.kernel:
-        print(json.dumps(kernel_only(args.kernel, int(args.ks.split(",")[-1]), args.steps, args.basis)), flush=True)
+        print(json.dumps(kernel_only(args.kernel, int(args.ks.split(",")[-1]), args.steps, args.basis, args.precision)),
+              flush=True)
         return
     if args.serve:
         print(json.dumps(serve([float(x) for x in args.serve.split(",")], args.slots, args.kmax)), flush=True)

@@ -62,6 +62,11 @@ int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
  * on ctx -- 1 the device-decided iteration, 2 the host-synchronised restart -- or 0 before any run. */
 int zq_scf_last_driver(const zq_ctx *ctx);
 
+/* Diagnostics, with no counterpart in the reference: how many SCF iterations zq_scf_run_device has
+ * captured as hipGraphs on ctx since zq_create.  A run that replays a cached capture (same molecule and
+ * options, no buffer of ctx re-allocated since) leaves it unchanged. */
+int zq_scf_captures(const zq_ctx *ctx);
+
 /* Diagnostics, with no counterpart in the reference: the SP2 densities of the last zq_scf_run_device on
  * ctx that the device-decided iteration finished (0s otherwise): how many, how many of them took
  * certified scale-and-fold steps (HOMO/LUMO bounds from an earlier plain density of the same SCF,

@@ -243,6 +243,36 @@ void orc_get_fock(int n, const double *g, const double *H, const double *P, int 
     }
 }
 
+/* getFock, DFT branch (src/hartree_fock.zig:701-719 and the symmetrisation :729-734): F = H; the Coulomb
+ * loop F[l,m] += P[j,k] g[j,l,k,m]; if exx > 0 the exchange loop F[k,l] -= x exx P[i,j] g[i,j,k,l]
+ * (x = 1/2 RHF, 1 GHF); then F += Vxc (the exchange-correlation potential the caller evaluated, the
+ * pot.Vxc of :713-717; NULL = 0). */
+void orc_get_fock_dft(int n, const double *g, const double *H, const double *P, int generalized, double exx,
+                      const double *Vxc, double *F) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) AT2(F, n, i, j) = AT2(H, n, i, j);
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k)
+      for (int l = 0; l < n; ++l)
+        for (int m = 0; m < n; ++m) AT2(F, n, l, m) += AT2(P, n, j, k) * AT4(g, n, j, l, k, m);
+  if (exx > 0) {
+    const double factor = generalized ? 1.0 : 0.5;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        for (int k = 0; k < n; ++k)
+          for (int l = 0; l < n; ++l) AT2(F, n, k, l) -= factor * exx * AT2(P, n, i, j) * AT4(g, n, i, j, k, l);
+  }
+  if (Vxc)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) AT2(F, n, j, k) += AT2(Vxc, n, j, k);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double avg = (AT2(F, n, i, j) + AT2(F, n, j, i)) / 2;
+      AT2(F, n, i, j) = avg;
+      AT2(F, n, j, i) = avg;
+    }
+}
+
 /* diis, src/hartree_fock.zig:176-227.  fh/eh: m matrices of n*n.  Returns 1 if the LU solve fails. */
 int orc_diis(int m, int n, const double *fh, const double *eh, double *F) {
   if (m < 2) return 0;
@@ -280,9 +310,11 @@ int orc_diis(int m, int n, const double *fh, const double *eh, double *F) {
  * Inputs are the (spin-blocked for GHF) H, S, g of dimension n.  diis <= 0 disables DIIS (diis = null).
  * Outputs: C, e, P, F (the last diagonalised, DIIS-extrapolated F); returns 0, or 1 for
  * ScfDidNotConverge, 2 for LapackError.  *iters_out receives the number of iterations run. */
-int orc_scf(int n, int nocc, int generalized, const double *H, const double *S, const double *g, double VN, int diis,
-            int iterations, double threshold, double *E_out, double *C, double *e, double *P, double *F,
-            int *iters_out) {
+void orc_get_fock_packed(int n, const double *pk, const double *H, const double *P, double x, double *F);
+
+static int scf_impl(int n, int nocc, int generalized, const double *H, const double *S, const double *g, int packed,
+                    double VN, int diis, int iterations, double threshold, double *E_out, double *C, double *e,
+                    double *P, double *F, int *iters_out) {
   const size_t nn = (size_t)n * n;
   memset(P, 0, sizeof(double) * nn);
   if (orc_geigh(n, H, S, e, C)) return 2;
@@ -293,7 +325,10 @@ int orc_scf(int n, int nocc, int generalized, const double *H, const double *S, 
   double e_old = VN, e_new = VN;
   int converged = 0, it;
   for (it = 0; it < iterations; ++it) {
-    orc_get_fock(n, g, H, P, generalized, F);
+    if (packed)
+      orc_get_fock_packed(n, g, H, P, generalized ? 1.0 : 0.5, F);
+    else
+      orc_get_fock(n, g, H, P, generalized, F);
     e_new = orc_get_energy(n, H, F, P) + VN;
     if (diis > 0) {
       if (m >= diis) {
@@ -328,6 +363,20 @@ int orc_scf(int n, int nocc, int generalized, const double *H, const double *S, 
   *E_out = e_new;
   *iters_out = it;
   return converged ? 0 : 1;
+}
+
+int orc_scf(int n, int nocc, int generalized, const double *H, const double *S, const double *g, double VN, int diis,
+            int iterations, double threshold, double *E_out, double *C, double *e, double *P, double *F,
+            int *iters_out) {
+  return scf_impl(n, nocc, generalized, H, S, g, 0, VN, diis, iterations, threshold, E_out, C, e, P, F, iters_out);
+}
+
+/* orc_scf with the Fock build read from canonical packed ERIs (orc_get_fock_packed): bitwise orc_scf, for
+ * sizes whose dense tensor does not fit. */
+int orc_scf_packed(int n, int nocc, int generalized, const double *H, const double *S, const double *pk, double VN,
+                   int diis, int iterations, double threshold, double *E_out, double *C, double *e, double *P,
+                   double *F, int *iters_out) {
+  return scf_impl(n, nocc, generalized, H, S, pk, 1, VN, diis, iterations, threshold, E_out, C, e, P, F, iters_out);
 }
 
 /* ------------------------------------------------------------------ transforms (src/integral_transform.zig) */
@@ -469,6 +518,113 @@ void orc_ao2mo_oovv_par(int N, int nocc, const double *g, const double *C, doubl
 #undef T2
 #undef T3
 #undef OUT
+  free(t1), free(t2), free(t3);
+}
+
+/* ------------------------------------------------------------------ the same loops over packed ERIs
+ * For sizes whose dense n^4 tensor does not fit (n = 192: 10.9 GB, n = 240: 26.5 GB) the loops above are
+ * restated reading the canonical 8-fold packed array the integral engine writes (zinq_ints.h:
+ * out[I(I+1)/2+J] = (pq|rs), I = p(p+1)/2+q >= J = r(r+1)/2+s).  Each value fetched is the one the dense
+ * tensor holds at that position (the dense tensor is scattered from the same unique integrals,
+ * src/libint.cpp:143-150) and the sums run in the same order, so the results are bitwise those of
+ * orc_get_fock / orc_ao2mo_oovv on the rows or pairs computed. */
+static inline size_t pk_pair(size_t p, size_t q) { return p >= q ? p * (p + 1) / 2 + q : q * (q + 1) / 2 + p; }
+
+/* chemist (pq|rs) = dense g[p,r,q,s] */
+static inline double pk_chem(const double *pk, size_t p, size_t q, size_t r, size_t s) {
+  const size_t I = pk_pair(p, q), J = pk_pair(r, s);
+  return I >= J ? pk[I * (I + 1) / 2 + J] : pk[J * (J + 1) / 2 + I];
+}
+
+/* Rows `rows[0..nrows)` of getFock's symmetrised F (src/hartree_fock.zig:686-735, non-DFT branch) with an
+ * explicit exchange factor x: F[k,l] = H[k,l] + sum_ij P[i,j] (g[i,k,j,l] - x g[i,j,k,l]), row k of
+ * (F + F^T)/2.  Frows is nrows x n. */
+void orc_fock_rows_packed(int n, const double *pk, const double *H, const double *P, double x, int nrows,
+                          const int *rows, double *Frows) {
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int r = 0; r < nrows; ++r)
+    for (int l = 0; l < n; ++l) {
+      const int k = rows[r];
+      double fkl = AT2(H, n, k, l), flk = AT2(H, n, l, k);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+          /* g[i,k,j,l] = (ij|kl), g[i,j,k,l] = (ik|jl) */
+          fkl += AT2(P, n, i, j) * (pk_chem(pk, i, j, k, l) - x * pk_chem(pk, i, k, j, l));
+          flk += AT2(P, n, i, j) * (pk_chem(pk, i, j, l, k) - x * pk_chem(pk, i, l, j, k));
+        }
+      AT2(Frows, n, r, l) = k == l ? fkl : (fkl + flk) / 2;
+    }
+}
+
+/* The whole of getFock (src/hartree_fock.zig:686-735, non-DFT branch) from packed ERIs, exchange factor x:
+ * each F[k,l] accumulated over i, j ascending from H[k,l], then (F + F^T)/2 as orc_get_fock does. */
+void orc_get_fock_packed(int n, const double *pk, const double *H, const double *P, double x, double *F) {
+#pragma omp parallel for schedule(dynamic)
+  for (int k = 0; k < n; ++k)
+    for (int l = 0; l < n; ++l) {
+      double f = AT2(H, n, k, l);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) f += AT2(P, n, i, j) * (pk_chem(pk, i, j, k, l) - x * pk_chem(pk, i, k, j, l));
+      AT2(F, n, k, l) = f;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double avg = (AT2(F, n, i, j) + AT2(F, n, j, i)) / 2;
+      AT2(F, n, i, j) = avg;
+      AT2(F, n, j, i) = avg;
+    }
+}
+
+/* (ia|jb) = <ij|ab> of ao2mo_oovv (src/integral_transform.zig:165-219) for the occupied pairs
+ * (pi[t], pj[t]), t < npairs: out[t] is the v x v block [a][b].  The four quarter sums run over mu, lam,
+ * nu, sg in ascending order as in orc_ao2mo_oovv; the first quarter is kept for one i at a time, indexed
+ * by the canonical ket pair (lam sg) since g[mu,lam,nu,sg] = (mu nu|lam sg) is symmetric in it. */
+void orc_ovov_pairs_packed(int N, int nocc, const double *pk, const double *C, int npairs, const int *pi,
+                           const int *pj, double *out) {
+  const size_t n = N, o = nocc, v = N - nocc, P = n * (n + 1) / 2;
+  double *t1 = malloc(sizeof(double) * n * P); /* t1[nu][pair(lam,sg)] for the current i */
+  double *t2 = malloc(sizeof(double) * n * n), *t3 = malloc(sizeof(double) * v * n);
+  int cur = -1;
+  for (int t = 0; t < npairs; ++t) {
+    const size_t i = pi[t], j = pj[t];
+    if (i >= o || j >= o) continue;
+    if ((int)i != cur) {
+      cur = (int)i;
+      /* ket pair outermost: for I = (mu nu) below J the reads run along the packed row of J */
+#pragma omp parallel for schedule(dynamic, 64)
+      for (size_t J = 0; J < P; ++J) {
+        size_t lam = 0;
+        while ((lam + 1) * (lam + 2) / 2 <= J) ++lam;
+        const size_t sg = J - lam * (lam + 1) / 2;
+        for (size_t nu = 0; nu < n; ++nu) {
+          double s = 0;
+          for (size_t mu = 0; mu < n; ++mu) s += AT2(C, N, mu, i) * pk_chem(pk, mu, nu, lam, sg);
+          t1[nu * P + J] = s;
+        }
+      }
+    }
+#pragma omp parallel for
+    for (size_t nu = 0; nu < n; ++nu)
+      for (size_t sg = 0; sg < n; ++sg) {
+        double s = 0;
+        for (size_t lam = 0; lam < n; ++lam) s += AT2(C, N, lam, j) * t1[nu * P + pk_pair(lam, sg)];
+        t2[nu * n + sg] = s;
+      }
+#pragma omp parallel for
+    for (size_t a = 0; a < v; ++a)
+      for (size_t sg = 0; sg < n; ++sg) {
+        double s = 0;
+        for (size_t nu = 0; nu < n; ++nu) s += AT2(C, N, nu, o + a) * t2[nu * n + sg];
+        t3[a * n + sg] = s;
+      }
+#pragma omp parallel for
+    for (size_t a = 0; a < v; ++a)
+      for (size_t b = 0; b < v; ++b) {
+        double s = 0;
+        for (size_t sg = 0; sg < n; ++sg) s += AT2(C, N, sg, o + b) * t3[a * n + sg];
+        out[((size_t)t * v + a) * v + b] = s;
+      }
+  }
   free(t1), free(t2), free(t3);
 }
 

@@ -51,6 +51,11 @@ def lib():
         L.orc_get_fock_par.argtypes = [c_int, vp, vp, vp, c_int, vp]
         L.orc_ao2mo_oovv_par.argtypes = [c_int, c_int, vp, vp, vp]
         L.orc_max_threads.restype = c_int
+        L.orc_fock_rows_packed.argtypes = [c_int, vp, vp, vp, c_double, c_int, vp, vp]
+        L.orc_ovov_pairs_packed.argtypes = [c_int, c_int, vp, vp, c_int, vp, vp, vp]
+        L.orc_get_fock_packed.argtypes = [c_int, vp, vp, vp, c_double, vp]
+        L.orc_scf_packed.argtypes = L.orc_scf.argtypes
+        L.orc_get_fock_dft.argtypes = [c_int, vp, vp, vp, c_int, c_double, vp, vp]
     return _lib
 
 
@@ -140,19 +145,60 @@ class ScfDidNotConverge(RuntimeError):
     pass
 
 
-def scf(H, S, g, VN, nocc, generalized=False, diis=8, iterations=100, threshold=1e-8):
-    """src/hartree_fock.zig:738-875 with the core guess of :425-431.  Returns dict(E, C, e, P, F, iterations)."""
+def scf(H, S, g, VN, nocc, generalized=False, diis=8, iterations=100, threshold=1e-8, packed=None):
+    """src/hartree_fock.zig:738-875 with the core guess of :425-431.  Returns dict(E, C, e, P, F, iterations).
+    With packed= (canonical packed ERIs, g ignored) the Fock builds read those: bitwise the dense run."""
     n = H.shape[0]
+    fn = lib().orc_scf
+    if packed is not None:
+        g, fn = packed, lib().orc_scf_packed
     E = np.zeros(1)
     C, e, P, F = np.zeros((n, n)), np.zeros(n), np.zeros((n, n)), np.zeros((n, n))
     it = np.zeros(1, dtype=np.int32)
-    rc = lib().orc_scf(n, nocc, int(generalized), _p(_c(H)), _p(_c(S)), _p(_c(g)), VN, diis or 0, iterations, threshold,
+    rc = fn(n, nocc, int(generalized), _p(_c(H)), _p(_c(S)), _p(_c(g)), VN, diis or 0, iterations, threshold,
                        _p(E), _p(C), _p(e), _p(P), _p(F), _p(it))
     if rc == 1:
         raise ScfDidNotConverge()
     if rc == 2:
         raise RuntimeError("LapackError")
     return dict(E=float(E[0]), C=C, e=e, P=P, F=F, iterations=int(it[0]))
+
+
+def get_fock_packed(packed, H, P, x_exch=0.5):
+    """getFock from canonical packed ERIs; bitwise get_fock."""
+    n = H.shape[0]
+    F = np.zeros((n, n))
+    lib().orc_get_fock_packed(n, _p(_c(packed)), _p(_c(H)), _p(_c(P)), float(x_exch), _p(F))
+    return F
+
+
+def fock_rows_packed(packed, H, P, rows, x_exch=0.5):
+    """Rows of getFock's symmetrised F read from canonical packed ERIs; bitwise those of get_fock."""
+    n = H.shape[0]
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.zeros((len(r), n))
+    lib().orc_fock_rows_packed(n, _p(_c(packed)), _p(_c(H)), _p(_c(P)), float(x_exch), len(r), _p(r), _p(out))
+    return out
+
+
+def ovov_pairs_packed(packed, C, nocc, pairs):
+    """(ia|jb) blocks [t][a][b] of ao2mo_oovv for occupied pairs (i, j) from canonical packed ERIs; bitwise
+    those of ao2mo_oovv.  Pairs with the same i should be adjacent (the first quarter is kept per i)."""
+    n = C.shape[0]
+    pi = np.ascontiguousarray([p[0] for p in pairs], dtype=np.int32)
+    pj = np.ascontiguousarray([p[1] for p in pairs], dtype=np.int32)
+    out = np.zeros((len(pairs), n - nocc, n - nocc))
+    lib().orc_ovov_pairs_packed(n, nocc, _p(_c(packed)), _p(_c(C)), len(pairs), _p(pi), _p(pj), _p(out))
+    return out
+
+
+def get_fock_dft(g, H, P, exx, Vxc=None, generalized=False):
+    """getFock's DFT branch (Coulomb loop, exx-scaled exchange loop, + Vxc, symmetrised)."""
+    n = H.shape[0]
+    F = np.zeros((n, n))
+    V = _p(_c(Vxc)) if Vxc is not None else None
+    lib().orc_get_fock_dft(n, _p(_c(g)), _p(_c(H)), _p(_c(P)), int(generalized), float(exx), V, _p(F))
+    return F
 
 
 def ao2mo_pp(A, C):

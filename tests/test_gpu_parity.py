@@ -434,12 +434,14 @@ def test_serve_burst_reattach_eager():
 
 
 def test_attach_switch_back_replays_cached_iteration():
-    """One context attached to set A, then B, then A again (plain and mixed Fock): the SCF iteration
-    captured for A is kept across the switch and replayed, and A's results are bitwise those of its
-    first run; B's equal those of a fresh context."""
+    """One context attached to set A, then B, then A again (plain and mixed Fock), A the larger molecule
+    so that B's run grows no buffer: the SCF iteration captured for A is kept across the switch and
+    replayed (the second A run captures nothing), A's results are bitwise those of its first run, and
+    B's equal those of a fresh context."""
     from zinq_amd import hip
     mols = _serve_molecules()
-    a, b = mols[1], mols[2]
+    a, b = mols[2], mols[1]  # (H2O)_3 then (H2O)_2: the switch back to A finds every buffer large enough
+    assert a.n > b.n
     sa, sb = hip.EriSet(a.packed, a.n, 0, True), hip.EriSet(b.packed, b.n, 0, True)
     try:
         for fock in ("fp64", "fp32"):
@@ -450,8 +452,11 @@ def test_attach_switch_back_replays_cached_iteration():
                 return r["E"], r["iterations"], r["C"]
             with hip.Context(0) as c, hip.Context(0) as fresh:
                 first = run(c, a, sa)
+                assert c.scf_captures() == 1
                 mid = run(c, b, sb)
+                assert c.scf_captures() == 2
                 again = run(c, a, sa)
+                assert c.scf_captures() == 2, "A's cached iteration was captured again"
                 ref_b = run(fresh, b, sb)
             assert first[:2] == again[:2] and np.array_equal(first[2], again[2]), (fock, first[:2], again[:2])
             assert mid[:2] == ref_b[:2], (fock, mid[:2], ref_b[:2])

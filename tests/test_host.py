@@ -137,10 +137,13 @@ def test_water_cluster_generator_is_deterministic():
         assert np.linalg.norm(h1 - o) == pytest.approx(np.hypot(0.757, 0.587), abs=1e-12)
         cosang = np.dot(h1 - o, h2 - o) / np.linalg.norm(h1 - o) / np.linalg.norm(h2 - o)
         assert cosang == pytest.approx((0.587 ** 2 - 0.757 ** 2) / (0.757 ** 2 + 0.587 ** 2), abs=1e-12)
-    golden = os.path.join(ROOT, "tests", "golden", "water_k4.xyz")
-    if os.path.exists(golden):
-        ref = np.array([[float(v) for v in ln.split()[1:]] for ln in open(golden).read().splitlines()[2:]])
-        np.testing.assert_allclose(x1, ref, atol=1e-9)
+    # the committed fixtures of the benchmark clusters (tests/golden/make_clusters.py) match the generator
+    for k in (4, 6, 8, 10):
+        Zk, xk = water_cluster(k)
+        lines = open(os.path.join(ROOT, "tests", "golden", f"water_k{k}.xyz")).read().splitlines()
+        assert int(lines[0]) == 3 * k and [ln.split()[0] for ln in lines[2:]] == ["O", "H", "H"] * k
+        ref = np.array([[float(v) for v in ln.split()[1:]] for ln in lines[2:]])
+        np.testing.assert_allclose(xk, ref, atol=1e-9)
 
 
 def test_batch_specs_cover_the_workload():

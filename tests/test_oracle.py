@@ -218,3 +218,57 @@ def test_many_core_oracle_is_bitwise_the_faithful_one():
     C = np.linalg.qr(rng.standard_normal((n, n)))[0]
     np.testing.assert_array_equal(O.ao2mo_oovv_par(g, C, o), O.ao2mo_oovv(g, C, o))
     assert O.max_threads() >= 1
+
+
+@pytest.mark.parametrize("kind", ["sto3g", "ccpvdz", "w2"])
+def test_packed_oracle_is_bitwise_the_dense_one(kind):
+    """The packed-ERI restatements used at n >= 144 (where the dense tensor does not fit) give bit for bit
+    the rows of orc_get_fock (any exchange factor, also for a non-symmetric P) and the (ia|jb) blocks of
+    orc_ao2mo_oovv (same values fetched, same summation order)."""
+    s, H, S, g, VN = system_ints(kind)
+    n, o = s.nbf, s.nel // 2
+    pk = s.coulomb_packed()
+    rng = np.random.default_rng(3)
+    P = rng.standard_normal((n, n))
+    rows = [0, n // 3, n - 1]
+    for Pm, gen in ((P + P.T, False), (P, True)):
+        F = O.get_fock(g, H, Pm, gen)
+        np.testing.assert_array_equal(O.fock_rows_packed(pk, H, Pm, rows, 1.0 if gen else 0.5), F[rows])
+    C = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    T = O.ao2mo_oovv(g, C, o)
+    pairs = [(0, 0), (0, o - 1), (o - 1, 1)]
+    np.testing.assert_array_equal(O.ovov_pairs_packed(pk, C, o, pairs), np.stack([T[i, j] for i, j in pairs]))
+
+
+def test_dft_branch_fock():
+    """getFock's DFT branch: exx = 1 with no Vxc is the Hartree-Fock Fock matrix (loops in another order,
+    so to rounding); exx = 0 leaves the Coulomb part alone, linear in P with Tr(P J(P)) > 0; a symmetric
+    Vxc is added unchanged."""
+    s, H, S, g, VN = system_ints("ccpvdz")
+    n = s.nbf
+    rng = np.random.default_rng(4)
+    P = rng.standard_normal((n, n))
+    P = P + P.T
+    F_hf = O.get_fock(g, H, P)
+    np.testing.assert_allclose(O.get_fock_dft(g, H, P, 1.0), F_hf, rtol=0, atol=1e-12)
+    Z = np.zeros_like(H)
+    J = O.get_fock_dft(g, Z, P, 0.0)
+    np.testing.assert_allclose(O.get_fock_dft(g, Z, 2 * P, 0.0), 2 * J, rtol=0, atol=1e-11)
+    assert np.sum(P * J) > 0
+    K = (J - O.get_fock(g, Z, P)) / 0.5  # F_hf - H = J - K/2
+    np.testing.assert_allclose(O.get_fock_dft(g, H, P, 0.25), H + J - 0.125 * K, rtol=0, atol=1e-11)
+    V = rng.standard_normal((n, n))
+    V = V + V.T
+    np.testing.assert_allclose(O.get_fock_dft(g, H, P, 0.25, V), H + J - 0.125 * K + V, rtol=0, atol=1e-11)
+
+
+def test_packed_scf_is_bitwise_the_dense_one():
+    """orc_scf_packed (the Fock builds read from packed ERIs, used for the committed cluster energies
+    at n >= 144) runs the same SCF bit for bit as orc_scf."""
+    s, H, S, g, VN = system_ints("w2")
+    o = s.nel // 2
+    a = O.scf(H, S, g, VN, o, threshold=1e-10)
+    b = O.scf(H, S, None, VN, o, threshold=1e-10, packed=s.coulomb_packed())
+    assert a["E"] == b["E"] and a["iterations"] == b["iterations"]
+    np.testing.assert_array_equal(a["C"], b["C"])
+    np.testing.assert_array_equal(O.get_fock_packed(s.coulomb_packed(), H, a["P"]), O.get_fock(g, H, a["P"]))

@@ -56,14 +56,14 @@ ZqCaptureWindow::~ZqCaptureWindow() {
   }
 }
 
-void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh) {
+void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh, bool captured) {
   DevBuf &b = ctx->scratch[name];
   if (fresh) *fresh = false;
   if (b.bytes >= bytes && b.ptr) return b.ptr;
   if (b.ptr) (void)hipFree(b.ptr);
   b.ptr = nullptr;
   b.bytes = 0;
-  ++ctx->scratch_gen;
+  if (captured) ++ctx->scratch_gen;
   if (hipMalloc(&b.ptr, std::max<size_t>(bytes, 256)) != hipSuccess) {
     b.ptr = nullptr;
     (void)hipGetLastError();

@@ -118,6 +118,7 @@ struct zq_ctx {
   // is re-allocated; the cache is emptied then (scf_graphs_gen records the generation it holds).
   std::map<std::string, void *> scf_graphs;
   unsigned scf_graphs_gen = 0;
+  unsigned scf_captures = 0;  // SCF iterations captured over the context's lifetime (diagnostics)
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
   int scf_driver = 0;    // driver that finished the last device SCF: 1 device-decided, 2 host-synchronised
@@ -172,7 +173,10 @@ struct ZqCaptureWindow {
 // ---- helpers implemented in zq_context.hip
 int zq_fail(zq_ctx *ctx, int code, const std::string &msg);
 int zq_check(zq_ctx *ctx, hipError_t e, const char *what);
-void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh = nullptr);
+// Named grow-only device buffer of the context.  `captured`: the buffer may be held by a captured SCF
+// iteration, so re-allocating it empties the capture cache (transform and multi-density buffers, never
+// captured, pass false).
+void *zq_scratch(zq_ctx *ctx, const std::string &name, size_t bytes, bool *fresh = nullptr, bool captured = true);
 void *zq_pinned(zq_ctx *ctx, const std::string &name, size_t bytes);
 void zq_free_eri(zq_ctx *ctx);
 void zq_prof_begin(zq_ctx *ctx, const char *name);

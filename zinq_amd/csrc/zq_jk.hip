@@ -591,9 +591,9 @@ extern "C" int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, s
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   const int n = ctx->n, np = ctx->npad;
   const size_t nn = size_t(n) * n, pstride = size_t(np) * np, rstride = ctx->nrecords * 64;
-  auto *dP = (double *)zq_scratch(ctx, "jkm_P", ndens * pstride * sizeof(double));
-  auto *dR = (double *)zq_scratch(ctx, "jkm_rec", ndens * rstride * sizeof(double));
-  auto *dF = (double *)zq_scratch(ctx, "jkm_F", ndens * nn * sizeof(double));
+  auto *dP = (double *)zq_scratch(ctx, "jkm_P", ndens * pstride * sizeof(double), nullptr, false);
+  auto *dR = (double *)zq_scratch(ctx, "jkm_rec", ndens * rstride * sizeof(double), nullptr, false);
+  auto *dF = (double *)zq_scratch(ctx, "jkm_F", ndens * nn * sizeof(double), nullptr, false);
   if (!dP || !dR || !dF) return zq_fail(ctx, ZQ_ERR_OOM, "zq_fock_jk_multi: scratch");
   // each density into its own zero-padded npad x npad slot
   ZQ_TRY(ctx, hipMemsetAsync(dP, 0, ndens * pstride * sizeof(double), ctx->stream));

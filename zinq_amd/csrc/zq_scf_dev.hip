@@ -192,6 +192,10 @@ int alloc(zq_ctx *ctx, Dev &d, int n, int maxh) {
   d.slots = (int *)zq_scratch(ctx, "scf_slots", 64 * sizeof(int));
   d.info = (int *)zq_scratch(ctx, "scf_info", 4 * sizeof(int));
   if (!d.slots || !d.info) return zq_fail(ctx, ZQ_ERR_OOM, "device SCF buffers");
+  // the final eigensolve's buffers (zq_eigen.hip) too: a buffer first allocated after the iteration was
+  // captured would empty the capture cache at the next run
+  if (!zq_scratch(ctx, "eig_Q", nn * sizeof(double)) || !zq_scratch(ctx, "eig_Z", nn * sizeof(double)))
+    return zq_fail(ctx, ZQ_ERR_OOM, "eigen-solver buffers");
   return ZQ_OK;
 }
 
@@ -716,6 +720,7 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
     (void)hipGraphDestroy(graph);
     if (e2 != hipSuccess) return zq_check(ctx, e2, "hipGraphInstantiate");
     ctx->scf_graphs[key] = exec;
+    ++ctx->scf_captures;
     graph_exec = exec;
   }
   // the stream is idle here (the previous run ended with a synchronisation), so no kernel writes it now
@@ -927,6 +932,8 @@ extern "C" int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps) {
 }
 
 extern "C" int zq_scf_last_driver(const zq_ctx *ctx) { return ctx ? ctx->scf_driver : 0; }
+
+extern "C" int zq_scf_captures(const zq_ctx *ctx) { return ctx ? int(ctx->scf_captures) : 0; }
 
 extern "C" int zq_scf_last_sp2(const zq_ctx *ctx, int *densities, int *folded, int *steps) {
   if (!ctx || !densities || !folded || !steps) return ZQ_ERR_INVALID;

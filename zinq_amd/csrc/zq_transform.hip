@@ -641,7 +641,7 @@ int launch_gemm_bf16x3(zq_ctx *ctx, const GemmArgs<float> &g, int batch, const c
 
 template <class T>
 int upload(zq_ctx *ctx, const char *name, const std::vector<T> &h, T **d) {
-  *d = (T *)zq_scratch(ctx, name, h.size() * sizeof(T));
+  *d = (T *)zq_scratch(ctx, name, h.size() * sizeof(T), nullptr, false);
   if (!*d) return zq_fail(ctx, ZQ_ERR_OOM, std::string("transform scratch ") + name);
   ZQ_TRY(ctx, hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
   return ZQ_OK;
@@ -689,10 +689,10 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
 
   const size_t nX = size_t(Nb) * 64 * m1 * np, nY = size_t(m1) * m2 * np * np, nW = size_t(m1) * m2 * np * m3,
                nO = size_t(m1) * m2 * m3 * m4;
-  T *X = (T *)zq_scratch(ctx, "X", nX * sizeof(T));
-  T *Y = (T *)zq_scratch(ctx, "Y", nY * sizeof(T));
-  T *W = (T *)zq_scratch(ctx, "W", nW * sizeof(T));
-  T *O = (T *)zq_scratch(ctx, "OUT", nO * sizeof(T));
+  T *X = (T *)zq_scratch(ctx, "X", nX * sizeof(T), nullptr, false);
+  T *Y = (T *)zq_scratch(ctx, "Y", nY * sizeof(T), nullptr, false);
+  T *W = (T *)zq_scratch(ctx, "W", nW * sizeof(T), nullptr, false);
+  T *O = (T *)zq_scratch(ctx, "OUT", nO * sizeof(T), nullptr, false);
   if (!X || !Y || !W || !O) return zq_fail(ctx, ZQ_ERR_OOM, "transform scratch");
 
   // Q1: MT (<= 4) tiles of 16 occupied orbitals per workgroup, ceil(m1 / (16 MT)) workgroups along i.
@@ -839,7 +839,7 @@ extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, in
   const size_t cnt = size_t(dims[0]) * dims[1] * dims[2] * dims[3];
   const double *src = (const double *)d;
   if (f32) {
-    double *wide = (double *)zq_scratch(ctx, "OUT64", cnt * sizeof(double));
+    double *wide = (double *)zq_scratch(ctx, "OUT64", cnt * sizeof(double), nullptr, false);
     if (!wide) return zq_fail(ctx, ZQ_ERR_OOM, "zq_ao2mo: widen buffer");
     hipLaunchKernelGGL(zq_widen, dim3(1024), dim3(256), 0, ctx->stream, (const float *)d, cnt, wide);
     ZQ_LAUNCH_CHECK(ctx, "zq_widen");
@@ -856,8 +856,8 @@ int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int p
   bool f32 = false;
   if (int rc = zq_transform_device(ctx, C, nocc, ZQ_BLOCK_OVOV, precision, &T, dims, &f32)) return rc;
   const int o = dims[0], v = dims[2];
-  double *de = (double *)zq_scratch(ctx, "mp2_e", size_t(ctx->n) * 8);
-  double *part = (double *)zq_scratch(ctx, "mp2_part", size_t(o) * o * 8 + 8);
+  double *de = (double *)zq_scratch(ctx, "mp2_e", size_t(ctx->n) * 8, nullptr, false);
+  double *part = (double *)zq_scratch(ctx, "mp2_part", size_t(o) * o * 8 + 8, nullptr, false);
   if (!de || !part) return zq_fail(ctx, ZQ_ERR_OOM, "mp2 scratch");
   ZQ_TRY(ctx, hipMemcpyAsync(de, e, size_t(ctx->n) * 8, hipMemcpyHostToDevice, ctx->stream));
   zq_prof_begin(ctx, "mp2_energy");

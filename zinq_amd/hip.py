@@ -18,7 +18,7 @@ EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_l
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
             "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
-            "zq_scf_last_driver", "zq_scf_last_sp2"]
+            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures"]
 
 
 class GpuError(RuntimeError):
@@ -87,6 +87,7 @@ def lib():
         L.zq_scf_set_sp2_budget.argtypes = [vp, ci]
         L.zq_eigh.argtypes = [vp, sz, vp, vp, vp]
         L.zq_scf_last_driver.argtypes = [vp]
+        L.zq_scf_captures.argtypes = [vp]
         L.zq_scf_last_sp2.argtypes = [vp, vp, vp, vp]
         L.zq_eri_set_create.restype = vp
         L.zq_eri_set_create.argtypes = [ci, vp, sz, ci]
@@ -250,6 +251,10 @@ class Context:
     def last_scf_driver(self):
         """'device' or 'synced': the driver that finished the last device SCF (zq_scf_last_driver)."""
         return {0: None, 1: "device", 2: "synced"}[self._L.zq_scf_last_driver(self._h)]
+
+    def scf_captures(self):
+        """SCF iterations captured as hipGraphs by this context so far (zq_scf_captures)."""
+        return int(self._L.zq_scf_captures(self._h))
 
     def last_scf_sp2(self):
         """{densities, folded, steps} of the last device SCF's SP2 runs (zq_scf_last_sp2)."""

@@ -31,6 +31,8 @@
 
 namespace {
 
+constexpr int kMaxGridZ = 65535;  // largest gridDim.z of one launch
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16;
@@ -620,20 +622,32 @@ __global__ void zq_widen(const float *__restrict__ x, size_t n, double *__restri
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) y[i] = x[i];
 }
 
+// The batch index of the GEMM kernels is blockIdx.z, at most kMaxGridZ per launch: larger batches (the
+// FULL transform's n^2 (i,j) pairs from n = 256 on) run as several launches over consecutive slices.
+template <class T>
+GemmArgs<T> batch_slice(GemmArgs<T> g, long long b0) {
+  g.A += b0 * g.sAb, g.B += b0 * g.sBb, g.C += b0 * g.sCb;
+  return g;
+}
+
 template <class MF>
 int launch_gemm(zq_ctx *ctx, const GemmArgs<typename MF::T> &g, int batch, const char *name) {
-  dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(batch));
   zq_prof_begin(ctx, name);
-  hipLaunchKernelGGL(zq_gemm<MF>, grid, dim3(256), 0, ctx->stream, g);
+  for (int b0 = 0; b0 < batch; b0 += kMaxGridZ) {
+    const dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(std::min(kMaxGridZ, batch - b0)));
+    hipLaunchKernelGGL(zq_gemm<MF>, grid, dim3(256), 0, ctx->stream, batch_slice(g, b0));
+  }
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, name);
   return ZQ_OK;
 }
 
 int launch_gemm_bf16x3(zq_ctx *ctx, const GemmArgs<float> &g, int batch, const char *name) {
-  dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(batch));
   zq_prof_begin(ctx, name);
-  hipLaunchKernelGGL(zq_gemm_bf16x3, grid, dim3(256), 0, ctx->stream, g);
+  for (int b0 = 0; b0 < batch; b0 += kMaxGridZ) {
+    const dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(std::min(kMaxGridZ, batch - b0)));
+    hipLaunchKernelGGL(zq_gemm_bf16x3, grid, dim3(256), 0, ctx->stream, batch_slice(g, b0));
+  }
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, name);
   return ZQ_OK;

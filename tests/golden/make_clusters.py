@@ -1,9 +1,9 @@
-"""Writes the seeded water-cluster fixtures: water_k{4,6,8,10}.xyz (the geometries of the (H2O)_k cc-pVDZ
+"""Writes the seeded water-cluster fixtures: water_k{4..10}.xyz (the geometries of the (H2O)_k cc-pVDZ
 workloads, from zinq_amd.molecules.water_cluster) and water_clusters_oracle.json, the CPU oracle's RHF
 (DIIS-8, threshold 1e-10) and closed-shell MP2 energies of each.  The oracle reads the canonical packed
 ERIs (orc_scf_packed / orc_ovov_pairs_packed, bitwise the dense loops), so n = 240 fits in memory.
 
-    python tests/golden/make_clusters.py [k ...]      (default 4 6 8 10; minutes of CPU time for k = 10)
+    python tests/golden/make_clusters.py [k ...]      (default 4..10; minutes of CPU time for k = 10)
 """
 import json
 import os
@@ -49,4 +49,4 @@ def main(ks):
 
 
 if __name__ == "__main__":
-    main([int(a) for a in sys.argv[1:]] or [4, 6, 8, 10])
+    main([int(a) for a in sys.argv[1:]] or list(range(4, 11)))

@@ -86,6 +86,17 @@ int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision
  * equal to the order-2 term of mp() (src/moller_plesset.zig:340-481) for an RHF reference. */
 int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr);
 
+/* zq_mp2 that allocates nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same contract and
+ * result as zq_mp2 (the FP64 Q4 product runs on the hand-written tile kernel instead of rocBLAS, so
+ * FP64 results agree to rounding), but every intermediate lives in one static device array of 64 MiB
+ * per GPU, held for the life of the process: molecules up to n = 72 fit (n x nocc dependent; a larger
+ * one returns ZQ_ERR_INVALID with the reason).  The FP32 forms need the FP32 ERI copy already resident
+ * (an ERI set created with_fp32, or an earlier FP32 call on ctx), else ZQ_ERR_STATE.  All calls on one
+ * GPU share that array: calls that overlap in time on one GPU (several threads or contexts) race and
+ * must be serialised by the caller. */
+int zq_mp2_noalloc(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr);
+
 /* ---- measurement hooks (no counterpart in the reference) ----
  * With profiling on, every kernel launch is bracketed by HIP events on the context's stream;
  * zq_profile_get returns the summed milliseconds and launch count of kernels whose name starts

@@ -212,6 +212,7 @@ zq_ctx *zq_create(int device) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
   if (hipSetDevice(device) != hipSuccess) return nullptr;
+  zq_noalloc_setup(device);
   auto *ctx = new zq_ctx;
   ctx->device = device;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

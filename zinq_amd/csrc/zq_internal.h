@@ -190,6 +190,15 @@ int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, ma
 void zq_narrow_eri(hipStream_t stream, const double *eri, size_t cnt, float *out);  // zq_jk.hip
 // MP2 correlation energy of the resident ERIs (zq_transform.hip); C and e are host arrays.
 int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr);
+// Q1..Q4 of the resident ERIs into context scratch (zq_transform.hip): *d_out = out[i][j][a][b] with dims
+// m1 x m2 x m3 x m4 (FULL: n^4, <pq|rs>; OVOV: o o v v), double or (*is_f32) float.
+int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, void **d_out, int dims[4],
+                        bool *is_f32);
+// Devices whose static no-allocation scratch address is kept (zq_transform.hip).
+constexpr int kZqMaxDevices = 64;
+// Looks up, once per device, the device address of zq_mp2_noalloc's static scratch; called by zq_create
+// with `device` current (zq_transform.hip).
+void zq_noalloc_setup(int device);
 // Grouped J/K on `stream`: F = H + G(P) (x_exch) for every molecule of a grouped plan whose done flag
 // is 0.  blk[g] = (molecule, output block) for the nblk CSR rows.  Returns the launch status.
 hipError_t zq_jk_group(hipStream_t stream, const double *eri, const JKItem *items, int nitems, const double *P,

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include <rocblas/rocblas.h>
@@ -653,6 +654,21 @@ int launch_gemm_bf16x3(zq_ctx *ctx, const GemmArgs<float> &g, int batch, const c
   return ZQ_OK;
 }
 
+// Device buffers of one transform: the context's grow-only scratch, or (base != null) consecutive
+// 256-byte aligned pieces of the static no-allocation scratch, cap bytes long.
+struct Arena {
+  zq_ctx *ctx;
+  char *base = nullptr;
+  size_t cap = 0, off = 0;
+  void *get(const char *name, size_t bytes) {
+    if (!base) return zq_scratch(ctx, name, bytes, nullptr, false);
+    const size_t at = (off + 255) & ~size_t(255);
+    if (at + bytes > cap) return nullptr;
+    off = at + bytes;
+    return base + at;
+  }
+};
+
 template <class T>
 int upload(zq_ctx *ctx, const char *name, const std::vector<T> &h, T **d) {
   *d = (T *)zq_scratch(ctx, name, h.size() * sizeof(T), nullptr, false);
@@ -661,53 +677,126 @@ int upload(zq_ctx *ctx, const char *name, const std::vector<T> &h, T **d) {
   return ZQ_OK;
 }
 
+// The coefficient slices of run_chain built on the device from C (n x n, already on the device): the
+// same element conversions as the host staging, so the values are identical.
+template <class T>
+__global__ void zq_c_slices(const double *__restrict__ C, int n, int np, int m1, int m2, int m3, int off3,
+                            T *__restrict__ c1, T *__restrict__ c2t, T *__restrict__ c3) {
+  const int mu = blockIdx.x;  // np rows
+  for (int t = threadIdx.x; t < m1 + m2 + m3; t += blockDim.x) {
+    if (t < m1) {
+      c1[size_t(mu) * m1 + t] = mu < n ? T(C[size_t(mu) * n + t]) : T(0);
+    } else if (t < m1 + m2) {
+      const int j = t - m1;
+      c2t[size_t(j) * np + mu] = mu < n ? T(C[size_t(mu) * n + j]) : T(0);
+    } else {
+      const int a = t - m1 - m2;
+      c3[size_t(mu) * m3 + a] = mu < n ? T(C[size_t(mu) * n + off3 + a]) : T(0);
+    }
+  }
+}
+
+// BF16X3 Q1 operand: C1 split into bf16 hi + lo and transposed, [i][mu] with leading dimension ldt.
+__global__ void zq_c1_split(const double *__restrict__ C, int n, int m1, int ldt, bf16 *__restrict__ hh,
+                            bf16 *__restrict__ hl) {
+  const int i = blockIdx.x;  // m1 rows
+  for (int mu = threadIdx.x; mu < ldt; mu += blockDim.x) {
+    bf16 h = (bf16)0.f, l = (bf16)0.f;
+    if (mu < n) {
+      const float x = float(C[size_t(mu) * n + i]);
+      h = (bf16)x;
+      l = (bf16)(x - (float)h);
+    }
+    hh[size_t(i) * ldt + mu] = h;
+    hl[size_t(i) * ldt + mu] = l;
+  }
+}
+
 struct Dims {
   int n, np, nb, Nb, m1, m2, m3, m4, off3;
 };
 
 // Q1..Q4 with element type T (double: FP64, float: FP32 or BF16X3).
+// Bytes run_chain carves from a static arena for these dimensions (each piece 256-byte aligned), the
+// no-allocation entry point's size check.
+size_t chain_bytes(const Dims &d, int precision) {
+  const size_t t = precision == ZQ_PREC_FP64 ? 8 : 4, np = d.np, m1 = d.m1, m2 = d.m2, m3 = d.m3, m4 = d.m4;
+  const size_t ldt = ((d.nb + 3) / 4) * 32;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  size_t b = al(size_t(d.n) * d.n * 8) + al(np * m1 * t) + al(m2 * np * t) + al(np * m3 * t);
+  if (precision == ZQ_PREC_BF16X3) b += 2 * al(m1 * ldt * sizeof(bf16));
+  b += al(size_t(d.Nb) * 64 * m1 * np * t) + al(m1 * m2 * np * np * t) + al(m1 * m2 * np * m3 * t) + al(m1 * m2 * m3 * m4 * t);
+  return b;
+}
+
+// Q1..Q4.  With a static arena (ar.base set) nothing is allocated: C goes to the arena and is sliced
+// there, FP32 forms need the resident FP32 ERI copy already made, and FP64 Q4 runs on the tile kernel.
 template <class T, class MF>
-int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_out) {
+int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_out, Arena &ar) {
   const int n = d.n, np = d.np, nb = d.nb, Nb = d.Nb, m1 = d.m1, m2 = d.m2, m3 = d.m3, m4 = d.m4;
-  // coefficient slices, zero rows for padded AOs: C1 (np x m1), C2T (m2 x np), C3 = C4 (np x m3)
-  std::vector<T> h1(size_t(np) * m1, T(0)), h2t(size_t(m2) * np, T(0)), h3(size_t(np) * m3, T(0));
-  for (int mu = 0; mu < n; ++mu) {
-    for (int i = 0; i < m1; ++i) h1[size_t(mu) * m1 + i] = T(C[size_t(mu) * n + i]);
-    for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = T(C[size_t(mu) * n + j]);
-    for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = T(C[size_t(mu) * n + d.off3 + a]);
-  }
-  if constexpr (sizeof(T) == 4) {
-    if (precision != ZQ_PREC_BF16X3)
-      if (int rc = zq_eri32(ctx)) return rc;
-  }
-  T *d1, *d2t, *d3;
-  if (int rc = upload(ctx, "C1", h1, &d1)) return rc;
-  if (int rc = upload(ctx, "C2T", h2t, &d2t)) return rc;
-  if (int rc = upload(ctx, "C3", h3, &d3)) return rc;
+  const bool fixed = ar.base != nullptr;
   // BF16X3 Q1 takes C1 split and transposed: C1h/C1l[i][mu], mu padded to a multiple of 32
   const int ldt = ((nb + 3) / 4) * 32;
+  T *d1, *d2t, *d3;
   bf16 *d1h = nullptr, *d1l = nullptr;
-  if (precision == ZQ_PREC_BF16X3) {
-    std::vector<bf16> hh(size_t(m1) * ldt, (bf16)0.f), hl(size_t(m1) * ldt, (bf16)0.f);
-    for (int i = 0; i < m1; ++i)
-      for (int mu = 0; mu < n; ++mu) {
-        const float x = float(C[size_t(mu) * n + i]);
-        const bf16 h = (bf16)x;
-        hh[size_t(i) * ldt + mu] = h;
-        hl[size_t(i) * ldt + mu] = (bf16)(x - (float)h);
-      }
-    if (int rc = upload(ctx, "C1h", hh, &d1h)) return rc;
-    if (int rc = upload(ctx, "C1l", hl, &d1l)) return rc;
+  if constexpr (sizeof(T) == 4) {
+    if (precision != ZQ_PREC_BF16X3) {
+      if (fixed && !ctx->eri32)
+        return zq_fail(ctx, ZQ_ERR_STATE, "zq_mp2_noalloc: FP32 forms need the FP32 ERI copy resident (an ERI set made "
+                                          "with_fp32, or an earlier FP32 call on this context)");
+      if (int rc = zq_eri32(ctx)) return rc;
+    }
   }
-  ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the host staging vectors go out of scope below
+  if (fixed) {
+    auto *dC = (double *)ar.get("C", size_t(n) * n * 8);
+    d1 = (T *)ar.get("C1", size_t(np) * m1 * sizeof(T));
+    d2t = (T *)ar.get("C2T", size_t(m2) * np * sizeof(T));
+    d3 = (T *)ar.get("C3", size_t(np) * m3 * sizeof(T));
+    if (precision == ZQ_PREC_BF16X3) {
+      d1h = (bf16 *)ar.get("C1h", size_t(m1) * ldt * sizeof(bf16));
+      d1l = (bf16 *)ar.get("C1l", size_t(m1) * ldt * sizeof(bf16));
+    }
+    if (!dC || !d1 || !d2t || !d3 || (precision == ZQ_PREC_BF16X3 && (!d1h || !d1l)))
+      return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2_noalloc: static scratch too small");
+    ZQ_TRY(ctx, hipMemcpyAsync(dC, C, size_t(n) * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(zq_c_slices<T>, dim3(unsigned(np)), dim3(256), 0, ctx->stream, dC, n, np, m1, m2, m3, d.off3, d1,
+                       d2t, d3);
+    if (precision == ZQ_PREC_BF16X3)
+      hipLaunchKernelGGL(zq_c1_split, dim3(unsigned(m1)), dim3(256), 0, ctx->stream, dC, n, m1, ldt, d1h, d1l);
+    ZQ_LAUNCH_CHECK(ctx, "zq_c_slices");
+  } else {
+    // coefficient slices, zero rows for padded AOs: C1 (np x m1), C2T (m2 x np), C3 = C4 (np x m3)
+    std::vector<T> h1(size_t(np) * m1, T(0)), h2t(size_t(m2) * np, T(0)), h3(size_t(np) * m3, T(0));
+    for (int mu = 0; mu < n; ++mu) {
+      for (int i = 0; i < m1; ++i) h1[size_t(mu) * m1 + i] = T(C[size_t(mu) * n + i]);
+      for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = T(C[size_t(mu) * n + j]);
+      for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = T(C[size_t(mu) * n + d.off3 + a]);
+    }
+    if (int rc = upload(ctx, "C1", h1, &d1)) return rc;
+    if (int rc = upload(ctx, "C2T", h2t, &d2t)) return rc;
+    if (int rc = upload(ctx, "C3", h3, &d3)) return rc;
+    if (precision == ZQ_PREC_BF16X3) {
+      std::vector<bf16> hh(size_t(m1) * ldt, (bf16)0.f), hl(size_t(m1) * ldt, (bf16)0.f);
+      for (int i = 0; i < m1; ++i)
+        for (int mu = 0; mu < n; ++mu) {
+          const float x = float(C[size_t(mu) * n + i]);
+          const bf16 h = (bf16)x;
+          hh[size_t(i) * ldt + mu] = h;
+          hl[size_t(i) * ldt + mu] = (bf16)(x - (float)h);
+        }
+      if (int rc = upload(ctx, "C1h", hh, &d1h)) return rc;
+      if (int rc = upload(ctx, "C1l", hl, &d1l)) return rc;
+    }
+    ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the host staging vectors go out of scope below
+  }
 
   const size_t nX = size_t(Nb) * 64 * m1 * np, nY = size_t(m1) * m2 * np * np, nW = size_t(m1) * m2 * np * m3,
                nO = size_t(m1) * m2 * m3 * m4;
-  T *X = (T *)zq_scratch(ctx, "X", nX * sizeof(T), nullptr, false);
-  T *Y = (T *)zq_scratch(ctx, "Y", nY * sizeof(T), nullptr, false);
-  T *W = (T *)zq_scratch(ctx, "W", nW * sizeof(T), nullptr, false);
-  T *O = (T *)zq_scratch(ctx, "OUT", nO * sizeof(T), nullptr, false);
-  if (!X || !Y || !W || !O) return zq_fail(ctx, ZQ_ERR_OOM, "transform scratch");
+  T *X = (T *)ar.get("X", nX * sizeof(T));
+  T *Y = (T *)ar.get("Y", nY * sizeof(T));
+  T *W = (T *)ar.get("W", nW * sizeof(T));
+  T *O = (T *)ar.get("OUT", nO * sizeof(T));
+  if (!X || !Y || !W || !O) return zq_fail(ctx, fixed ? ZQ_ERR_INVALID : ZQ_ERR_OOM, "transform scratch");
 
   // Q1: MT (<= 4) tiles of 16 occupied orbitals per workgroup, ceil(m1 / (16 MT)) workgroups along i.
   // Every workgroup along i reads all of its quartets, so one pass (m1 <= 64) beats the higher
@@ -779,6 +868,12 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
   if (int rc = launch_gemm<MF>(ctx, g2, m1, "ao2mo_q2")) return rc;
   if (int rc = launch_gemm<MF>(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
   if constexpr (std::is_same<MF, MF64>::value) {
+    if (fixed) {
+      // no library call in the no-allocation path: the tile kernel
+      if (int rc = launch_gemm<MF>(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
+      *d_out = O;
+      return ZQ_OK;
+    }
     // FP64 Q4 is a plain strided-batched product with W_ij read transposed: rocBLAS DGEMM (column-major,
     // so each row-major matrix is read as its transpose), atomics off, measured faster than the tile
     // kernel above for this shape.
@@ -814,10 +909,9 @@ int zq_dgemm(zq_ctx *ctx, const double *A, long long sAm, long long sAk, const d
 
 // Runs Q1..Q4 into device memory.  *d_out receives out[i][j][a][b] (dims m1 x m2 x m3 x m4) as double
 // (FP64) or float (FP32, BF16X3); *is_f32 tells which.
-int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, void **d_out, int dims[4],
-                        bool *is_f32) {
+namespace {
+int make_dims(zq_ctx *ctx, int nocc, int block, int precision, Dims &d) {
   if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "no ERIs loaded");
-  Dims d;
   d.n = ctx->n, d.np = ctx->npad, d.nb = ctx->nb, d.Nb = ctx->Nb;
   if (block != ZQ_BLOCK_FULL && block != ZQ_BLOCK_OVOV) return zq_fail(ctx, ZQ_ERR_INVALID, "bad block");
   if (precision != ZQ_PREC_FP64 && precision != ZQ_PREC_FP32 && precision != ZQ_PREC_BF16X3 &&
@@ -826,20 +920,34 @@ int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int p
   if (block == ZQ_BLOCK_OVOV && (nocc <= 0 || nocc >= d.n)) return zq_fail(ctx, ZQ_ERR_INVALID, "bad nocc");
   d.m1 = block == ZQ_BLOCK_OVOV ? nocc : d.n, d.m2 = d.m1;
   d.off3 = block == ZQ_BLOCK_OVOV ? nocc : 0, d.m3 = d.n - d.off3, d.m4 = d.m3;
+  return ZQ_OK;
+}
+
+int transform_in(zq_ctx *ctx, const double *C, const Dims &d, int precision, Arena &ar, void **d_out, int dims[4],
+                 bool *is_f32) {
   dims[0] = d.m1, dims[1] = d.m2, dims[2] = d.m3, dims[3] = d.m4;
   if (precision == ZQ_PREC_FP64) {
     *is_f32 = false;
     double *o = nullptr;
-    int rc = run_chain<double, MF64>(ctx, C, d, precision, &o);
+    int rc = run_chain<double, MF64>(ctx, C, d, precision, &o, ar);
     *d_out = o;
     return rc;
   }
   *is_f32 = true;
   float *o = nullptr;
-  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, C, d, precision, &o)
-                                      : run_chain<float, MF32>(ctx, C, d, precision, &o);
+  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, C, d, precision, &o, ar)
+                                      : run_chain<float, MF32>(ctx, C, d, precision, &o, ar);
   *d_out = o;
   return rc;
+}
+}  // namespace
+
+int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, void **d_out, int dims[4],
+                        bool *is_f32) {
+  Dims d;
+  if (int rc = make_dims(ctx, nocc, block, precision, d)) return rc;
+  Arena ar{ctx};
+  return transform_in(ctx, C, d, precision, ar, d_out, dims, is_f32);
 }
 
 extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out) {
@@ -864,15 +972,20 @@ extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, in
   return ZQ_OK;
 }
 
-int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr) {
+namespace {
+int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr, Arena &ar) {
+  Dims dm;
+  if (int rc = make_dims(ctx, nocc, ZQ_BLOCK_OVOV, precision, dm)) return rc;
+  if (ar.base && chain_bytes(dm, precision) + 2 * 256 + size_t(ctx->n) * 8 + size_t(nocc) * nocc * 8 + 8 > ar.cap)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2_noalloc: this molecule needs more than the 64 MiB static scratch");
   void *T = nullptr;
   int dims[4];
   bool f32 = false;
-  if (int rc = zq_transform_device(ctx, C, nocc, ZQ_BLOCK_OVOV, precision, &T, dims, &f32)) return rc;
+  if (int rc = transform_in(ctx, C, dm, precision, ar, &T, dims, &f32)) return rc;
   const int o = dims[0], v = dims[2];
-  double *de = (double *)zq_scratch(ctx, "mp2_e", size_t(ctx->n) * 8, nullptr, false);
-  double *part = (double *)zq_scratch(ctx, "mp2_part", size_t(o) * o * 8 + 8, nullptr, false);
-  if (!de || !part) return zq_fail(ctx, ZQ_ERR_OOM, "mp2 scratch");
+  double *de = (double *)ar.get("mp2_e", size_t(ctx->n) * 8);
+  double *part = (double *)ar.get("mp2_part", size_t(o) * o * 8 + 8);
+  if (!de || !part) return zq_fail(ctx, ar.base ? ZQ_ERR_INVALID : ZQ_ERR_OOM, "mp2 scratch");
   ZQ_TRY(ctx, hipMemcpyAsync(de, e, size_t(ctx->n) * 8, hipMemcpyHostToDevice, ctx->stream));
   zq_prof_begin(ctx, "mp2_energy");
   if (f32)
@@ -888,9 +1001,60 @@ int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int p
   return ZQ_OK;
 }
 
+// ---- the static no-allocation scratch (zq_mp2_noalloc)
+// kNoAllocBytes hold every intermediate of the MP2 transform for n <= 72 in any precision (the
+// molecules of the batched workload: 47.5 MB for (H2O)_3 cc-pVDZ in FP64).  Each device holds its own
+// copy for the life of the process.  In host code the array's name is not a device address: its address
+// on each device comes from hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kZqMaxDevices] = {};
+int g_scratch_rc[kZqMaxDevices] = {};  // hipError_t of the lookup + 1, 0 before it
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr) {
+  Arena ar{ctx};
+  return mp2_in(ctx, C, e, nocc, precision, e_corr, ar);
+}
+
+// Called by zq_create with `device` current: the device address of this device's static scratch.
+void zq_noalloc_setup(int device) {
+  if (device < 0 || device >= kZqMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_rc[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_rc[device] = int(e) + 1;
+  g_scratch_ptr[device] = e == hipSuccess ? (char *)p : nullptr;
+  if (e != hipSuccess) (void)hipGetLastError();
+}
+
 extern "C" int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr) {
   ZqGate _gate;
   if (!ctx || !C || !e || !e_corr) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2: null argument");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   return zq_mp2_device(ctx, C, e, int(nocc), precision, e_corr);
+}
+
+extern "C" int zq_mp2_noalloc(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision,
+                              double *e_corr) {
+  ZqGate _gate;
+  if (!ctx || !C || !e || !e_corr) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2_noalloc: null argument");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < kZqMaxDevices) base = g_scratch_ptr[ctx->device];
+  }
+  if (!base) return zq_fail(ctx, ZQ_ERR_LAUNCH, "zq_mp2_noalloc: no static scratch address for this device");
+  // profiling would create events: off for this call
+  const bool prof = ctx->profile;
+  ctx->profile = false;
+  Arena ar{ctx, base, kNoAllocBytes, 0};
+  const int rc = mp2_in(ctx, C, e, int(nocc), precision, e_corr, ar);
+  ctx->profile = prof;
+  return rc;
 }

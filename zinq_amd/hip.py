@@ -18,7 +18,7 @@ EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_l
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
             "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
-            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures"]
+            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_mp2_noalloc"]
 
 
 class GpuError(RuntimeError):
@@ -75,6 +75,7 @@ def lib():
         L.zq_fock_jk.argtypes = [vp, vp, vp, cd, vp]
         L.zq_ao2mo.argtypes = [vp, vp, sz, ci, ci, vp]
         L.zq_mp2.argtypes = [vp, vp, vp, sz, ci, vp]
+        L.zq_mp2_noalloc.argtypes = [vp, vp, vp, sz, ci, vp]
         L.zq_profile_enable.argtypes = [vp, ci]
         L.zq_profile_reset.argtypes = [vp]
         L.zq_profile_get.argtypes = [vp, ctypes.c_char_p, vp, vp]
@@ -222,6 +223,13 @@ class Context:
         C, e = _f64(C), _f64(e)
         out = np.zeros(1)
         self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
+        return float(out[0])
+
+    def mp2_noalloc(self, C, e, nocc, precision="fp64"):
+        """zq_mp2 from the library's static scratch, allocating nothing (zq_mp2_noalloc; n <= 72)."""
+        C, e = _f64(C), _f64(e)
+        out = np.zeros(1)
+        self._check(self._L.zq_mp2_noalloc(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
         return float(out[0])
 
     def scf(self, H, S, VN, nocc, diis=8, iterations=100, threshold=1e-8, generalized=False, device=False,

@@ -86,6 +86,29 @@ int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision
  * equal to the order-2 term of mp() (src/moller_plesset.zig:340-481) for an RHF reference. */
 int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr);
 
+/* ---- determinant consumers of the FULL transform ----
+ * Both run the FULL transform of the resident ERIs (FP64) on the GPU and read the spin-orbital
+ * integrals from it on the fly: spin orbital p = 2P + s for a restricted reference (the column order of
+ * ao2so_coef, src/integral_transform.zig:82-101; spatial ERIs loaded, C n x n), or the loaded spin-blocked
+ * ERIs and a 2 nbf x 2 nbf C for generalized = 1.  Determinants (at most 128 spin orbitals) are those of
+ * generateDets (src/configuration_interaction.zig:126-183); H_CI is built on the GPU with the
+ * Slater-Condon rules of slater (:323-432), dense, at most 16384 determinants.  *ndet_out (may be NULL)
+ * receives the determinant count.
+ *
+ * zq_ao2mo_pp: out = C^T A C for n x n A and C, ao2mo_pp (src/integral_transform.zig:12-34).
+ * zq_ci: computeCiStates (src/configuration_interaction.zig:597-683): the lowest min(nstates, ndet)
+ *   eigenvalues of H_CI over the excitation levels excitations[0..nexc) (1 = CIS, {1, 2} = CISD), + VN;
+ *   H is the core Hamiltonian in the basis of C (spin-blocked for generalized).
+ * zq_mp: mp (src/moller_plesset.zig:340-481): energies[0..order] of Rayleigh-Schroedinger perturbation
+ *   theory over singles .. order-fold excitations with H_MS = diag(e) - sum_i (<pi|qi> - <pi|iq>);
+ *   energies[0] is the sum of the occupied orbital energies, the total energy is E_HF + sum_{k>=2}.
+ *   Order 2 forms only row 0 of H_CI (up to 2^24 determinants); higher orders the dense H_CI. */
+int zq_ao2mo_pp(zq_ctx *ctx, size_t n, const double *A, const double *C, double *out);
+int zq_ci(zq_ctx *ctx, const double *H, const double *C, size_t nocc, int generalized, const int *excitations,
+          size_t nexc, double VN, size_t nstates, double *energies, size_t *ndet_out);
+int zq_mp(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int generalized, int order, double *energies,
+          size_t *ndet_out);
+
 /* zq_mp2 that allocates nothing.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same contract and
  * result as zq_mp2 (the FP64 Q4 product runs on the hand-written tile kernel instead of rocBLAS, so

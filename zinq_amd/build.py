@@ -12,7 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["zq_context.hip", "zq_jk.hip", "zq_transform.hip", "zq_dense.hip", "zq_eigen.hip", "zq_scf_dev.hip", "zq_group.hip"]
+HIP_SOURCES = ["zq_context.hip", "zq_jk.hip", "zq_transform.hip", "zq_dense.hip", "zq_eigen.hip", "zq_scf_dev.hip", "zq_group.hip",
+               "zq_ci.hip"]
 HOST_SOURCES = ["zq_scf.cpp", "zq_linalg.cpp"]  # plain C++ (g++), linked into libzinq_hip.so
 BUILD_DIR = os.path.join(ROOT, "_build")
 # Packed-FP32 VALU instructions are disabled for all device code (-packed-fp32-ops).

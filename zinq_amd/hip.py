@@ -18,7 +18,7 @@ EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_l
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
             "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
-            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_mp2_noalloc"]
+            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_mp2_noalloc", "zq_ao2mo_pp", "zq_ci", "zq_mp"]
 
 
 class GpuError(RuntimeError):
@@ -76,6 +76,9 @@ def lib():
         L.zq_ao2mo.argtypes = [vp, vp, sz, ci, ci, vp]
         L.zq_mp2.argtypes = [vp, vp, vp, sz, ci, vp]
         L.zq_mp2_noalloc.argtypes = [vp, vp, vp, sz, ci, vp]
+        L.zq_ao2mo_pp.argtypes = [vp, sz, vp, vp, vp]
+        L.zq_ci.argtypes = [vp, vp, vp, sz, ci, vp, sz, cd, sz, vp, vp]
+        L.zq_mp.argtypes = [vp, vp, vp, sz, ci, ci, vp, vp]
         L.zq_profile_enable.argtypes = [vp, ci]
         L.zq_profile_reset.argtypes = [vp]
         L.zq_profile_get.argtypes = [vp, ctypes.c_char_p, vp, vp]
@@ -224,6 +227,34 @@ class Context:
         out = np.zeros(1)
         self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
         return float(out[0])
+
+    def ao2mo_pp(self, A, C):
+        """C^T A C on the GPU (zq_ao2mo_pp; ao2mo_pp of src/integral_transform.zig:12-34)."""
+        A, C = _f64(A), _f64(C)
+        out = np.zeros_like(A)
+        self._check(self._L.zq_ao2mo_pp(self._h, A.shape[0], ptr(A), ptr(C), ptr(out)))
+        return out
+
+    def ci(self, H, C, nocc, excitations, VN, nstates=5, generalized=False):
+        """Lowest CI state energies over the given excitation levels (zq_ci; computeCiStates of
+        src/configuration_interaction.zig:597-683).  Returns (energies, number of determinants)."""
+        H, C = _f64(H), _f64(C)
+        exc = np.ascontiguousarray(excitations, dtype=np.int32)
+        out = np.zeros(nstates)
+        nd = ctypes.c_size_t()
+        self._check(self._L.zq_ci(self._h, ptr(H), ptr(C), nocc, int(bool(generalized)), ptr(exc), len(exc), float(VN),
+                                  nstates, ptr(out), ctypes.byref(nd)))
+        return out[:min(nstates, nd.value)], nd.value
+
+    def mp(self, C, e, nocc, order, generalized=False):
+        """energies[0..order] of the determinant-space RS perturbation theory (zq_mp; mp of
+        src/moller_plesset.zig:340-481).  Returns (energies, number of determinants)."""
+        C, e = _f64(C), _f64(e)
+        out = np.zeros(order + 1)
+        nd = ctypes.c_size_t()
+        self._check(self._L.zq_mp(self._h, ptr(C), ptr(e), nocc, int(bool(generalized)), int(order), ptr(out),
+                                  ctypes.byref(nd)))
+        return out, nd.value
 
     def mp2_noalloc(self, C, e, nocc, precision="fp64"):
         """zq_mp2 from the library's static scratch, allocating nothing (zq_mp2_noalloc; n <= 72)."""

@@ -8,8 +8,9 @@ transform + FP64 sum).  value = molecules / second over the timed steps.
 
 Per-kernel durations come from HIP events recorded on the stream the kernels run on (zq_profile_*),
 in one extra profiled step after the timed region; the roofline object is computed from them for the
-kernel with the largest total time.  cpu_baseline times the CPU oracle (test infrastructure) on a
-bounded sample and scales it to the same workload.
+kernel with the largest total time, and a second one for the transform's Q1 stage.  cpu_baseline
+times the CPU oracle (test infrastructure) end to end on the step's smallest molecule; oracle_parity
+compares the step's FP64 energies with the oracle's.
 """
 import argparse
 import json
@@ -101,37 +102,45 @@ def timed(mols, mode, args, dev, conc):
 
 def cpu_baseline(mols):
     """The CPU oracle (oracle.c: the reference's loops restated; test infrastructure) timed on the host
-    cores on a bounded sample: one getFock and one ao2mo_oovv + closed-shell MP2 of the smallest
-    molecule, scaled to the workload by n^4 x SCF iterations (Fock) and o n^4 (transform).  Two modes,
-    as the reference would run: one core (its Zig loops are serial), and the same arithmetic on every
-    core (OpenMP over independent outputs, bitwise equal).  The all-core mode is the reported value."""
+    cores: the smallest molecule of the step, RHF (DIIS-8 to 1e-10; the Fock builds on every core,
+    bitwise the faithful loops; the Jacobi eigensolver serial) + MP2 (ao2mo_oovv on every core + the
+    closed-shell sum), end to end.  value = that molecule per second.  Also returns the oracle's
+    energies, against which the GPU results of the same molecule are compared."""
     from oracle import oracle as O
     m = min(mols, key=lambda x: x["n"])
     s = m["sys"]
     g = s.coulomb()
-    P = np.zeros((m["n"], m["n"]))
-    C = np.linalg.qr(np.random.default_rng(0).standard_normal((m["n"], m["n"])))[0]
-    e = np.arange(m["n"], dtype=float)
-    modes = {}
-    for name, fock, tr in (("one_core", O.get_fock, O.ao2mo_oovv), ("all_cores", O.get_fock_par, O.ao2mo_oovv_par)):
-        t = time.time()
-        fock(g, m["H"], P)
-        t_fock = time.time() - t
-        t = time.time()
-        O.mp2_closed(tr(g, C, m["o"]), e)
-        t_tr = time.time() - t
-        total = 0.0
-        for x in mols:
-            sc4 = (x["n"] / m["n"]) ** 4
-            total += x.get("iters", 15) * t_fock * sc4 + t_tr * sc4 * (x["o"] / m["o"])
-        modes[name] = dict(value=len(mols) / total, t_fock_s=round(t_fock, 3), t_transform_s=round(t_tr, 3))
+    t = time.time()
+    r = O.scf(m["H"], m["S"], g, m["VN"], m["o"], threshold=1e-10, par=True)
+    t_scf = time.time() - t
+    e2 = O.mp2_closed(O.ao2mo_oovv_par(g, r["C"], m["o"]), r["e"])
+    total = time.time() - t
+    del g
     threads = O.max_threads()
-    return dict(value=modes["all_cores"]["value"], unit="molecules/s", cores=threads, kind="port",
-                one_core=dict(value=modes["one_core"]["value"], cores=1),
-                sample=f"oracle getFock + ao2mo_oovv/MP2 on (H2O)_{m['k']} n={m['n']} (one core: "
-                       f"{modes['one_core']['t_fock_s']}s + {modes['one_core']['t_transform_s']}s; {threads} threads: "
-                       f"{modes['all_cores']['t_fock_s']}s + {modes['all_cores']['t_transform_s']}s), scaled by "
-                       f"n^4 x SCF iterations and o*n^4 to k={[x['k'] for x in mols]}")
+    base = dict(value=1.0 / total, unit="molecules/s", cores=threads, kind="port",
+                sample=f"(H2O)_{m['k']} cc-pVDZ (n={m['n']}) RHF + MP2 end to end with the oracle on {threads} threads: "
+                       f"{total:.2f} s (SCF {t_scf:.2f} s, {r['iterations']} iterations; MP2 {total - t_scf:.2f} s); "
+                       f"value = that molecule per second (the step's other molecules take longer)")
+    return base, dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2)
+
+
+def oracle_parity(results, live):
+    """|dE| of the step's FP64 GPU results against the CPU oracle: the molecule the cpu_baseline leg ran
+    live, the others from the committed oracle energies (tests/golden/water_clusters_oracle.json, written
+    by tests/golden/make_clusters.py with the same oracle)."""
+    path = os.path.join(ROOT, "tests", "golden", "water_clusters_oracle.json")
+    fixed = json.load(open(path)) if os.path.exists(path) else {}
+    out = {}
+    for r in results:
+        if live is not None and r["k"] == live["k"]:
+            ref, src = live, "oracle run in this bench"
+        elif str(r["k"]) in fixed:
+            ref, src = fixed[str(r["k"])], "committed oracle energies"
+        else:
+            continue
+        out[f"k{r['k']}"] = dict(abs_dE_rhf=abs(r["E_rhf"] - ref["E_rhf"]), abs_dE_mp2=abs(r["E_mp2"] - ref["E_mp2"]),
+                                 source=src)
+    return out
 
 
 def sweep(k, basis):
@@ -269,10 +278,10 @@ def kernel_only(which, k, reps, basis, precision="fp64"):
         o = s.nel // 2
         r = ctx.scf(H, s.overlap(), s.nrep(), o, threshold=1e-10, device=True)
         for _ in range(reps):
-            ctx.mp2(r["C"], r["e"], o, precision)
+            e2 = ctx.mp2(r["C"], r["e"], o, precision)
         ctx.close()
         return dict(kernel="zq_q1/zq_q2/zq_gemm", n=s.nbf, o=o, precision=precision,
-                    algorithmic_flops=ovov_flops(s.nbf, o), reps=reps)
+                    algorithmic_flops=ovov_flops(s.nbf, o), reps=reps, E_corr=e2)
     P = np.eye(s.nbf) * (s.nel / s.nbf)
     for _ in range(reps):
         ctx.fock_jk(H, P)
@@ -335,6 +344,15 @@ def main():
                                              "products, FP64 sums per K block"),
                      results=res_m)
     if mixed is not None:
+        # the same mixed arithmetic converged to the plain form's threshold: precision and threshold apart
+        mixed_tight = dict(MIXED, threshold=PLAIN["threshold"])
+        el_t, res_t = timed(mols, mixed_tight, args, dev, conc)
+        de_t = [abs(a["E_mp2"] - b["E_mp2"]) for a, b in zip(res_t, results)]
+        mixed["at_plain_threshold"] = dict(threshold=PLAIN["threshold"], value=round(len(mols) * args.steps / el_t, 3),
+                                           speedup_vs_fp64=round(elapsed / el_t, 3),
+                                           ms_per_step=round(1e3 * el_t / args.steps, 2),
+                                           max_abs_dE_mp2_vs_fp64=max(de_t),
+                                           iterations=[x["iterations"] for x in res_t])
         # one profiled mixed pass on the largest molecule: the FP32-ERI incremental Fock build
         big_m = max(mols, key=lambda x: x["n"])
         c = big_m["ctx"]
@@ -380,6 +398,7 @@ def main():
                             jk_GBs=jk_bytes(m["n"]) / (jk_ms / max(jk_cnt, 1) * 1e-3) / 1e9,
                             transform_ms=t_tr * 1e3, transform_TFLOPs=ovov_flops(m["n"], m["o"]) / t_tr / 1e12,
                             q_ms={q: d[q][0] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")},
+                            q_launches={q: d[q][1] for q in ("ao2mo_q1", "ao2mo_q2", "ao2mo_q3", "ao2mo_q4")},
                             eig_ms=d["scf_eig"][0], scf_gemm_ms=d["scf_gemm"][0], sp2_ms=d["scf_sp2"][0],
                             sp2_calls=d["scf_sp2"][1], scf_wall_ms=wall[m["k"]][0], mp2_wall_ms=wall[m["k"]][1]))
         for name, (ms, cnt) in d.items():
@@ -404,6 +423,20 @@ def main():
         roof = dict(kernel="zq_q1", bound="mfma", achieved=round(achieved, 2), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=None, per_launch=f"2 o n^4 = {fl} flops at n={n}, o={o}")
 
+    # the transform's dominant stage, Q1 (matrix-core bound), for the largest molecule: algorithmic
+    # dense-chain flops 2 o n^4 per launch over its average launch time (ket-pair symmetry halves the
+    # flops it executes; its executed MFMA utilisation is in profiles/)
+    q1_ms, q1_cnt = big["q_ms"]["ao2mo_q1"], max(1, big["q_launches"]["ao2mo_q1"])
+    n_b, o_b = big["n"], big["o"]
+    fl_q1 = 2 * o_b * n_b ** 4
+    ach_q1 = fl_q1 / (q1_ms / q1_cnt * 1e-3) / 1e12
+    peak_t = PEAK_TFLOPS[args.precision]
+    roof_t = dict(kernel="zq_q1", bound="mfma", achieved=round(ach_q1, 2), peak=peak_t, unit="TFLOP/s",
+                  frac=round(ach_q1 / peak_t, 4), traffic=None,
+                  per_launch=f"2 o n^4 = {fl_q1} flops at n={n_b}, o={o_b}",
+                  transform_TFLOPs=round(big["transform_TFLOPs"], 2),
+                  transform_per_launch=f"2 n (o n^3 + o^2 n^2 + o^2 v n + o^2 v^2) = {ovov_flops(n_b, o_b)} flops")
+
     value = len(mols) * args.steps / elapsed
     out = dict(metric=METRIC, value=round(value, 3), unit="molecules/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=round(ms_per_step, 2), higher_is_better=True, scaling="weak", vs_baseline=None,
@@ -413,10 +446,12 @@ def main():
                            molecules_concurrent=conc),
                transform_TFLOPs={f"k{p['k']}": round(p["transform_TFLOPs"], 2) for p in per_mol},
                jk_GBs={f"k{p['k']}": round(p["jk_GBs"], 1) for p in per_mol},
-               roofline=roof, results=results, mixed=mixed, per_molecule=per_mol,
+               roofline=roof, roofline_transform=roof_t, results=results, mixed=mixed, per_molecule=per_mol,
                integrals_s={f"k{m['k']}": round(m["t_int"], 2) for m in mols})
+    live = None
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(mols)
+        out["cpu_baseline"], live = cpu_baseline(mols)
+    out["oracle_parity"] = oracle_parity(results, live)
     print(json.dumps(out), flush=True)
     for m in mols:
         m["ctx"].close()

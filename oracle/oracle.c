@@ -311,6 +311,7 @@ int orc_diis(int m, int n, const double *fh, const double *eh, double *F) {
  * Outputs: C, e, P, F (the last diagonalised, DIIS-extrapolated F); returns 0, or 1 for
  * ScfDidNotConverge, 2 for LapackError.  *iters_out receives the number of iterations run. */
 void orc_get_fock_packed(int n, const double *pk, const double *H, const double *P, double x, double *F);
+void orc_get_fock_par(int n, const double *g, const double *H, const double *P, int generalized, double *F);
 
 static int scf_impl(int n, int nocc, int generalized, const double *H, const double *S, const double *g, int packed,
                     double VN, int diis, int iterations, double threshold, double *E_out, double *C, double *e,
@@ -325,8 +326,10 @@ static int scf_impl(int n, int nocc, int generalized, const double *H, const dou
   double e_old = VN, e_new = VN;
   int converged = 0, it;
   for (it = 0; it < iterations; ++it) {
-    if (packed)
+    if (packed == 1)
       orc_get_fock_packed(n, g, H, P, generalized ? 1.0 : 0.5, F);
+    else if (packed == 2)
+      orc_get_fock_par(n, g, H, P, generalized, F);
     else
       orc_get_fock(n, g, H, P, generalized, F);
     e_new = orc_get_energy(n, H, F, P) + VN;
@@ -369,6 +372,13 @@ int orc_scf(int n, int nocc, int generalized, const double *H, const double *S, 
             int iterations, double threshold, double *E_out, double *C, double *e, double *P, double *F,
             int *iters_out) {
   return scf_impl(n, nocc, generalized, H, S, g, 0, VN, diis, iterations, threshold, E_out, C, e, P, F, iters_out);
+}
+
+/* orc_scf with the Fock builds on all cores (orc_get_fock_par, bitwise orc_get_fock): bitwise orc_scf. */
+int orc_scf_par(int n, int nocc, int generalized, const double *H, const double *S, const double *g, double VN, int diis,
+                int iterations, double threshold, double *E_out, double *C, double *e, double *P, double *F,
+                int *iters_out) {
+  return scf_impl(n, nocc, generalized, H, S, g, 2, VN, diis, iterations, threshold, E_out, C, e, P, F, iters_out);
 }
 
 /* orc_scf with the Fock build read from canonical packed ERIs (orc_get_fock_packed): bitwise orc_scf, for

@@ -55,6 +55,7 @@ def lib():
         L.orc_ovov_pairs_packed.argtypes = [c_int, c_int, vp, vp, c_int, vp, vp, vp]
         L.orc_get_fock_packed.argtypes = [c_int, vp, vp, vp, c_double, vp]
         L.orc_scf_packed.argtypes = L.orc_scf.argtypes
+        L.orc_scf_par.argtypes = L.orc_scf.argtypes
         L.orc_get_fock_dft.argtypes = [c_int, vp, vp, vp, c_int, c_double, vp, vp]
     return _lib
 
@@ -145,11 +146,12 @@ class ScfDidNotConverge(RuntimeError):
     pass
 
 
-def scf(H, S, g, VN, nocc, generalized=False, diis=8, iterations=100, threshold=1e-8, packed=None):
+def scf(H, S, g, VN, nocc, generalized=False, diis=8, iterations=100, threshold=1e-8, packed=None, par=False):
     """src/hartree_fock.zig:738-875 with the core guess of :425-431.  Returns dict(E, C, e, P, F, iterations).
-    With packed= (canonical packed ERIs, g ignored) the Fock builds read those: bitwise the dense run."""
+    With packed= (canonical packed ERIs, g ignored) the Fock builds read those; with par=True they run on
+    all cores.  Both are bitwise the faithful single-core run."""
     n = H.shape[0]
-    fn = lib().orc_scf
+    fn = lib().orc_scf_par if par else lib().orc_scf
     if packed is not None:
         g, fn = packed, lib().orc_scf_packed
     E = np.zeros(1)

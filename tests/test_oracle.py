@@ -271,4 +271,6 @@ def test_packed_scf_is_bitwise_the_dense_one():
     b = O.scf(H, S, None, VN, o, threshold=1e-10, packed=s.coulomb_packed())
     assert a["E"] == b["E"] and a["iterations"] == b["iterations"]
     np.testing.assert_array_equal(a["C"], b["C"])
+    c = O.scf(H, S, g, VN, o, threshold=1e-10, par=True)
+    assert c["E"] == a["E"] and np.array_equal(c["C"], a["C"])
     np.testing.assert_array_equal(O.get_fock_packed(s.coulomb_packed(), H, a["P"]), O.get_fock(g, H, a["P"]))

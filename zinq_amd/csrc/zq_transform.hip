@@ -26,7 +26,6 @@
 #include <mutex>
 #include <type_traits>
 
-#include <rocblas/rocblas.h>
 
 #include "zq_internal.h"
 
@@ -389,32 +388,34 @@ __device__ inline T gemm_b(const GemmArgs<T> &g, const T *B, int k, int n) {
   return B[k * g.sBk + n * g.sBn];
 }
 
-// FP64 / FP32 / FP32C: 64 x 64 tiles, BK = 16, 4 waves of 32 x 32 (2 x 2 MFMA tiles).  The next K tile
-// is loaded into registers while the current one is multiplied out of a double-buffered LDS tile, so
-// each K step waits on one barrier and no global-load round trip.
-template <class MF>
+// FP64 / FP32 / FP32C: (32 TM) x (32 TN) tiles, BK = 16, 4 waves of (16 TM) x (16 TN) (TM x TN MFMA
+// tiles each, so each LDS operand read feeds TN or TM MFMAs).  The next K tile is loaded into registers
+// while the current one is multiplied out of a double-buffered LDS tile, so each K step waits on one
+// barrier and no global-load round trip.
+template <class MF, int TM, int TN>
 __device__ inline void gemm_load(const GemmArgs<typename MF::T> &g, const typename MF::T *A, const typename MF::T *B,
-                                 int m0, int n0, int k0, bool a_kfast, bool b_nfast, typename MF::T ra[4],
-                                 typename MF::T rb[4]) {
+                                 int m0, int n0, int k0, bool a_kfast, bool b_nfast, typename MF::T (&ra)[2 * TM],
+                                 typename MF::T (&rb)[2 * TN]) {
   typedef typename MF::T T;
+  constexpr int WM = 32 * TM, WN = 32 * TN;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < 2 * TM; ++j) {
     const int e = tid + 256 * j;
     int mm, kk;
     if (a_kfast)
       mm = e >> 4, kk = e & 15;
     else
-      kk = e >> 6, mm = e & 63;
+      kk = e / WM, mm = e % WM;
     const int m = m0 + mm, k = k0 + kk;
     ra[j] = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : T(0);
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < 2 * TN; ++j) {
     const int e = tid + 256 * j;
     int nn, kk;
     if (b_nfast)
-      kk = e >> 6, nn = e & 63;
+      kk = e / WN, nn = e % WN;
     else
       nn = e >> 4, kk = e & 15;
     const int n = n0 + nn, k = k0 + kk;
@@ -422,83 +423,90 @@ __device__ inline void gemm_load(const GemmArgs<typename MF::T> &g, const typena
   }
 }
 
-template <class T>
-__device__ inline void gemm_stage(T (*As)[65], T (*Bs)[65], const T ra[4], const T rb[4], bool a_kfast, bool b_nfast) {
+template <class T, int TM, int TN>
+__device__ inline void gemm_stage(T (*As)[32 * TM + 1], T (*Bs)[32 * TN + 1], const T (&ra)[2 * TM],
+                                  const T (&rb)[2 * TN], bool a_kfast, bool b_nfast) {
+  constexpr int WM = 32 * TM, WN = 32 * TN;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < 2 * TM; ++j) {
     const int e = tid + 256 * j;
     if (a_kfast)
       As[e & 15][e >> 4] = ra[j];
     else
-      As[e >> 6][e & 63] = ra[j];
+      As[e / WM][e % WM] = ra[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 2 * TN; ++j) {
+    const int e = tid + 256 * j;
     if (b_nfast)
-      Bs[e >> 6][e & 63] = rb[j];
+      Bs[e / WN][e % WN] = rb[j];
     else
       Bs[e & 15][e >> 4] = rb[j];
   }
 }
 
-template <class MF>
+template <class MF, int TM = 2, int TN = 2>
 __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
   typedef typename MF::T T;
-  __shared__ T As[2][16][65];
-  __shared__ T Bs[2][16][65];
+  constexpr int WM = 32 * TM, WN = 32 * TN;
+  __shared__ T As[2][16][WM + 1];
+  __shared__ T Bs[2][16][WN + 1];
   const int b = blockIdx.z;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = blockIdx.y * WM, n0 = blockIdx.x * WN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
   const T *A = g.A + b * g.sAb;
   const T *B = g.B + b * g.sBb;
-  typename MF::acc acc[2][2];
+  typename MF::acc acc[TM][TN];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) acc[x][y] = MF::zero();
+    for (int y = 0; y < TN; ++y) acc[x][y] = MF::zero();
   const bool a_kfast = (g.sAk == 1);
   const bool b_nfast = (g.mode == 1) || (g.sBn == 1);
   // 16 x 16 sub-tiles wholly outside M or N (o = 40 rows in a 64-row tile, v = 152 columns in three
   // 64-column tiles) take no MFMA: the test is uniform over the wave, and the SIMD time goes to others
-  bool live[2][2];
+  bool live[TM][TN];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) live[x][y] = m0 + wm + 16 * x < g.M && n0 + wn + 16 * y < g.N;
-  T ra[4], rb[4];
-  gemm_load<MF>(g, A, B, m0, n0, 0, a_kfast, b_nfast, ra, rb);
-  gemm_stage<T>(As[0], Bs[0], ra, rb, a_kfast, b_nfast);
+    for (int y = 0; y < TN; ++y) live[x][y] = m0 + wm + 16 * x < g.M && n0 + wn + 16 * y < g.N;
+  T ra[2 * TM], rb[2 * TN];
+  gemm_load<MF, TM, TN>(g, A, B, m0, n0, 0, a_kfast, b_nfast, ra, rb);
+  gemm_stage<T, TM, TN>(As[0], Bs[0], ra, rb, a_kfast, b_nfast);
   __syncthreads();
   int cur = 0;
   for (int k0 = 0; k0 < g.K; k0 += 16) {
     const bool more = k0 + 16 < g.K;
-    if (more) gemm_load<MF>(g, A, B, m0, n0, k0 + 16, a_kfast, b_nfast, ra, rb);
+    if (more) gemm_load<MF, TM, TN>(g, A, B, m0, n0, k0 + 16, a_kfast, b_nfast, ra, rb);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int kk = ks * 4 + (lane >> 4);
-      T a[2], bb[2];
+      T a[TM], bb[TN];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) a[x] = As[cur][kk][wm + x * 16 + (lane & 15)];
+      for (int x = 0; x < TM; ++x) a[x] = As[cur][kk][wm + x * 16 + (lane & 15)];
 #pragma unroll
-      for (int y = 0; y < 2; ++y) bb[y] = Bs[cur][kk][wn + y * 16 + (lane & 15)];
+      for (int y = 0; y < TN; ++y) bb[y] = Bs[cur][kk][wn + y * 16 + (lane & 15)];
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int x = 0; x < TM; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+        for (int y = 0; y < TN; ++y)
           if (live[x][y]) acc[x][y] = MF::mma(a[x], bb[y], acc[x][y]);
     }
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < TM; ++x)
 #pragma unroll
-      for (int y = 0; y < 2; ++y) MF::flush(acc[x][y]);
-    if (more) gemm_stage<T>(As[cur ^ 1], Bs[cur ^ 1], ra, rb, a_kfast, b_nfast);
+      for (int y = 0; y < TN; ++y) MF::flush(acc[x][y]);
+    if (more) gemm_stage<T, TM, TN>(As[cur ^ 1], Bs[cur ^ 1], ra, rb, a_kfast, b_nfast);
     __syncthreads();
     cur ^= 1;
   }
   T *C = g.C + b * g.sCb;
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
+    for (int y = 0; y < TN; ++y) {
       const int n = n0 + wn + y * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -587,36 +595,63 @@ __global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
 }
 
 // ---------------------------------------------------------------- MP2 energy (FP64 arithmetic)
-// One workgroup per (i,j): sum_ab T[i,j,a,b] (2 T[i,j,a,b] - T[i,j,b,a]) / (e_i + e_j - e_a - e_b)
-// in a fixed order; partial[i*o + j].
+// One workgroup per (i,j): sum_ab T[a][b] (2 T[a][b] - T[b][a]) / (e_i + e_j - e_a - e_b).  The v x v
+// block is taken in 32 x 32 tile pairs (A <= B): tiles T[A][B] and T[B][A] are staged through LDS along
+// their rows, and each (a,b), a < b, contributes with its mirror 2 (x^2 - x y + y^2) / d (x = T[a][b],
+// y = T[b][a]), a diagonal element x^2 / d.  Every thread accumulates its positions in a fixed order and
+// the threads are summed in a fixed tree: partial[i*o + j], bitwise reproducible.
 template <class T>
 __global__ __launch_bounds__(256) void zq_mp2_pairs(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
                                                     double *__restrict__ partial) {
+  __shared__ double X[32][33], Yt[32][33];
+  __shared__ double red[256];
   const int ij = blockIdx.x, i = ij / o, j = ij % o;
   const T *t = t4 + size_t(ij) * v * v;
   const double eij = e[i] + e[j];
+  const double *ev = e + o;
+  const int tid = threadIdx.x, c = tid & 31, r0 = tid >> 5;  // r0 = 0..7
+  const int nt = (v + 31) / 32;
   double s = 0.0;
-  for (int ab = threadIdx.x; ab < v * v; ab += 256) {
-    const int a = ab / v, b = ab % v;
-    const double x = double(t[ab]), y = double(t[b * v + a]);
-    s += x * (2.0 * x - y) / (eij - e[o + a] - e[o + b]);
-  }
-  __shared__ double red[256];
-  red[threadIdx.x] = s;
+  for (int A = 0; A < nt; ++A)
+    for (int B = A; B < nt; ++B) {
+      __syncthreads();  // the previous pair's tiles are consumed
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r = r0 + 8 * k;
+        const int a = A * 32 + r, b = B * 32 + c, b2 = B * 32 + r, a2 = A * 32 + c;
+        X[r][c] = (a < v && b < v) ? double(t[size_t(a) * v + b]) : 0.0;    // T[A*32 + r][B*32 + c]
+        Yt[r][c] = (b2 < v && a2 < v) ? double(t[size_t(b2) * v + a2]) : 0.0;  // T[B*32 + r][A*32 + c]
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r = r0 + 8 * k;
+        const int a = A * 32 + r, b = B * 32 + c;
+        if (a >= v || b >= v) continue;
+        const double x = X[r][c], y = Yt[c][r];  // T[a][b], T[b][a]
+        const double d = eij - ev[a] - ev[b];
+        if (A < B || r < c)
+          s += 2.0 * (x * x - x * y + y * y) / d;
+        else if (r == c)
+          s += x * x / d;
+      }
+    }
+  red[tid] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[ij] = red[0];
+  if (tid == 0) partial[ij] = red[0];
 }
 
-__global__ void zq_sum_serial(const double *__restrict__ x, int n, double *__restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += x[i];
-    *out = s;
-  }
+// *out = sum of x[0..n): lane l adds x[l], x[l + 64], ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(64) void zq_sum_wave(const double *__restrict__ x, int n, double *__restrict__ out) {
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += 64) s += x[k];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void zq_widen(const float *__restrict__ x, size_t n, double *__restrict__ y) {
@@ -631,12 +666,13 @@ GemmArgs<T> batch_slice(GemmArgs<T> g, long long b0) {
   return g;
 }
 
-template <class MF>
+template <class MF, int TM = 2, int TN = 2>
 int launch_gemm(zq_ctx *ctx, const GemmArgs<typename MF::T> &g, int batch, const char *name) {
   zq_prof_begin(ctx, name);
   for (int b0 = 0; b0 < batch; b0 += kMaxGridZ) {
-    const dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(std::min(kMaxGridZ, batch - b0)));
-    hipLaunchKernelGGL(zq_gemm<MF>, grid, dim3(256), 0, ctx->stream, batch_slice(g, b0));
+    const dim3 grid(unsigned((g.N + 32 * TN - 1) / (32 * TN)), unsigned((g.M + 32 * TM - 1) / (32 * TM)),
+                    unsigned(std::min(kMaxGridZ, batch - b0)));
+    hipLaunchKernelGGL((zq_gemm<MF, TM, TN>), grid, dim3(256), 0, ctx->stream, batch_slice(g, b0));
   }
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, name);
@@ -867,28 +903,7 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
   }
   if (int rc = launch_gemm<MF>(ctx, g2, m1, "ao2mo_q2")) return rc;
   if (int rc = launch_gemm<MF>(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
-  if constexpr (std::is_same<MF, MF64>::value) {
-    if (fixed) {
-      // no library call in the no-allocation path: the tile kernel
-      if (int rc = launch_gemm<MF>(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
-      *d_out = O;
-      return ZQ_OK;
-    }
-    // FP64 Q4 is a plain strided-batched product with W_ij read transposed: rocBLAS DGEMM (column-major,
-    // so each row-major matrix is read as its transpose), atomics off, measured faster than the tile
-    // kernel above for this shape.
-    auto h = (rocblas_handle)zq_blas(ctx);
-    if (!h) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocblas handle");
-    const double one = 1.0, zero = 0.0;
-    zq_prof_begin(ctx, "ao2mo_q4");
-    const rocblas_status st =
-        rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m4, m3, np, &one, d3, m4, 0, W,
-                                      m3, rocblas_stride(np) * m3, &zero, O, m4, rocblas_stride(m3) * m4, m1 * m2);
-    zq_prof_end(ctx);
-    if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAUNCH, "ao2mo_q4: rocblas_dgemm_strided_batched");
-    *d_out = O;
-    return ZQ_OK;
-  }
+  // Q4 on the tile kernel in every precision (no library GEMM in the transform)
   if (int rc = launch_gemm<MF>(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
   *d_out = O;
   return ZQ_OK;
@@ -993,7 +1008,7 @@ int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precisio
   else
     hipLaunchKernelGGL(zq_mp2_pairs<double>, dim3(unsigned(o * o)), dim3(256), 0, ctx->stream, (const double *)T, de, o, v,
                        part);
-  hipLaunchKernelGGL(zq_sum_serial, dim3(1), dim3(64), 0, ctx->stream, part, o * o, part + size_t(o) * o);
+  hipLaunchKernelGGL(zq_sum_wave, dim3(1), dim3(64), 0, ctx->stream, part, o * o, part + size_t(o) * o);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_mp2_pairs");
   ZQ_TRY(ctx, hipMemcpyAsync(e_corr, part + size_t(o) * o, 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--kmax", type=int, default=10)
     ap.add_argument("--kernel", default="", choices=["", "jk", "mp2", "copy"], help="single-kernel run for counter passes")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the short batched (512 molecules) and serving (k = 1..10, 100 req/s) runs of the default line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: only the one-GPU path is built; a multi-GPU run is not supported.", file=sys.stderr)
@@ -452,6 +454,12 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"], live = cpu_baseline(mols)
     out["oracle_parity"] = oracle_parity(results, live)
+    if not args.no_extras:
+        # the batched and serving workloads, shortened, so that every default run reports them too
+        b = batch(512, 2, 1)
+        out["batch"] = {k: b[k] for k in ("value", "unit", "steps", "ms_per_step", "status_ok", "config", "integrals_s")}
+        sv = serve([100.0], 8, 10)
+        out["serve"] = {k: sv[k] for k in ("serve", "slots", "unit", "workload", "max_abs_dE_mp2_vs_fp64", "resident_sets_gb")}
     print(json.dumps(out), flush=True)
     for m in mols:
         m["ctx"].close()

@@ -48,6 +48,7 @@ enum { ZQ_BLOCK_FULL = 0, ZQ_BLOCK_OVOV = 1 };
 enum { ZQ_PREC_FP64 = 0, ZQ_PREC_FP32 = 1, ZQ_PREC_BF16X3 = 2, ZQ_PREC_FP32C = 3 };
 
 typedef struct zq_ctx zq_ctx;
+typedef struct zq_eri_set zq_eri_set; /* resident ERIs shared by contexts, see zq_eri_set_create */
 
 /* Context on HIP device `device` (its own stream).  NULL on failure. */
 zq_ctx *zq_create(int device);
@@ -73,6 +74,14 @@ int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double x_exch, dou
  * (src/hartree_fock.zig:701-719) is zq_fock_jk with x_exch = exx / 2. */
 int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, size_t ndens, double x_exch, double *F_out);
 
+/* The two-electron part of the HF nuclear gradient, g_G of src/hartree_fock.zig:263-273: for each of
+ * ncoord derivative ERI sets (dg_packed: ncoord consecutive canonical packed arrays of the
+ * zq_eri_load_packed layout, each the derivative of (pq|rs) with respect to one nuclear coordinate),
+ *   out[x] = sum_pqrs 1/2 P[p,q] P[r,s] ((pq|rs)_x - x_exch (pr|qs)_x),  x_exch = exx (1/2 RHF, 1 GHF),
+ * one J/K pass over each set on the GPU (a private context holds the set; ctx's own ERIs stay loaded). */
+int zq_grad_eri(zq_ctx *ctx, const double *dg_packed, size_t n, size_t ncoord, const double *P, double x_exch,
+                double *out);
+
 /* AO->MO transform of the loaded ERIs with MO coefficients C (n x n, columns = MOs).
  *   ZQ_BLOCK_FULL: out[p,q,r,s] = <pq|rs> for all MOs, the result of ao2mo_pppp
  *                  (src/integral_transform.zig:37-79); out has n^4 entries.
@@ -80,6 +89,16 @@ int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, size_t ndens
  *                  (src/integral_transform.zig:165-219); out has nocc^2 (n-nocc)^2 entries.
  * precision: ZQ_PREC_FP64 | ZQ_PREC_FP32 | ZQ_PREC_BF16X3 | ZQ_PREC_FP32C (quarter transforms only). */
 int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out);
+
+/* The dual-number transform of mp's gradient path (src/moller_plesset.zig:304-333, ao2mo_pppp over
+ * ScalarDual values): the transform of the ERI dual g + eps dg with the coefficient dual C + eps dC,
+ *   out  = T(g; C, C, C, C),
+ *   dout = T(dg; C, C, C, C) + T(g; dC, C, C, C) + T(g; C, dC, C, C) + T(g; C, C, dC, C) + T(g; C, C, C, dC),
+ * T(x; A, B, A', B')[i,j,a,b] = sum A[mu,i] B[lam,j] A'[nu,a] B'[sig,b] <mu lam|nu sig>_x, in FP64 on the GPU.
+ * g is ctx's resident ERIs; dg the quartets of dset (an ERI set of the same size, e.g. the derivative with
+ * respect to one nuclear coordinate).  block, nocc and the output layout are those of zq_ao2mo. */
+int zq_ao2mo_dual(zq_ctx *ctx, zq_eri_set *dset, const double *C, const double *dC, size_t nocc, int block,
+                  double *out, double *dout);
 
 /* Closed-shell MP2 correlation energy from the OVOV transform, FP64 denominators and summation:
  *   E2 = sum_ijab (ia|jb) [2 (ia|jb) - (ib|ja)] / (e_i + e_j - e_a - e_b),
@@ -140,11 +159,14 @@ size_t zq_device_bytes(const zq_ctx *ctx);
  * set's HBM footprint (the quartets and their FP32 copy; the plan adds under 1%).  zq_attach points
  * ctx at the set (no copy; the set must outlive the attachment,
  * which ends with the next zq_attach, zq_eri_load* or zq_destroy of ctx). */
-typedef struct zq_eri_set zq_eri_set;
 zq_eri_set *zq_eri_set_create(int device, const double *g_packed, size_t n, int with_fp32);
 void zq_eri_set_destroy(zq_eri_set *s);
 size_t zq_eri_set_bytes(const zq_eri_set *s);
 int zq_attach(zq_ctx *ctx, zq_eri_set *s);
+
+/* J/K records (64 doubles each) a context keeps for a molecule of n basis functions: the size of its
+ * grow-only record buffer, for admission control (host arithmetic, no device needed). */
+size_t zq_jk_record_count(size_t n);
 
 /* Free and total HBM of `device` in bytes (for admission control). */
 int zq_mem_info(int device, size_t *free_bytes, size_t *total_bytes);

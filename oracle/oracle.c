@@ -273,6 +273,26 @@ void orc_get_fock_dft(int n, const double *g, const double *H, const double *P, 
     }
 }
 
+/* The two-electron part of the HF nuclear gradient, src/hartree_fock.zig:263-273: for each coordinate
+ * x < ncoord of the derivative tensor dg (ncoord dense n^4 blocks, the layout of ints.dg),
+ * out[x] = sum_pqrs 0.5 P[p,q] P[r,s] (dg[x,p,r,q,s] - exx x_f dg[x,p,q,r,s]), x_f = 1/2 RHF, 1 GHF. */
+void orc_grad_eri(int n, int ncoord, const double *dg, const double *P, int generalized, double exx, double *out) {
+  const double exch_factor = generalized ? 1.0 : 0.5;
+  const size_t n4 = (size_t)n * n * n * n;
+  for (int x = 0; x < ncoord; ++x) {
+    const double *d = dg + (size_t)x * n4;
+    double g_G = 0;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q)
+        for (int r = 0; r < n; ++r)
+          for (int s = 0; s < n; ++s) {
+            const double dg1 = AT4(d, n, p, r, q, s), dg2 = AT4(d, n, p, q, r, s);
+            g_G += 0.5 * AT2(P, n, p, q) * AT2(P, n, r, s) * (dg1 - exx * exch_factor * dg2);
+          }
+    out[x] = g_G;
+  }
+}
+
 /* diis, src/hartree_fock.zig:176-227.  fh/eh: m matrices of n*n.  Returns 1 if the LU solve fails. */
 int orc_diis(int m, int n, const double *fh, const double *eh, double *F) {
   if (m < 2) return 0;
@@ -428,6 +448,60 @@ void orc_ao2mo_pppp(int N, const double *g, const double *C, double *out) {
         for (int sg = 0; sg < N; ++sg)
           for (int b = 0; b < N; ++b) AT4(out, N, i, j, a, b) += AT2(C, N, sg, b) * AT4(t3, N, i, j, a, sg);
   free(t1), free(t2), free(t3);
+}
+
+/* ao2mo_pppp over ScalarDual values (src/integral_transform.zig:37-79 with T = ScalarDual, as
+ * src/moller_plesset.zig:304-333 runs it): (g, dg) and (C, dC) are the value and derivative parts;
+ * every += a * b is the dual product (val a.val b.val, der a.val b.der + b.val a.der, src/dual.zig:51-53)
+ * and the dual sum. */
+void orc_ao2mo_pppp_dual(int N, const double *g, const double *dg, const double *C, const double *dC, double *out,
+                         double *dout) {
+  const size_t n4 = (size_t)N * N * N * N;
+  double *t1 = calloc(n4, sizeof(double)), *t2 = calloc(n4, sizeof(double)), *t3 = calloc(n4, sizeof(double));
+  double *d1 = calloc(n4, sizeof(double)), *d2 = calloc(n4, sizeof(double)), *d3 = calloc(n4, sizeof(double));
+  memset(out, 0, sizeof(double) * n4);
+  memset(dout, 0, sizeof(double) * n4);
+  for (int i = 0; i < N; ++i)
+    for (int lam = 0; lam < N; ++lam)
+      for (int nu = 0; nu < N; ++nu)
+        for (int mu = 0; mu < N; ++mu)
+          for (int sg = 0; sg < N; ++sg) {
+            const double a = AT2(C, N, mu, i), da = AT2(dC, N, mu, i), b = AT4(g, N, mu, lam, nu, sg),
+                         db = AT4(dg, N, mu, lam, nu, sg);
+            AT4(t1, N, i, lam, nu, sg) += a * b;
+            AT4(d1, N, i, lam, nu, sg) += a * db + b * da;
+          }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      for (int nu = 0; nu < N; ++nu)
+        for (int lam = 0; lam < N; ++lam)
+          for (int sg = 0; sg < N; ++sg) {
+            const double a = AT2(C, N, lam, j), da = AT2(dC, N, lam, j), b = AT4(t1, N, i, lam, nu, sg),
+                         db = AT4(d1, N, i, lam, nu, sg);
+            AT4(t2, N, i, j, nu, sg) += a * b;
+            AT4(d2, N, i, j, nu, sg) += a * db + b * da;
+          }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      for (int a_ = 0; a_ < N; ++a_)
+        for (int nu = 0; nu < N; ++nu)
+          for (int sg = 0; sg < N; ++sg) {
+            const double a = AT2(C, N, nu, a_), da = AT2(dC, N, nu, a_), b = AT4(t2, N, i, j, nu, sg),
+                         db = AT4(d2, N, i, j, nu, sg);
+            AT4(t3, N, i, j, a_, sg) += a * b;
+            AT4(d3, N, i, j, a_, sg) += a * db + b * da;
+          }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      for (int a_ = 0; a_ < N; ++a_)
+        for (int sg = 0; sg < N; ++sg)
+          for (int b_ = 0; b_ < N; ++b_) {
+            const double a = AT2(C, N, sg, b_), da = AT2(dC, N, sg, b_), b = AT4(t3, N, i, j, a_, sg),
+                         db = AT4(d3, N, i, j, a_, sg);
+            AT4(out, N, i, j, a_, b_) += a * b;
+            AT4(dout, N, i, j, a_, b_) += a * db + b * da;
+          }
+  free(t1), free(t2), free(t3), free(d1), free(d2), free(d3);
 }
 
 /* ao2mo_oovv, src/integral_transform.zig:165-219: out[i,j,a,b] = <ij|ab> = (ia|jb), i,j occupied,

@@ -56,6 +56,8 @@ def lib():
         L.orc_get_fock_packed.argtypes = [c_int, vp, vp, vp, c_double, vp]
         L.orc_scf_packed.argtypes = L.orc_scf.argtypes
         L.orc_scf_par.argtypes = L.orc_scf.argtypes
+        L.orc_ao2mo_pppp_dual.argtypes = [c_int, vp, vp, vp, vp, vp, vp]
+        L.orc_grad_eri.argtypes = [c_int, c_int, vp, vp, c_int, c_double, vp]
         L.orc_get_fock_dft.argtypes = [c_int, vp, vp, vp, c_int, c_double, vp, vp]
     return _lib
 
@@ -194,6 +196,15 @@ def ovov_pairs_packed(packed, C, nocc, pairs):
     return out
 
 
+def grad_eri(dg, P, generalized=False, exx=1.0):
+    """Two-electron part of the HF gradient for a stack of dense derivative tensors dg[x]."""
+    dg = _c(dg)
+    ncoord, n = dg.shape[0], dg.shape[1]
+    out = np.zeros(ncoord)
+    lib().orc_grad_eri(n, ncoord, _p(dg), _p(_c(P)), int(generalized), float(exx), _p(out))
+    return out
+
+
 def get_fock_dft(g, H, P, exx, Vxc=None, generalized=False):
     """getFock's DFT branch (Coulomb loop, exx-scaled exchange loop, + Vxc, symmetrised)."""
     n = H.shape[0]
@@ -215,6 +226,14 @@ def ao2mo_pppp(g, C):
     out = np.zeros((N,) * 4)
     lib().orc_ao2mo_pppp(N, _p(_c(g)), _p(_c(C)), _p(out))
     return out
+
+
+def ao2mo_pppp_dual(g, dg, C, dC):
+    """ao2mo_pppp over dual numbers: (value, derivative) of the transform of g + eps dg with C + eps dC."""
+    N = C.shape[0]
+    out, dout = np.zeros((N,) * 4), np.zeros((N,) * 4)
+    lib().orc_ao2mo_pppp_dual(N, _p(_c(g)), _p(_c(dg)), _p(_c(C)), _p(_c(dC)), _p(out), _p(dout))
+    return out, dout
 
 
 def ao2mo_oovv(g, C, nocc):

@@ -130,3 +130,61 @@ def test_dft_branch_coulomb_and_scaled_exchange(ctx):
         F = ctx.fock_jk(H + V, P, 0.5 * exx)
         Fo = O.get_fock_dft(g, H, P, exx, V)
         assert np.abs(F - Fo).max() < 1e-11, exx
+
+
+def _derivative_eris(s, basis, h=1e-4):
+    """Central differences (step h bohr) of the ERIs with respect to every nuclear coordinate: dense
+    (for the oracle) and canonical packed (for the GPU) stacks.  Both contractions read the same numbers,
+    so the comparison is of the contraction alone."""
+    from zinq_amd.system import MolecularSystem
+    dense, packed = [], []
+    for x in range(3 * len(s.atoms)):
+        xyz = s.coors.copy()
+        xyz[x] += h
+        sp = MolecularSystem.from_atoms(s.atoms, xyz, basis)
+        xyz[x] -= 2 * h
+        sm = MolecularSystem.from_atoms(s.atoms, xyz, basis)
+        dense.append((sp.coulomb() - sm.coulomb()) / (2 * h))
+        packed.append((sp.coulomb_packed() - sm.coulomb_packed()) / (2 * h))
+    return np.stack(dense), np.stack(packed)
+
+
+@pytest.mark.parametrize("basis,kind", [("builtin:sto-3g", "sto3g"), ("builtin:cc-pvdz", "ccpvdz")])
+def test_gradient_eri_contraction(ctx, basis, kind):
+    """The two-electron part of the HF nuclear gradient (src/hartree_fock.zig:263-273) for the 9 water
+    coordinates, RHF exchange (exx 1) and a hybrid-like exx = 0.2: against the oracle to 1e-10."""
+    s, H, g, VN, o, r = _rhf(ctx, kind)
+    dgd, dgp = _derivative_eris(s, basis)
+    for exx in (1.0, 0.2):
+        out = ctx.grad_eri(dgp, s.nbf, r["P"], 0.5 * exx)
+        ref = O.grad_eri(dgd, r["P"], exx=exx)
+        print(f"{kind} exx={exx}: max|d g_G| = {np.abs(out - ref).max():.2e} (|g_G| <= {np.abs(ref).max():.2e})", flush=True)
+        np.testing.assert_allclose(out, ref, rtol=0, atol=1e-10)
+    # the context's own ERIs stay loaded
+    assert np.abs(ctx.fock_jk(H, r["P"]) - O.get_fock(g, H, r["P"])).max() < 1e-11
+
+
+@pytest.mark.parametrize("kind,basis", [("sto3g", "builtin:sto-3g"), ("ccpvdz", "builtin:cc-pvdz")])
+def test_dual_number_transform(ctx, kind, basis):
+    """The transform over dual numbers of mp's gradient path (src/moller_plesset.zig:304-333): ERIs + eps
+    (their derivative with respect to one nuclear coordinate), C + eps dC.  FULL and OVOV blocks against
+    the oracle's dual ao2mo_pppp (value 1e-11; derivative 1e-10 relative to its largest element)."""
+    from zinq_amd import hip
+    s, H, g, VN, o, r = _rhf(ctx, kind)
+    dgd, dgp = _derivative_eris(s, basis)
+    x = 2  # the z coordinate of the oxygen
+    dC = np.random.default_rng(31).standard_normal(r["C"].shape) * 0.1
+    ref, dref = O.ao2mo_pppp_dual(g, dgd[x], r["C"], dC)
+    dset = hip.EriSet(dgp[x], s.nbf, 0, False)
+    try:
+        out, dout = ctx.ao2mo_dual(dset, r["C"], dC)
+        scale = max(1.0, np.abs(dref).max())
+        print(f"{kind} FULL: max|d out| {np.abs(out - ref).max():.1e}, max|d dout| {np.abs(dout - dref).max():.1e} "
+              f"(|dout| <= {np.abs(dref).max():.1e})", flush=True)
+        assert np.abs(out - ref).max() < 1e-11
+        assert np.abs(dout - dref).max() < 1e-10 * scale
+        out2, dout2 = ctx.ao2mo_dual(dset, r["C"], dC, o, "ovov")
+        assert np.abs(out2 - ref[:o, :o, o:, o:]).max() < 1e-11
+        assert np.abs(dout2 - dref[:o, :o, o:, o:]).max() < 1e-10 * scale
+    finally:
+        dset.close()

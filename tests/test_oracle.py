@@ -274,3 +274,40 @@ def test_packed_scf_is_bitwise_the_dense_one():
     c = O.scf(H, S, g, VN, o, threshold=1e-10, par=True)
     assert c["E"] == a["E"] and np.array_equal(c["C"], a["C"])
     np.testing.assert_array_equal(O.get_fock_packed(s.coulomb_packed(), H, a["P"]), O.get_fock(g, H, a["P"]))
+
+
+def test_gradient_eri_contraction(rhf):
+    """The gradient's two-electron contraction (src/hartree_fock.zig:263-273) taken on g itself is the
+    two-electron energy 1/2 sum P (F - H) of getFock/getEnergy (a transposed index in either term would
+    give an exchange-like value instead); it is linear in the tensor and scales exchange by exx."""
+    s, H, S, g, VN, r = rhf
+    P = r["P"]
+    F = O.get_fock(g, H, P)
+    e2 = 0.5 * np.sum(P * (F - H))
+    assert O.grad_eri(g[None], P)[0] == pytest.approx(e2, abs=1e-12)
+    dg = np.stack([g, 2.0 * g - 0.5 * g.transpose(2, 3, 0, 1)])
+    out = O.grad_eri(dg, P)
+    assert out[1] == pytest.approx(1.5 * e2, abs=1e-11)  # <rs|pq> = <pq|rs>
+    Fj = O.get_fock_dft(g, np.zeros_like(H), P, 0.0)  # H + J only
+    ej = 0.5 * np.sum(P * Fj)
+    assert O.grad_eri(g[None], P, exx=0.0)[0] == pytest.approx(ej, abs=1e-12)
+
+
+def test_dual_transform(rhf):
+    """ao2mo_pppp over dual numbers: the value part is ao2mo_pppp bit for bit; the derivative part is the
+    directional derivative of the transform, checked against a central difference (the transform is
+    quartic in C and linear in g, so the difference error is O(h^2)); zero tangents give zero."""
+    s, H, S, g, VN, r = rhf
+    rng = np.random.default_rng(23)
+    dg = rng.standard_normal(g.shape)
+    dg = dg + dg.transpose(1, 0, 3, 2)
+    dg = dg + dg.transpose(2, 3, 0, 1)
+    dg = dg + dg.transpose(2, 1, 0, 3)  # <pq|rs> symmetries
+    dC = rng.standard_normal(r["C"].shape)
+    out, dout = O.ao2mo_pppp_dual(g, dg, r["C"], dC)
+    np.testing.assert_array_equal(out, O.ao2mo_pppp(g, r["C"]))
+    h = 1e-4
+    fd = (O.ao2mo_pppp(g + h * dg, r["C"] + h * dC) - O.ao2mo_pppp(g - h * dg, r["C"] - h * dC)) / (2 * h)
+    assert np.abs(dout - fd).max() < 1e-6 * max(1.0, np.abs(dout).max())
+    z = O.ao2mo_pppp_dual(g, np.zeros_like(g), r["C"], np.zeros_like(dC))[1]
+    assert not z.any()

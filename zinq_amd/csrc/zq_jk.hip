@@ -16,6 +16,8 @@
 // output block, its records in a fixed order (a host-built CSR list): no atomics, bitwise reproducible.
 
 #include <algorithm>
+#include <string>
+#include <vector>
 
 #include "zq_internal.h"
 
@@ -632,4 +634,60 @@ extern "C" int zq_fock_jk(zq_ctx *ctx, const double *H, const double *P, double 
   ZQ_TRY(ctx, hipMemcpyAsync(F_out, ctx->d_F, size_t(n) * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return ZQ_OK;
+}
+
+extern "C" int zq_grad_eri(zq_ctx *ctx, const double *dg_packed, size_t n, size_t ncoord, const double *P, double x_exch,
+                           double *out) {
+  ZqGate _gate;
+  if (!ctx || !dg_packed || !P || !out || n == 0 || n > 4096 || ncoord == 0)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_grad_eri: bad arguments");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  // a private context on the same device holds each coordinate's derivative ERIs in turn, so ctx's own
+  // resident ERIs, plan and buffers stay as they are
+  zq_ctx *w = zq_create(ctx->device);
+  if (!w) return zq_fail(ctx, ZQ_ERR_OOM, "zq_grad_eri: work context");
+  const size_t np = n * (n + 1) / 2, nu = np * (np + 1) / 2;
+  int rc = ZQ_OK;
+  std::vector<double> part;
+  for (size_t x = 0; x < ncoord && rc == ZQ_OK; ++x) {
+    rc = zq_eri_load_packed(w, dg_packed + x * nu, n);
+    if (rc) break;
+    auto *epart = (double *)zq_scratch(w, "grad_epart", size_t(w->Nb) * sizeof(double), nullptr, false);
+    if (!epart) {
+      rc = zq_fail(w, ZQ_ERR_OOM, "zq_grad_eri: energy partials");
+      break;
+    }
+    // 1/2 sum P (Hc + F) with Hc = 0 and F = G(P) of this set: the reduce's energy partials
+    hipError_t e = hipMemsetAsync(w->d_H, 0, n * n * sizeof(double), w->stream);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(w->d_P, w->npad * sizeof(double), P, n * sizeof(double), n * sizeof(double), n,
+                           hipMemcpyHostToDevice, w->stream);
+    if (e != hipSuccess) {
+      rc = zq_check(w, e, "zq_grad_eri: staging");
+      break;
+    }
+    rc = zq_fock_jk_device(w, w->d_P, w->d_H, x_exch, w->d_F, nullptr, false, epart);
+    if (rc) break;
+    part.assign(w->Nb, 0.0);
+    e = hipMemcpyAsync(part.data(), epart, part.size() * sizeof(double), hipMemcpyDeviceToHost, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) {
+      rc = zq_check(w, e, "zq_grad_eri: read-back");
+      break;
+    }
+    double sum = 0.0;
+    for (double v : part) sum += v;  // output blocks in order
+    out[x] = sum;
+  }
+  if (rc) zq_fail(ctx, rc, std::string("zq_grad_eri: ") + zq_last_error(w));
+  zq_destroy(w);
+  return rc;
+}
+
+extern "C" size_t zq_jk_record_count(size_t n) {
+  if (n == 0 || n > 4096) return 0;
+  const int nb = int((n + 7) / 8), Nb = nb * (nb + 1) / 2;
+  JKPlan plan;
+  zq_jk_plan(Nb, size_t(Nb) * (Nb + 1) / 2, 0, plan);
+  return plan.nrecords;
 }

@@ -753,6 +753,12 @@ struct Dims {
 };
 
 // Q1..Q4 with element type T (double: FP64, float: FP32 or BF16X3).
+// The coefficient matrix of each quarter (mu->i, lam->j, nu->a, sig->b); all the same C except in the
+// terms of the dual transform.
+struct Coefs {
+  const double *q[4];
+};
+
 // Bytes run_chain carves from a static arena for these dimensions (each piece 256-byte aligned), the
 // no-allocation entry point's size check.
 size_t chain_bytes(const Dims &d, int precision) {
@@ -768,12 +774,15 @@ size_t chain_bytes(const Dims &d, int precision) {
 // Q1..Q4.  With a static arena (ar.base set) nothing is allocated: C goes to the arena and is sliced
 // there, FP32 forms need the resident FP32 ERI copy already made, and FP64 Q4 runs on the tile kernel.
 template <class T, class MF>
-int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_out, Arena &ar) {
+int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_out, Arena &ar) {
   const int n = d.n, np = d.np, nb = d.nb, Nb = d.Nb, m1 = d.m1, m2 = d.m2, m3 = d.m3, m4 = d.m4;
   const bool fixed = ar.base != nullptr;
+  const double *C = cf.q[0];
+  const bool own4 = cf.q[3] != cf.q[2];  // a fourth quarter with its own coefficients (dual transform)
+  if (fixed && (own4 || cf.q[1] != C || cf.q[2] != C)) return zq_fail(ctx, ZQ_ERR_INVALID, "one coefficient matrix only");
   // BF16X3 Q1 takes C1 split and transposed: C1h/C1l[i][mu], mu padded to a multiple of 32
   const int ldt = ((nb + 3) / 4) * 32;
-  T *d1, *d2t, *d3;
+  T *d1, *d2t, *d3, *d4 = nullptr;
   bf16 *d1h = nullptr, *d1l = nullptr;
   if constexpr (sizeof(T) == 4) {
     if (precision != ZQ_PREC_BF16X3) {
@@ -802,15 +811,21 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
     ZQ_LAUNCH_CHECK(ctx, "zq_c_slices");
   } else {
     // coefficient slices, zero rows for padded AOs: C1 (np x m1), C2T (m2 x np), C3 = C4 (np x m3)
-    std::vector<T> h1(size_t(np) * m1, T(0)), h2t(size_t(m2) * np, T(0)), h3(size_t(np) * m3, T(0));
+    // (each quarter from its own matrix cf.q[0..3]; the same C unless this is a term of the dual transform)
+    std::vector<T> h1(size_t(np) * m1, T(0)), h2t(size_t(m2) * np, T(0)), h3(size_t(np) * m3, T(0)),
+        h4(own4 ? size_t(np) * m4 : 0, T(0));
     for (int mu = 0; mu < n; ++mu) {
-      for (int i = 0; i < m1; ++i) h1[size_t(mu) * m1 + i] = T(C[size_t(mu) * n + i]);
-      for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = T(C[size_t(mu) * n + j]);
-      for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = T(C[size_t(mu) * n + d.off3 + a]);
+      for (int i = 0; i < m1; ++i) h1[size_t(mu) * m1 + i] = T(cf.q[0][size_t(mu) * n + i]);
+      for (int j = 0; j < m2; ++j) h2t[size_t(j) * np + mu] = T(cf.q[1][size_t(mu) * n + j]);
+      for (int a = 0; a < m3; ++a) h3[size_t(mu) * m3 + a] = T(cf.q[2][size_t(mu) * n + d.off3 + a]);
+      if (own4)
+        for (int b = 0; b < m4; ++b) h4[size_t(mu) * m4 + b] = T(cf.q[3][size_t(mu) * n + d.off3 + b]);
     }
     if (int rc = upload(ctx, "C1", h1, &d1)) return rc;
     if (int rc = upload(ctx, "C2T", h2t, &d2t)) return rc;
     if (int rc = upload(ctx, "C3", h3, &d3)) return rc;
+    if (own4)
+      if (int rc = upload(ctx, "C4", h4, &d4)) return rc;
     if (precision == ZQ_PREC_BF16X3) {
       std::vector<bf16> hh(size_t(m1) * ldt, (bf16)0.f), hl(size_t(m1) * ldt, (bf16)0.f);
       for (int i = 0; i < m1; ++i)
@@ -826,6 +841,7 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
     ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the host staging vectors go out of scope below
   }
 
+  if (!d4) d4 = d3;
   const size_t nX = size_t(Nb) * 64 * m1 * np, nY = size_t(m1) * m2 * np * np, nW = size_t(m1) * m2 * np * m3,
                nO = size_t(m1) * m2 * m3 * m4;
   T *X = (T *)ar.get("X", nX * sizeof(T));
@@ -889,7 +905,7 @@ int run_chain(zq_ctx *ctx, const double *C, const Dims &d, int precision, T **d_
   g3.M = np, g3.N = m3, g3.K = np;
   // Q4: batch over (i,j); A[a][sig] = W[sig][a], B = C4 [sig][b], out[i][j][a][b]
   g4.A = W, g4.sAb = (long long)np * m3, g4.sAm = 1, g4.sAk = m3;
-  g4.B = d3, g4.sBb = 0, g4.sBk = m4, g4.sBn = 1;
+  g4.B = d4, g4.sBb = 0, g4.sBk = m4, g4.sBn = 1;
   g4.C = O, g4.sCb = (long long)m3 * m4, g4.sCm = m4, g4.sCn = 1;
   g4.M = m3, g4.N = m4, g4.K = np;
   if constexpr (sizeof(T) == 4) {
@@ -938,20 +954,20 @@ int make_dims(zq_ctx *ctx, int nocc, int block, int precision, Dims &d) {
   return ZQ_OK;
 }
 
-int transform_in(zq_ctx *ctx, const double *C, const Dims &d, int precision, Arena &ar, void **d_out, int dims[4],
+int transform_in(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, Arena &ar, void **d_out, int dims[4],
                  bool *is_f32) {
   dims[0] = d.m1, dims[1] = d.m2, dims[2] = d.m3, dims[3] = d.m4;
   if (precision == ZQ_PREC_FP64) {
     *is_f32 = false;
     double *o = nullptr;
-    int rc = run_chain<double, MF64>(ctx, C, d, precision, &o, ar);
+    int rc = run_chain<double, MF64>(ctx, cf, d, precision, &o, ar);
     *d_out = o;
     return rc;
   }
   *is_f32 = true;
   float *o = nullptr;
-  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, C, d, precision, &o, ar)
-                                      : run_chain<float, MF32>(ctx, C, d, precision, &o, ar);
+  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, cf, d, precision, &o, ar)
+                                      : run_chain<float, MF32>(ctx, cf, d, precision, &o, ar);
   *d_out = o;
   return rc;
 }
@@ -962,7 +978,7 @@ int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int p
   Dims d;
   if (int rc = make_dims(ctx, nocc, block, precision, d)) return rc;
   Arena ar{ctx};
-  return transform_in(ctx, C, d, precision, ar, d_out, dims, is_f32);
+  return transform_in(ctx, Coefs{{C, C, C, C}}, d, precision, ar, d_out, dims, is_f32);
 }
 
 extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, int precision, double *out) {
@@ -996,7 +1012,7 @@ int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precisio
   void *T = nullptr;
   int dims[4];
   bool f32 = false;
-  if (int rc = transform_in(ctx, C, dm, precision, ar, &T, dims, &f32)) return rc;
+  if (int rc = transform_in(ctx, Coefs{{C, C, C, C}}, dm, precision, ar, &T, dims, &f32)) return rc;
   const int o = dims[0], v = dims[2];
   double *de = (double *)ar.get("mp2_e", size_t(ctx->n) * 8);
   double *part = (double *)ar.get("mp2_part", size_t(o) * o * 8 + 8);
@@ -1072,4 +1088,40 @@ extern "C" int zq_mp2_noalloc(zq_ctx *ctx, const double *C, const double *e, siz
   const int rc = mp2_in(ctx, C, e, int(nocc), precision, e_corr, ar);
   ctx->profile = prof;
   return rc;
+}
+
+// The dual-number transform: value T(g; C,C,C,C) and tangent
+// T(dg; C,C,C,C) + T(g; dC,C,C,C) + T(g; C,dC,C,C) + T(g; C,C,dC,C) + T(g; C,C,C,dC), FP64, summed on the
+// host in that order.  dg is the quartet set dset holds, read in place of ctx's own for the first term.
+extern "C" int zq_ao2mo_dual(zq_ctx *ctx, zq_eri_set *dset, const double *C, const double *dC, size_t nocc, int block,
+                             double *out, double *dout) {
+  ZqGate _gate;
+  if (!ctx || !dset || !C || !dC || !out || !dout) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_ao2mo_dual: null argument");
+  if (!ctx->eri) return zq_fail(ctx, ZQ_ERR_STATE, "zq_ao2mo_dual: no ERIs loaded");
+  if (dset->n != ctx->n || dset->device != ctx->device)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_ao2mo_dual: derivative set of another size or device");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  Dims d;
+  if (int rc = make_dims(ctx, int(nocc), block, ZQ_PREC_FP64, d)) return rc;
+  const size_t cnt = size_t(d.m1) * d.m2 * d.m3 * d.m4;
+  std::vector<double> part(cnt);
+  const Coefs terms[6] = {{{C, C, C, C}}, {{C, C, C, C}}, {{dC, C, C, C}},
+                          {{C, dC, C, C}}, {{C, C, dC, C}}, {{C, C, C, dC}}};
+  for (int t = 0; t < 6; ++t) {
+    double *dst = t == 0 ? out : (t == 1 ? dout : part.data());
+    double *own = ctx->eri;
+    if (t == 1) ctx->eri = dset->eri;  // the derivative quartets, same layout
+    Arena ar{ctx};
+    void *dv = nullptr;
+    int dims[4];
+    bool f32 = false;
+    const int rc = transform_in(ctx, terms[t], d, ZQ_PREC_FP64, ar, &dv, dims, &f32);
+    ctx->eri = own;
+    if (rc) return rc;
+    ZQ_TRY(ctx, hipMemcpyAsync(dst, dv, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (t >= 2)
+      for (size_t k = 0; k < cnt; ++k) dout[k] += part[k];
+  }
+  return ZQ_OK;
 }

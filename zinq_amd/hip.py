@@ -18,7 +18,7 @@ EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_l
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
             "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
-            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_mp2_noalloc", "zq_ao2mo_pp", "zq_ci", "zq_mp"]
+            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_mp2_noalloc", "zq_ao2mo_pp", "zq_ci", "zq_mp", "zq_grad_eri", "zq_ao2mo_dual", "zq_jk_record_count"]
 
 
 class GpuError(RuntimeError):
@@ -77,6 +77,8 @@ def lib():
         L.zq_mp2.argtypes = [vp, vp, vp, sz, ci, vp]
         L.zq_mp2_noalloc.argtypes = [vp, vp, vp, sz, ci, vp]
         L.zq_ao2mo_pp.argtypes = [vp, sz, vp, vp, vp]
+        L.zq_grad_eri.argtypes = [vp, vp, sz, sz, vp, cd, vp]
+        L.zq_ao2mo_dual.argtypes = [vp, vp, vp, vp, sz, ci, vp, vp]
         L.zq_ci.argtypes = [vp, vp, vp, sz, ci, vp, sz, cd, sz, vp, vp]
         L.zq_mp.argtypes = [vp, vp, vp, sz, ci, ci, vp, vp]
         L.zq_profile_enable.argtypes = [vp, ci]
@@ -100,6 +102,8 @@ def lib():
         L.zq_eri_set_bytes.argtypes = [vp]
         L.zq_attach.argtypes = [vp, vp]
         L.zq_mem_info.argtypes = [ci, vp, vp]
+        L.zq_jk_record_count.restype = sz
+        L.zq_jk_record_count.argtypes = [sz]
         L.zq_fock_jk_multi.argtypes = [vp, vp, vp, sz, cd, vp]
         L._zq_typed = True
     return L
@@ -117,6 +121,11 @@ def host_sygv(A, B):
     if rc:
         raise LapackError("generalized eigensolver failed")
     return w, A
+
+
+def jk_record_count(n):
+    """J/K records of the plan for n basis functions (zq_jk_record_count)."""
+    return int(lib().zq_jk_record_count(n))
 
 
 def mem_info(device=0):
@@ -227,6 +236,23 @@ class Context:
         out = np.zeros(1)
         self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
         return float(out[0])
+
+    def ao2mo_dual(self, dset, C, dC, nocc=0, block="full"):
+        """(out, dout): the transform of g + eps dg (ctx's ERIs, dset's) with C + eps dC (zq_ao2mo_dual)."""
+        C, dC = _f64(C), _f64(dC)
+        n = C.shape[0]
+        shape = (n, n, n, n) if block == "full" else (nocc, nocc, n - nocc, n - nocc)
+        out, dout = np.zeros(shape), np.zeros(shape)
+        self._check(self._L.zq_ao2mo_dual(self._h, dset._h, ptr(C), ptr(dC), nocc, BLOCK[block], ptr(out), ptr(dout)))
+        return out, dout
+
+    def grad_eri(self, dg_packed, n, P, x_exch=0.5):
+        """sum 1/2 P P ((pq|rs)_x - x_exch (pr|qs)_x) for each packed derivative set dg_packed[x] (zq_grad_eri)."""
+        dg, P = _f64(dg_packed), _f64(P)
+        ncoord = dg.shape[0]
+        out = np.zeros(ncoord)
+        self._check(self._L.zq_grad_eri(self._h, ptr(dg), n, ncoord, ptr(P), float(x_exch), ptr(out)))
+        return out
 
     def ao2mo_pp(self, A, C):
         """C^T A C on the GPU (zq_ao2mo_pp; ao2mo_pp of src/integral_transform.zig:12-34)."""

@@ -29,13 +29,12 @@ def eri_set_bytes(n, with_fp32):
 
 
 def scratch_bytes(n, o, transform_bytes=4):
-    """Transient device memory of one request: J/K records (~ 2.5 records of 64 doubles per quartet)
+    """Transient device memory of one request: the J/K records of the molecule's plan (64 doubles each)
     and the OVOV intermediates X, Y, W, OUT of the transform (element size transform_bytes)."""
     nb = (n + 7) // 8
     npd, Nb = nb * 8, nb * (nb + 1) // 2
-    nq = Nb * (Nb + 1) // 2
     v = n - o
-    records = int(2.5 * nq) * 64 * 8
+    records = hip.jk_record_count(n) * 64 * 8
     tr = (Nb * 64 * o * npd + o * o * npd * npd + o * o * npd * v + o * o * v * v) * transform_bytes
     return records + tr + 64 * n * n * 8
 
@@ -52,7 +51,9 @@ class Server:
     def __init__(self, molecules, slots=8, device=0, hbm_budget=None, mode=None):
         self.mols = {m.key: m for m in molecules}
         self.mode = mode or dict(fock="fp32", transform="fp32c", threshold=1e-6)
-        self.with_fp32 = self.mode["fock"] == "fp32"
+        # the FP32 ERI copy lives in the shared set whenever a Fock build or the transform reads it, so no
+        # slot context makes (and frees) a copy of its own
+        self.with_fp32 = self.mode["fock"] == "fp32" or self.mode["transform"] in ("fp32", "fp32c")
         self.tbytes = 8 if self.mode["transform"] == "fp64" else 4
         if hbm_budget is None:
             hbm_budget = int(0.85 * hip.mem_info(device)[0])

@@ -311,3 +311,35 @@ def test_dual_transform(rhf):
     assert np.abs(dout - fd).max() < 1e-6 * max(1.0, np.abs(dout).max())
     z = O.ao2mo_pppp_dual(g, np.zeros_like(g), r["C"], np.zeros_like(dC))[1]
     assert not z.any()
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_energies_invariant_under_rotation_and_translation(k):
+    """cc-pVDZ (H2O)_k rigidly rotated (seeded uniform rotations) and translated: the RHF and MP2 energies
+    and the spectra of S and of the ERI supermatrix do not change.  A wrong solid-harmonic combination of
+    the pure d functions, or a wrong Hermite term of their (dd|dd) recursions, makes the basis (or its
+    integrals) depend on the orientation, which these catch."""
+    from zinq_amd.molecules import water_cluster, BOHR_TO_ANGSTROM, _rotation
+    Z, xyz = water_cluster(k)
+    xyz = xyz / BOHR_TO_ANGSTROM
+
+    def energies(coords):
+        s = MolecularSystem.from_atoms(Z, coords, "builtin:cc-pvdz")
+        H, S, g, VN, o = s.hcore(), s.overlap(), s.coulomb(), s.nrep(), s.nel // 2
+        r = O.scf(H, S, g, VN, o, threshold=1e-11)
+        e2 = O.mp2_closed(O.ao2mo_oovv(g, r["C"], o), r["e"])
+        n = s.nbf
+        w = np.linalg.eigvalsh(g.transpose(0, 2, 1, 3).reshape(n * n, n * n))
+        return r["E"], e2, np.linalg.eigvalsh(S), w
+
+    E0, c0, s0, w0 = energies(xyz)
+    rng = np.random.default_rng(100 + k)
+    centre = xyz.mean(axis=0)
+    for t in range(2):
+        R = _rotation(*rng.uniform(size=3))
+        moved = (xyz - centre) @ R.T + centre + rng.uniform(-2.0, 2.0, size=3)
+        E, c, s_, w = energies(moved)
+        assert E == pytest.approx(E0, abs=1e-10)
+        assert c == pytest.approx(c0, abs=1e-10)
+        np.testing.assert_allclose(s_, s0, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(w, w0, rtol=0, atol=1e-11)

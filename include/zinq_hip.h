@@ -77,7 +77,7 @@ int zq_fock_jk_multi(zq_ctx *ctx, const double *H, const double *P, size_t ndens
 /* The two-electron part of the HF nuclear gradient, g_G of src/hartree_fock.zig:263-273: for each of
  * ncoord derivative ERI sets (dg_packed: ncoord consecutive canonical packed arrays of the
  * zq_eri_load_packed layout, each the derivative of (pq|rs) with respect to one nuclear coordinate),
- *   out[x] = sum_pqrs 1/2 P[p,q] P[r,s] ((pq|rs)_x - x_exch (pr|qs)_x),  x_exch = exx (1/2 RHF, 1 GHF),
+ *   out[x] = sum_pqrs 1/2 P[p,q] P[r,s] ((pq|rs)_x - x_exch (pr|qs)_x),  x_exch = exx / 2 (RHF; 1/2 for HF), exx (GHF),
  * one J/K pass over each set on the GPU (a private context holds the set; ctx's own ERIs stay loaded). */
 int zq_grad_eri(zq_ctx *ctx, const double *dg_packed, size_t n, size_t ncoord, const double *P, double x_exch,
                 double *out);
@@ -129,9 +129,8 @@ int zq_mp(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int genera
           size_t *ndet_out);
 
 /* zq_mp2 that allocates nothing.  No counterpart in the reference: it is for callers that cannot
- * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same contract and
- * result as zq_mp2 (the FP64 Q4 product runs on the hand-written tile kernel instead of rocBLAS, so
- * FP64 results agree to rounding), but every intermediate lives in one static device array of 64 MiB
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  Same contract,
+ * kernels and result as zq_mp2, but every intermediate lives in one static device array of 64 MiB
  * per GPU, held for the life of the process: molecules up to n = 72 fit (n x nocc dependent; a larger
  * one returns ZQ_ERR_INVALID with the reason).  The FP32 forms need the FP32 ERI copy already resident
  * (an ERI set created with_fp32, or an earlier FP32 call on ctx), else ZQ_ERR_STATE.  All calls on one

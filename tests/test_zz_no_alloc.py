@@ -1,8 +1,8 @@
 """zq_mp2_noalloc (the MP2 energy from the library's static device scratch, no allocation) against zq_mp2
 through the C ABI: two calls in a row on one GPU, the second with a smaller molecule, in FP64, BF16X3
 and FP32C; the context's and the device's memory unchanged by the calls; the size and state errors.
-Bars: FP64 1e-12 Eh (the two differ only in the summation order of the last quarter product); the
-reduced-precision forms share every kernel and agree bitwise."""
+Both entry points run the same kernels on the same shapes.  Bars: FP64 1e-12 Eh; the reduced-precision
+forms bitwise."""
 import numpy as np
 import pytest
 

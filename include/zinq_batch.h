@@ -10,8 +10,11 @@
  *   - one launch runs the rest of the iteration for every molecule, one workgroup per molecule
  *     (energy, DIIS error/history/LU/extrapolation, X^T F X, SP2 density, convergence test);
  *   - the iteration is captured as one hipGraph and replayed until every molecule has converged;
- *   - the final eigenpairs come from batched rocSOLVER dsyevd per distinct size, and the MP2 energies
- *     from the OVOV transform of zinq_hip.h, several molecules at a time on their own streams.
+ *   - the final eigenpairs come from batched rocSOLVER dsyevd per distinct size; the MP2 energies
+ *     from the OVOV transform of zinq_hip.h grouped over all molecules: one launch per stage (C slices,
+ *     Q1..Q4, pair sums, totals), with C and the orbital energies kept on the device.  Each molecule's
+ *     energy is bitwise the one zq_mp2 gives for its orbitals (FP64, FP32, FP32C; BF16X3 runs the
+ *     single-molecule transform for several molecules at a time on their own streams).
  * Limits: RHF, n <= 96 basis functions per molecule, DIIS history <= 8.
  */
 #ifndef ZINQ_BATCH_H
@@ -50,6 +53,10 @@ int zq_batch_autotune(zq_batch *b, int chosen[2]);
  * ZQ_ERR_LAPACK for that molecule).  Returns ZQ_OK when the batch ran (per-molecule status aside). */
 int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, double *E_rhf, double *E_corr,
                  int *iterations, int *status);
+
+/* Stage times of the last zq_batch_run, host wall clock in ms: {SCF iterations, final eigenpairs and
+ * orbitals, MP2, total}.  Diagnostics, with no counterpart in the reference. */
+int zq_batch_timings(const zq_batch *b, double ms[4]);
 
 /* Orbital energies and coefficients of molecule `index` from the last zq_batch_run: e (n), C (n x n). */
 int zq_batch_orbitals(const zq_batch *b, int index, double *e, double *C);

@@ -58,6 +58,17 @@ int zq_scf_run_device(zq_ctx *ctx, size_t n, size_t nocc, const double *H, const
  * that needs more restarts the run on the host-synchronised driver and raises the budget by 16. */
 int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
 
+/* The same for the core-Hamiltonian guess density of that iteration (1..128, default 128).  A guess
+ * that does not settle ends the device-decided run after its first iteration and restarts on the
+ * host-synchronised driver; unlike the per-iteration budget it is not raised.  A tuning and test hook
+ * with no counterpart in the reference. */
+int zq_scf_set_sp2_guess_budget(zq_ctx *ctx, int steps);
+
+/* Diagnostics, with no counterpart in the reference: the iterations the device-decided driver of the
+ * last zq_scf_run_device on ctx enqueued (graph replays; the host reads the state one iteration late,
+ * so a run that ends after iteration k enqueued k + 1 or k + 2 of them), 0 if it did not run. */
+int zq_scf_last_replays(const zq_ctx *ctx);
+
 /* Diagnostics, with no counterpart in the reference: which driver finished the last zq_scf_run_device
  * on ctx -- 1 the device-decided iteration, 2 the host-synchronised restart -- or 0 before any run. */
 int zq_scf_last_driver(const zq_ctx *ctx);

@@ -279,6 +279,36 @@ def test_device_scf_sp2_budget_fallback(ctx):
         ctx.set_sp2_budget(129)
 
 
+def test_device_scf_unsettled_guess_stops_the_replays(ctx):
+    """A guess density whose SP2 does not settle (a 2-step budget) marks the state before the first
+    iteration; that iteration's convergence kernel hands the mark to the host, which stops replaying
+    after at most two iterations and reruns the host-synchronised driver.  The result equals the oracle's
+    SCF; a plain run enqueues one iteration past the converged one."""
+    from zinq_amd import hip
+    s, H, S, g, VN = _load(ctx, "ccpvdz")
+    o = s.nel // 2
+    ro = O.scf(H, S, g, VN, o, threshold=1e-10)
+    ref = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    assert ctx.last_scf_driver() == "device"
+    assert ref["iterations"] + 1 <= ctx.scf_replays() <= ref["iterations"] + 2
+    ctx.set_sp2_guess_budget(2)
+    try:
+        r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+        replays = ctx.scf_replays()
+    finally:
+        ctx.set_sp2_guess_budget(128)
+        ctx.set_sp2_budget(100)
+    print(f"unsettled guess: {replays} replays, driver {ctx.last_scf_driver()}, dE vs oracle {r['E'] - ro['E']:+.1e}",
+          flush=True)
+    assert ctx.last_scf_driver() == "synced"
+    assert 1 <= replays <= 2
+    assert r["E"] == pytest.approx(ro["E"], abs=1e-10)
+    assert r["iterations"] == ref["iterations"]
+    for bad in (0, 129):
+        with pytest.raises(hip.InvalidInput):
+            ctx.set_sp2_guess_budget(bad)
+
+
 @pytest.mark.parametrize("kind", ["ccpvdz", "w2"])
 def test_mixed_path_within_bar(ctx, kind):
     """Mixed path: incremental Fock builds from FP32 ERIs to 1e-6, one full FP64 build, FP32C transform.
@@ -361,6 +391,34 @@ def test_batch_autotune_and_rerun():
     r1 = b.run(threshold=1e-10)
     np.testing.assert_allclose(r1["E_rhf"], r0["E_rhf"], atol=1e-11)
     np.testing.assert_allclose(r1["E_corr"], r0["E_corr"], atol=1e-11)
+    b.close()
+
+
+def test_batch_grouped_mp2_is_zq_mp2():
+    """The batch's MP2 stage (one grouped launch per transform stage over all molecules, C and e on the
+    device) gives, for every molecule, bitwise the energy zq_mp2 gives for the same orbitals: FP64, FP32
+    and FP32C grouped, BF16X3 on per-molecule streams.  (H2O)_4 cc-pVDZ (n = 96, o = 20) puts two 16-row
+    tiles of occupied orbitals in the grouped Q1 where the single-molecule launch takes one workgroup of
+    two tiles."""
+    from zinq_amd import hip
+    from zinq_amd.batch import Batch
+    from zinq_amd.molecules import water_system
+    systems = _batch_systems(7, seed=3) + [water_system(4)]
+    b = Batch(0)
+    for s in systems:
+        b.add_system(s)
+    with hip.Context(0) as ctx:
+        for prec in ("fp64", "fp32", "fp32c", "bf16x3"):
+            r = b.run(threshold=1e-10, mp2_precision=prec)
+            assert (r["status"] == 0).all(), r["status"]
+            t = b.timings()
+            print(f"{prec}: stage ms {t}", flush=True)
+            assert t["total"] >= t["scf"] + t["eigen"] + t["mp2"] - 1e-3
+            for m, s in enumerate(systems):
+                ctx.eri_load_packed(s.coulomb_packed(), s.nbf)
+                e, C = b.orbitals(m)
+                e2 = ctx.mp2(C, e, s.nel // 2, prec)
+                assert r["E_corr"][m] == e2, (prec, m, s.nbf, r["E_corr"][m] - e2)
     b.close()
 
 

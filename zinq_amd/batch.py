@@ -7,7 +7,7 @@ from ._native import ptr
 from .hip import PRECISION, ScfOptions, GpuError, _ERRORS, _f64, lib as _hip_lib
 
 EXPORTED = ["zq_batch_create", "zq_batch_destroy", "zq_batch_last_error", "zq_batch_add", "zq_batch_count",
-            "zq_batch_autotune", "zq_batch_run", "zq_batch_orbitals"]
+            "zq_batch_autotune", "zq_batch_run", "zq_batch_orbitals", "zq_batch_timings"]
 
 
 def lib():
@@ -24,6 +24,7 @@ def lib():
         L.zq_batch_autotune.argtypes = [vp, vp]
         L.zq_batch_run.argtypes = [vp, ctypes.POINTER(ScfOptions), ci, vp, vp, vp, vp]
         L.zq_batch_orbitals.argtypes = [vp, ci, vp, vp]
+        L.zq_batch_timings.argtypes = [vp, vp]
         L._zq_batch_typed = True
     return L
 
@@ -78,6 +79,14 @@ class Batch:
         if rc:
             self._err(rc)
         return dict(E_rhf=E, E_corr=Ec, iterations=it, status=st)
+
+    def timings(self):
+        """Stage times of the last run, ms (host wall clock): scf, eigen, mp2, total (zq_batch_timings)."""
+        ms = np.zeros(4)
+        rc = self._L.zq_batch_timings(self._h, ptr(ms))
+        if rc:
+            self._err(rc)
+        return dict(zip(("scf", "eigen", "mp2", "total"), (round(float(x), 3) for x in ms)))
 
     def orbitals(self, index):
         n = self.sizes[index]

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -409,12 +410,18 @@ struct zq_batch {
   int2 *d_blk = nullptr;
   int nitems = 0, nblk = 0;
   std::vector<long long> eri_off, ws_off;
+  size_t eri_count = 0;  // doubles in eri
   int *h_count = nullptr;
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
   rocblas_handle solver = nullptr;
-  // results of the last run
+  // the grouped MP2's FP32 copy of the quartets (FP32 forms; made once per upload) and workspace
+  float *eri32 = nullptr;
+  void *mp2_ws = nullptr;
+  size_t mp2_ws_bytes = 0;
+  // results of the last run, and its stage times (host wall clock, ms): SCF, eigenpairs, MP2, total
   std::vector<std::vector<double>> C, e;
+  double stage_ms[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -441,10 +448,10 @@ void free_plan(zq_batch *b) {
 void free_all(zq_batch *b) {
   free_plan(b);
   for (void *p : {(void *)b->eri, (void *)b->ws, (void *)b->d_mols, (void *)b->d_state, (void *)b->d_done,
-                  (void *)b->d_count, (void *)b->d_jkm})
+                  (void *)b->d_count, (void *)b->d_jkm, (void *)b->eri32, b->mp2_ws})
     if (p) (void)hipFree(p);
   b->eri = nullptr, b->ws = nullptr, b->d_mols = nullptr, b->d_state = nullptr, b->d_done = nullptr;
-  b->d_count = nullptr, b->d_jkm = nullptr;
+  b->d_count = nullptr, b->d_jkm = nullptr, b->eri32 = nullptr, b->mp2_ws = nullptr, b->mp2_ws_bytes = 0;
   b->uploaded = false;
 }
 
@@ -522,6 +529,7 @@ int upload(zq_batch *b) {
                   b->ws_off[m] + M_F * (long long)mm, mp, h.n};
   }
   BTRY(b, hipMalloc(&b->eri, eri_total * sizeof(double)));
+  b->eri_count = eri_total;
   BTRY(b, hipMalloc(&b->ws, ws_total * sizeof(double)));
   BTRY(b, hipMalloc(&b->d_mols, M * sizeof(BMol)));
   BTRY(b, hipMalloc(&b->d_state, M * sizeof(BState)));
@@ -720,6 +728,10 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
   BTRY(b, hipSetDevice(b->device));
   if (int rc = upload(b)) return rc;
   const int M = int(b->mols.size());
+  const auto t0 = std::chrono::steady_clock::now();
+  auto since = [](std::chrono::steady_clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+  };
 
   // core-Hamiltonian guess, then the captured iteration until every molecule is done
   launch_step(b, MODE_GUESS, opt->diis, opt->threshold, opt->iterations);
@@ -762,6 +774,8 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
   std::vector<BState> hs(M);
   BTRY(b, hipMemcpyAsync(hs.data(), b->d_state, M * sizeof(BState), hipMemcpyDeviceToHost, b->stream));
   BTRY(b, hipStreamSynchronize(b->stream));
+  b->stage_ms[0] = since(t0);
+  const auto t1 = std::chrono::steady_clock::now();
 
   // eigenpairs of X^T F_ext X: molecules ordered by n so that each size is one strided batch
   std::vector<int> order(M);
@@ -835,37 +849,66 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
     info.swap(by_mol);
   }
 
-  // MP2 on several view contexts at once, each with its own stream
-  const int nworkers = std::min(M, 8);
-  std::vector<zq_ctx *> views(nworkers, nullptr);
-  for (int w = 0; w < nworkers; ++w) {
-    views[w] = zq_create(b->device);
-    if (!views[w]) {
-      for (zq_ctx *v : views)
-        if (v) zq_destroy(v);
-      return bfail(b, ZQ_ERR_LAUNCH, "batch MP2: context creation");
+  b->stage_ms[1] = since(t1);
+  const auto t2 = std::chrono::steady_clock::now();
+
+  // MP2 of every molecule that finished its iterations
+  std::vector<int> rc_m(M, ZQ_OK);
+  for (int m = 0; m < M; ++m) E_corr[m] = 0.0;
+  if (mp2_precision != ZQ_PREC_BF16X3) {
+    // grouped: one launch per transform stage for all molecules, on the device C and e
+    const bool f32 = mp2_precision == ZQ_PREC_FP32 || mp2_precision == ZQ_PREC_FP32C;
+    if (f32 && !b->eri32) {
+      BTRY(b, hipMalloc(&b->eri32, b->eri_count * sizeof(float)));
+      zq_narrow_eri(b->stream, b->eri, b->eri_count, b->eri32);
+      BTRY(b, hipGetLastError());
+    }
+    std::vector<ZqGroupMol> gm;
+    std::vector<int> which;
+    for (int m = 0; m < M; ++m) {
+      if (hs[m].status != 1 && hs[m].status != 2) continue;
+      gm.push_back(ZqGroupMol{b->eri + b->eri_off[m], f32 ? b->eri32 + b->eri_off[m] : nullptr, b->mols[m].n,
+                              b->mols[m].nocc, d_c + coff[m], d_w + woff[m]});
+      which.push_back(m);
+    }
+    std::vector<double> ec(gm.size(), 0.0);
+    std::string err;
+    if (int rc = zq_mp2_grouped(b->stream, gm, mp2_precision, &b->mp2_ws, &b->mp2_ws_bytes, ec.data(), err))
+      return bfail(b, rc, err);
+    for (size_t k = 0; k < which.size(); ++k) E_corr[which[k]] = ec[k];
+  } else {
+    // BF16X3: several view contexts at once, each with its own stream
+    const int nworkers = std::min(M, 8);
+    std::vector<zq_ctx *> views(nworkers, nullptr);
+    for (int w = 0; w < nworkers; ++w) {
+      views[w] = zq_create(b->device);
+      if (!views[w]) {
+        for (zq_ctx *v : views)
+          if (v) zq_destroy(v);
+        return bfail(b, ZQ_ERR_LAUNCH, "batch MP2: context creation");
+      }
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int w = 0; w < nworkers; ++w)
+      th.emplace_back([&, w] {
+        ZqGate gate;
+        (void)hipSetDevice(b->device);
+        for (int m; (m = next.fetch_add(1)) < M;) {
+          if (hs[m].status != 1 && hs[m].status != 2) continue;
+          bind_view(views[w], b, m);
+          rc_m[m] = zq_mp2_device(views[w], b->C[m].data(), b->e[m].data(), b->mols[m].nocc, mp2_precision, &E_corr[m]);
+          views[w]->eri = nullptr;
+        }
+      });
+    for (auto &t : th) t.join();
+    for (zq_ctx *v : views) {
+      v->eri = nullptr;
+      zq_destroy(v);
     }
   }
-  std::atomic<int> next{0};
-  std::vector<int> rc_m(M, ZQ_OK);
-  std::vector<std::thread> th;
-  for (int w = 0; w < nworkers; ++w)
-    th.emplace_back([&, w] {
-      ZqGate gate;
-      (void)hipSetDevice(b->device);
-      for (int m; (m = next.fetch_add(1)) < M;) {
-        E_corr[m] = 0.0;
-        if (hs[m].status != 1 && hs[m].status != 2) continue;
-        bind_view(views[w], b, m);
-        rc_m[m] = zq_mp2_device(views[w], b->C[m].data(), b->e[m].data(), b->mols[m].nocc, mp2_precision, &E_corr[m]);
-        views[w]->eri = nullptr;
-      }
-    });
-  for (auto &t : th) t.join();
-  for (zq_ctx *v : views) {
-    v->eri = nullptr;
-    zq_destroy(v);
-  }
+  b->stage_ms[2] = since(t2);
+  b->stage_ms[3] = since(t0);
   for (int m = 0; m < M; ++m) {
     E_rhf[m] = hs[m].e_new;
     iterations[m] = hs[m].it;
@@ -873,6 +916,12 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
     if (info[m] && status[m] == ZQ_OK) status[m] = ZQ_ERR_LAPACK;
     if (rc_m[m] != ZQ_OK && status[m] == ZQ_OK) status[m] = rc_m[m];
   }
+  return ZQ_OK;
+}
+
+int zq_batch_timings(const zq_batch *b, double ms[4]) {
+  if (!b || !ms) return ZQ_ERR_INVALID;
+  std::copy(b->stage_ms, b->stage_ms + 4, ms);
   return ZQ_OK;
 }
 

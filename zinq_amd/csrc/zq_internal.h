@@ -121,6 +121,8 @@ struct zq_ctx {
   unsigned scf_captures = 0;  // SCF iterations captured over the context's lifetime (diagnostics)
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
+  int sp2_guess_budget = 128;  // SP2 steps for the core-Hamiltonian guess density of that SCF
+  int scf_replays = 0;   // iterations the asynchronous driver enqueued in the last device SCF
   int scf_driver = 0;    // driver that finished the last device SCF: 1 device-decided, 2 host-synchronised
   int scf_sp2[3] = {0, 0, 0};  // its SP2 densities, folded ones, steps (device-decided driver only)
   hipEvent_t scf_events[2] = {nullptr, nullptr};  // the asynchronous SCF's lagged state reads
@@ -190,6 +192,20 @@ int zq_eri32(zq_ctx *ctx);  // zq_jk.hip: FP32 copy of the resident quartets, ma
 void zq_narrow_eri(hipStream_t stream, const double *eri, size_t cnt, float *out);  // zq_jk.hip
 // MP2 correlation energy of the resident ERIs (zq_transform.hip); C and e are host arrays.
 int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr);
+// One molecule of a grouped MP2: its resident quartets (and their FP32 copy, for the FP32 forms), size,
+// doubly occupied count, and device C (n x n, row-major, columns = MOs) and orbital energies e (n).
+struct ZqGroupMol {
+  const double *eri;
+  const float *eri32;
+  int n, nocc;
+  const double *C, *e;
+};
+// Closed-shell MP2 correlation energies of many molecules, one launch per transform stage for all of
+// them, each molecule's result bitwise that of zq_mp2 (zq_transform.hip).  precision: FP64, FP32 or
+// FP32C.  ws / ws_bytes: the caller's grow-only device workspace (allocated here when too small, under
+// the caller's gate).  e_corr: host array of mols.size().  On error the reason is in err.
+int zq_mp2_grouped(hipStream_t stream, const std::vector<ZqGroupMol> &mols, int precision, void **ws, size_t *ws_bytes,
+                   double *e_corr, std::string &err);
 // Q1..Q4 of the resident ERIs into context scratch (zq_transform.hip): *d_out = out[i][j][a][b] with dims
 // m1 x m2 x m3 x m4 (FULL: n^4, <pq|rs>; OVOV: o o v v), double or (*is_f32) float.
 int zq_transform_device(zq_ctx *ctx, const double *C, int nocc, int block, int precision, void **d_out, int dims[4],

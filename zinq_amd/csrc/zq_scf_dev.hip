@@ -685,7 +685,8 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, ctx->stream, st, r.VN, r.opt->diis);
   // core-Hamiltonian guess, src/hartree_fock.zig:425-431; W = H until the first iteration replaces it
   ZQ_TRY(ctx, hipMemcpyAsync(d.W, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
-  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, kSp2MaxSteps, st, part_rms, r.mixed ? d.dP : nullptr, true))
+  if (int rc = density_async(d, ctx->d_H, nocc, r.factor, ctx->sp2_guess_budget, st, part_rms,
+                             r.mixed ? d.dP : nullptr, true))
     return rc;
   // the incremental builds start from F = H (dP = P_0 after the guess)
   if (r.mixed) ZQ_TRY(ctx, hipMemcpyAsync(ctx->d_F, ctx->d_H, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
@@ -725,8 +726,10 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   }
   // the stream is idle here (the previous run ended with a synchronisation), so no kernel writes it now
   *(volatile int *)hdone = 0;
+  ctx->scf_replays = 0;
   int k = 0;
   for (; k < r.opt->iterations; ++k) {
+    ++ctx->scf_replays;
     if (use_graph) {
       ZQ_TRY(ctx, hipGraphLaunch(graph_exec, ctx->stream));
     } else if (int rc = enqueue_iteration(d, r, budget, st, part_rms, ddone)) {
@@ -878,6 +881,7 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   int status = 0, iterations = 0;
   double energy = VN;
   if (int rc = setup(d, H, S)) return rc;
+  ctx->scf_replays = 0;
   if (opt->diis <= kMaxHist) {
     gate.reset();
     const int rc = scf_async(d, r, &status, &energy, &iterations);
@@ -930,6 +934,15 @@ extern "C" int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps) {
   ctx->sp2_budget = steps;
   return ZQ_OK;
 }
+
+extern "C" int zq_scf_set_sp2_guess_budget(zq_ctx *ctx, int steps) {
+  if (!ctx || steps < 1 || steps > kSp2MaxSteps)
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_set_sp2_guess_budget: steps outside 1..128");
+  ctx->sp2_guess_budget = steps;
+  return ZQ_OK;
+}
+
+extern "C" int zq_scf_last_replays(const zq_ctx *ctx) { return ctx ? ctx->scf_replays : 0; }
 
 extern "C" int zq_scf_last_driver(const zq_ctx *ctx) { return ctx ? ctx->scf_driver : 0; }
 

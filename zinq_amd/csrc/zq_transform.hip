@@ -167,15 +167,18 @@ __device__ inline Q1Lin q1_lin(const Q1Src &s) {
 // Two such workgroups fit a CU, so one's staging and barriers overlap the other's MFMAs.  The staging
 // loads follow the quartet's storage order: bra-fastest quartets (Ib > Jb) are read along (u,w),
 // ket-fastest ones (Ib < Jb) along K.
+// The body takes the workgroup's coordinates (bx, by, bz) as arguments, so that a grouped launch over
+// many molecules can run it for each molecule's share of the grid.
 template <class MF, int MT, class E>
-__global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
-                                             int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
+__device__ __forceinline__ void q1_body(const E *__restrict__ eri, const typename MF::T *__restrict__ C1, int ldc1,
+                                        int m1, int nb, int npad, typename MF::T *__restrict__ X, int bx, int by,
+                                        int bz) {
   typedef typename MF::T T;
   constexpr int TPW = MT * 4;  // tiles per wave
   __shared__ T Ts[2][32][65];
-  const int Jb = blockIdx.x / nb, bnu = blockIdx.x % nb;
-  const int i0 = blockIdx.y * (MT * 16);  // MT tiles of 16 occupied orbitals per workgroup
-  const int K0 = 32 * blockIdx.z;
+  const int Jb = int(unsigned(bx) / unsigned(nb)), bnu = int(unsigned(bx) % unsigned(nb));
+  const int i0 = by * (MT * 16);  // MT tiles of 16 occupied orbitals per workgroup
+  const int K0 = 32 * bz;
   int br, bs;
   zq_unpair(Jb, br, bs);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -281,6 +284,12 @@ __global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const ty
       if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = T(MF::val(acc[t], j));
     }
   }
+}
+
+template <class MF, int MT, class E>
+__global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
+                                             int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
+  q1_body<MF, MT, E>(eri, C1, ldc1, m1, nb, npad, X, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------- Q1, BF16X3
@@ -446,14 +455,15 @@ __device__ inline void gemm_stage(T (*As)[32 * TM + 1], T (*Bs)[32 * TN + 1], co
   }
 }
 
-template <class MF, int TM = 2, int TN = 2>
-__global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
+// Workgroup (bx, by, bz): the (32 TM) x (32 TN) output tile (by, bx) of batch entry bz.
+template <class MF, int TM, int TN>
+__device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int bx, int by, int bz) {
   typedef typename MF::T T;
   constexpr int WM = 32 * TM, WN = 32 * TN;
   __shared__ T As[2][16][WM + 1];
   __shared__ T Bs[2][16][WN + 1];
-  const int b = blockIdx.z;
-  const int m0 = blockIdx.y * WM, n0 = blockIdx.x * WN;
+  const int b = bz;
+  const int m0 = by * WM, n0 = bx * WN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
   const T *A = g.A + b * g.sAb;
@@ -514,6 +524,11 @@ __global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
         if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = T(MF::val(acc[x][y], j));
       }
     }
+}
+
+template <class MF, int TM = 2, int TN = 2>
+__global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
+  gemm_body<MF, TM, TN>(g, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // BF16X3: FP32 operands split into bf16 hi/lo while staged; 64 x 64 tiles, BK = 32 (one k-step).
@@ -601,11 +616,11 @@ __global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
 // y = T[b][a]), a diagonal element x^2 / d.  Every thread accumulates its positions in a fixed order and
 // the threads are summed in a fixed tree: partial[i*o + j], bitwise reproducible.
 template <class T>
-__global__ __launch_bounds__(256) void zq_mp2_pairs(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
-                                                    double *__restrict__ partial) {
+__device__ __forceinline__ void mp2_pairs_body(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
+                                               double *__restrict__ partial, int ij) {
   __shared__ double X[32][33], Yt[32][33];
   __shared__ double red[256];
-  const int ij = blockIdx.x, i = ij / o, j = ij % o;
+  const int i = ij / o, j = ij % o;
   const T *t = t4 + size_t(ij) * v * v;
   const double eij = e[i] + e[j];
   const double *ev = e + o;
@@ -645,13 +660,23 @@ __global__ __launch_bounds__(256) void zq_mp2_pairs(const T *__restrict__ t4, co
   if (tid == 0) partial[ij] = red[0];
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void zq_mp2_pairs(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
+                                                    double *__restrict__ partial) {
+  mp2_pairs_body<T>(t4, e, o, v, partial, blockIdx.x);
+}
+
 // *out = sum of x[0..n): lane l adds x[l], x[l + 64], ... in order, then a fixed butterfly.
-__global__ __launch_bounds__(64) void zq_sum_wave(const double *__restrict__ x, int n, double *__restrict__ out) {
+__device__ __forceinline__ void sum_wave_body(const double *__restrict__ x, int n, double *__restrict__ out) {
   double s = 0.0;
   for (int k = threadIdx.x; k < n; k += 64) s += x[k];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if (threadIdx.x == 0) *out = s;
+}
+
+__global__ __launch_bounds__(64) void zq_sum_wave(const double *__restrict__ x, int n, double *__restrict__ out) {
+  sum_wave_body(x, n, out);
 }
 
 __global__ void zq_widen(const float *__restrict__ x, size_t n, double *__restrict__ y) {
@@ -716,9 +741,9 @@ int upload(zq_ctx *ctx, const char *name, const std::vector<T> &h, T **d) {
 // The coefficient slices of run_chain built on the device from C (n x n, already on the device): the
 // same element conversions as the host staging, so the values are identical.
 template <class T>
-__global__ void zq_c_slices(const double *__restrict__ C, int n, int np, int m1, int m2, int m3, int off3,
-                            T *__restrict__ c1, T *__restrict__ c2t, T *__restrict__ c3) {
-  const int mu = blockIdx.x;  // np rows
+__device__ __forceinline__ void c_slices_body(const double *__restrict__ C, int n, int np, int m1, int m2, int m3,
+                                              int off3, T *__restrict__ c1, T *__restrict__ c2t, T *__restrict__ c3,
+                                              int mu) {
   for (int t = threadIdx.x; t < m1 + m2 + m3; t += blockDim.x) {
     if (t < m1) {
       c1[size_t(mu) * m1 + t] = mu < n ? T(C[size_t(mu) * n + t]) : T(0);
@@ -730,6 +755,12 @@ __global__ void zq_c_slices(const double *__restrict__ C, int n, int np, int m1,
       c3[size_t(mu) * m3 + a] = mu < n ? T(C[size_t(mu) * n + off3 + a]) : T(0);
     }
   }
+}
+
+template <class T>
+__global__ void zq_c_slices(const double *__restrict__ C, int n, int np, int m1, int m2, int m3, int off3,
+                            T *__restrict__ c1, T *__restrict__ c2t, T *__restrict__ c3) {
+  c_slices_body<T>(C, n, np, m1, m2, m3, off3, c1, c2t, c3, blockIdx.x);  // one workgroup per row mu < np
 }
 
 // BF16X3 Q1 operand: C1 split into bf16 hi + lo and transposed, [i][mu] with leading dimension ldt.
@@ -1049,6 +1080,234 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr) {
   Arena ar{ctx};
   return mp2_in(ctx, C, e, nocc, precision, e_corr, ar);
+}
+
+// ---------------------------------------------------------------- grouped MP2 of many molecules
+// One launch per stage (C slices, Q1, Q2, Q3, Q4, pair sums, total) over every molecule of a group.
+// Workgroup g of a stage belongs to the molecule m with start[m] <= g < start[m + 1] and runs the
+// single-molecule body at its local coordinates, so each molecule's arithmetic is exactly zq_mp2's: the
+// same kernels on the same shapes (Q1 with one 16-row tile of occupied orbitals per workgroup, which
+// changes which workgroup computes a tile, not how it is summed).
+namespace {
+
+template <class T, class E>
+struct GJob {
+  const E *eri;
+  const double *C, *e;
+  T *c1, *c2t, *c3, *X, *O;
+  double *partial, *out;
+  int n, np, nb, Nb, o, v, groups;
+  GemmArgs<T> g[3];  // Q2, Q3, Q4
+};
+
+enum { GS_SLICES, GS_Q1, GS_Q2, GS_Q3, GS_Q4, GS_PAIRS, GS_COUNT };
+
+// The molecule whose share of a stage's grid holds workgroup blk: the largest m with start[m] <= blk
+// (start[nmol] is the stage's workgroup total, so molecules with no share are passed over).
+__device__ inline int group_find(const int *__restrict__ start, int nmol, int blk) {
+  int lo = 0, hi = nmol;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(256) void zq_group_slices(const GJob<T, E> *__restrict__ jobs, const int *__restrict__ start,
+                                                       int nmol) {
+  const int m = group_find(start, nmol, blockIdx.x);
+  const GJob<T, E> J = jobs[m];
+  c_slices_body<T>(J.C, J.n, J.np, J.o, J.o, J.v, J.o, J.c1, J.c2t, J.c3, int(blockIdx.x) - start[m]);
+}
+
+template <class MF, class E>
+__global__ __launch_bounds__(256) void zq_group_q1(const GJob<typename MF::T, E> *__restrict__ jobs,
+                                                   const int *__restrict__ start, int nmol) {
+  const int m = group_find(start, nmol, blockIdx.x);
+  const GJob<typename MF::T, E> J = jobs[m];
+  const int l = int(blockIdx.x) - start[m], gx = J.Nb * J.nb, t = l / gx;
+  q1_body<MF, 1, E>(J.eri, J.c1, J.o, J.o, J.nb, J.np, J.X, l - t * gx, t % J.groups, t / J.groups);
+}
+
+template <class MF, class E>
+__global__ __launch_bounds__(256) void zq_group_gemm(const GJob<typename MF::T, E> *__restrict__ jobs,
+                                                     const int *__restrict__ start, int nmol, int q) {
+  const int m = group_find(start, nmol, blockIdx.x);
+  const GemmArgs<typename MF::T> g = jobs[m].g[q];
+  const int l = int(blockIdx.x) - start[m], gx = (g.N + 63) / 64, gy = (g.M + 63) / 64, t = l / gx;
+  gemm_body<MF, 2, 2>(g, l - t * gx, t % gy, t / gy);
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(256) void zq_group_pairs(const GJob<T, E> *__restrict__ jobs, const int *__restrict__ start,
+                                                      int nmol) {
+  const int m = group_find(start, nmol, blockIdx.x);
+  const GJob<T, E> J = jobs[m];
+  mp2_pairs_body<T>(J.O, J.e, J.o, J.v, J.partial, int(blockIdx.x) - start[m]);
+}
+
+template <class T, class E>
+__global__ __launch_bounds__(64) void zq_group_sum(const GJob<T, E> *__restrict__ jobs) {
+  const GJob<T, E> J = jobs[blockIdx.x];
+  sum_wave_body(J.partial, J.o * J.o, J.out);
+}
+
+// Device bytes of one molecule's pieces (each 256-byte aligned) for element type T.
+size_t group_bytes(int n, int o, size_t t) {
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t nb = (n + ZQ_BLK - 1) / ZQ_BLK, np = nb * ZQ_BLK, Nb = nb * (nb + 1) / 2, v = n - o, oo = size_t(o) * o;
+  return al(np * o * t) + al(o * np * t) + al(np * v * t) + al(Nb * 64 * o * np * t) + al(oo * np * np * t) +
+         al(oo * np * v * t) + al(oo * v * v * t) + al(oo * 8 + 8);
+}
+
+constexpr size_t kGroupCap = size_t(8) << 30;  // device bytes of one group's intermediates
+
+template <class T, class MF, class E>
+int group_run(hipStream_t stream, const std::vector<ZqGroupMol> &mols, void **ws, size_t *ws_bytes, double *e_corr,
+              std::string &err) {
+  typedef GJob<T, E> Job;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const int M = int(mols.size());
+  for (int c0 = 0; c0 < M;) {
+    // a chunk of consecutive molecules whose pieces fit the cap (at least one molecule)
+    int c1 = c0;
+    size_t bytes = 0;
+    while (c1 < M) {
+      const size_t b = group_bytes(mols[c1].n, mols[c1].nocc, sizeof(T));
+      if (c1 > c0 && bytes + b > kGroupCap) break;
+      bytes += b;
+      ++c1;
+    }
+    const int nm = c1 - c0;
+    const size_t head = al(nm * sizeof(Job)) + al(size_t(GS_COUNT) * (nm + 1) * sizeof(int)) + al(nm * 8);
+    if (head + bytes > *ws_bytes) {
+      if (*ws) (void)hipFree(*ws);
+      *ws = nullptr, *ws_bytes = 0;
+      if (hipMalloc(ws, head + bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ws = nullptr;
+        err = "grouped MP2: out of device memory";
+        return ZQ_ERR_OOM;
+      }
+      *ws_bytes = head + bytes;
+    }
+    char *base = (char *)*ws;
+    Job *d_jobs = (Job *)base;
+    int *d_start = (int *)(base + al(nm * sizeof(Job)));
+    double *d_out = (double *)(base + al(nm * sizeof(Job)) + al(size_t(GS_COUNT) * (nm + 1) * sizeof(int)));
+    size_t off = head;
+    auto take = [&](size_t b) {
+      char *p = base + off;
+      off += al(b);
+      return p;
+    };
+    std::vector<Job> jobs(nm);
+    std::vector<int> start(size_t(GS_COUNT) * (nm + 1), 0);
+    for (int k = 0; k < nm; ++k) {
+      const ZqGroupMol &mo = mols[c0 + k];
+      Job &J = jobs[k];
+      J.n = mo.n, J.nb = (mo.n + ZQ_BLK - 1) / ZQ_BLK, J.np = J.nb * ZQ_BLK, J.Nb = J.nb * (J.nb + 1) / 2;
+      J.o = mo.nocc, J.v = mo.n - mo.nocc, J.groups = (mo.nocc + 15) / 16;
+      const size_t np = J.np, o = J.o, v = J.v;
+      if constexpr (std::is_same<E, float>::value)
+        J.eri = mo.eri32;
+      else
+        J.eri = mo.eri;
+      J.C = mo.C, J.e = mo.e;
+      J.c1 = (T *)take(np * o * sizeof(T));
+      J.c2t = (T *)take(o * np * sizeof(T));
+      J.c3 = (T *)take(np * v * sizeof(T));
+      J.X = (T *)take(size_t(J.Nb) * 64 * o * np * sizeof(T));
+      T *Y = (T *)take(o * o * np * np * sizeof(T));
+      T *W = (T *)take(o * o * np * v * sizeof(T));
+      J.O = (T *)take(o * o * v * v * sizeof(T));
+      J.partial = (double *)take(o * o * 8 + 8);
+      J.out = d_out + k;
+      // Q2: batch over i; A = C2T (o x np), B gathered from X, out Y[i][j][sig][nu]
+      GemmArgs<T> &g2 = J.g[0];
+      g2 = GemmArgs<T>{};
+      g2.A = J.c2t, g2.sAb = 0, g2.sAm = J.np, g2.sAk = 1;
+      g2.B = J.X, g2.sBb = J.np, g2.mode = 1, g2.npad = J.np, g2.sXs = (long long)o * np;
+      g2.C = Y, g2.sCb = (long long)o * np * np, g2.sCm = (long long)np * np, g2.sCn = 1;
+      g2.M = J.o, g2.N = J.np * J.np, g2.K = J.np;
+      // Q3: batch over (i,j); A = Y_ij [sig][nu], B = C3 [nu][a], out W[i][j][sig][a]
+      GemmArgs<T> &g3 = J.g[1];
+      g3 = GemmArgs<T>{};
+      g3.A = Y, g3.sAb = (long long)np * np, g3.sAm = J.np, g3.sAk = 1;
+      g3.B = J.c3, g3.sBb = 0, g3.sBk = J.v, g3.sBn = 1;
+      g3.C = W, g3.sCb = (long long)np * v, g3.sCm = J.v, g3.sCn = 1;
+      g3.M = J.np, g3.N = J.v, g3.K = J.np;
+      // Q4: batch over (i,j); A[a][sig] = W[sig][a], B = C3 [sig][b], out[i][j][a][b]
+      GemmArgs<T> &g4 = J.g[2];
+      g4 = GemmArgs<T>{};
+      g4.A = W, g4.sAb = (long long)np * v, g4.sAm = 1, g4.sAk = J.v;
+      g4.B = J.c3, g4.sBb = 0, g4.sBk = J.v, g4.sBn = 1;
+      g4.C = J.O, g4.sCb = (long long)v * v, g4.sCm = J.v, g4.sCn = 1;
+      g4.M = J.v, g4.N = J.v, g4.K = J.np;
+      const long long cnt[GS_COUNT] = {
+          J.np,
+          (long long)J.Nb * J.nb * J.groups * 2,
+          (long long)((g2.N + 63) / 64) * ((g2.M + 63) / 64) * J.o,
+          (long long)((g3.N + 63) / 64) * ((g3.M + 63) / 64) * J.o * J.o,
+          (long long)((g4.N + 63) / 64) * ((g4.M + 63) / 64) * J.o * J.o,
+          (long long)J.o * J.o};
+      for (int s = 0; s < GS_COUNT; ++s) {
+        const long long next = start[size_t(s) * (nm + 1) + k] + cnt[s];
+        if (next > (1LL << 30)) {
+          err = "grouped MP2: too many workgroups in one stage";
+          return ZQ_ERR_INVALID;
+        }
+        start[size_t(s) * (nm + 1) + k + 1] = int(next);
+      }
+    }
+    if (hipMemcpyAsync(d_jobs, jobs.data(), nm * sizeof(Job), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(d_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice, stream) != hipSuccess) {
+      err = "grouped MP2: upload of the job table";
+      return ZQ_ERR_LAUNCH;
+    }
+    auto st = [&](int s) { return d_start + size_t(s) * (nm + 1); };
+    auto total = [&](int s) { return unsigned(start[size_t(s) * (nm + 1) + nm]); };
+    hipLaunchKernelGGL((zq_group_slices<T, E>), dim3(total(GS_SLICES)), dim3(256), 0, stream, d_jobs, st(GS_SLICES), nm);
+    hipLaunchKernelGGL((zq_group_q1<MF, E>), dim3(total(GS_Q1)), dim3(256), 0, stream, d_jobs, st(GS_Q1), nm);
+    for (int q = 0; q < 3; ++q)
+      hipLaunchKernelGGL((zq_group_gemm<MF, E>), dim3(total(GS_Q2 + q)), dim3(256), 0, stream, d_jobs, st(GS_Q2 + q), nm, q);
+    hipLaunchKernelGGL((zq_group_pairs<T, E>), dim3(total(GS_PAIRS)), dim3(256), 0, stream, d_jobs, st(GS_PAIRS), nm);
+    hipLaunchKernelGGL((zq_group_sum<T, E>), dim3(unsigned(nm)), dim3(64), 0, stream, d_jobs);
+    if (hipGetLastError() != hipSuccess) {
+      err = "grouped MP2: launch";
+      return ZQ_ERR_LAUNCH;
+    }
+    if (hipMemcpyAsync(e_corr + c0, d_out, nm * 8, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+      err = "grouped MP2: read-back";
+      return ZQ_ERR_LAUNCH;
+    }
+    c0 = c1;
+  }
+  return ZQ_OK;
+}
+
+}  // namespace
+
+int zq_mp2_grouped(hipStream_t stream, const std::vector<ZqGroupMol> &mols, int precision, void **ws, size_t *ws_bytes,
+                   double *e_corr, std::string &err) {
+  for (const ZqGroupMol &m : mols)
+    if (!m.eri || !m.C || !m.e || m.n <= 0 || m.nocc <= 0 || m.nocc >= m.n ||
+        ((precision == ZQ_PREC_FP32 || precision == ZQ_PREC_FP32C) && !m.eri32)) {
+      err = "grouped MP2: bad molecule";
+      return ZQ_ERR_INVALID;
+    }
+  if (mols.empty()) return ZQ_OK;
+  switch (precision) {
+    case ZQ_PREC_FP64: return group_run<double, MF64, double>(stream, mols, ws, ws_bytes, e_corr, err);
+    case ZQ_PREC_FP32: return group_run<float, MF32, float>(stream, mols, ws, ws_bytes, e_corr, err);
+    case ZQ_PREC_FP32C: return group_run<float, MF32C, float>(stream, mols, ws, ws_bytes, e_corr, err);
+    default: err = "grouped MP2: FP64, FP32 or FP32C"; return ZQ_ERR_INVALID;
+  }
 }
 
 // Called by zq_create with `device` current: the device address of this device's static scratch.

@@ -374,6 +374,9 @@ __global__ __launch_bounds__(512) void zq_q1_bf16x3(const double *__restrict__ e
 // ---------------------------------------------------------------- batched GEMM
 // C[b][m][n] = sum_k A[b][m][k] B[b][k][n] with arbitrary element strides.  Mode 1 gathers B from X:
 // B[lam][col = sig*npad + nu] = X[slot(lam, sig)][b][nu] (slot stride sXs; B already offset by b).
+// Batch entry b = b0 + blockIdx.z.  tri 0: A, B, C at b sAb, b sBb, b sCb.  tri 1: the same, with rows
+// m <= b only (Q2 over j <= i).  tri 2: b is the pair index p = i (i + 1) / 2 + j of i >= j, A at
+// i sAi + j sAj (Q3 over the pairs of a symmetric transform).
 template <class T>
 struct GemmArgs {
   const T *A;
@@ -386,7 +389,33 @@ struct GemmArgs {
   int mode;
   int npad;
   long long sXs;
+  int tri, b0;
+  long long sAi, sAj;
 };
+
+// (i, j), i >= j, of the pair index p = i (i + 1) / 2 + j.
+__device__ inline void pair_of(int p, int &i, int &j) {
+  int r = int((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= p) ++r;
+  while (r * (r + 1) / 2 > p) --r;
+  i = r, j = p - r * (r + 1) / 2;
+}
+
+// Operand bases and the row count of batch entry b.
+template <class T>
+__device__ inline void gemm_batch(const GemmArgs<T> &g, int b, const T *&A, const T *&B, T *&C, int &M) {
+  M = g.M;
+  if (g.tri == 2) {
+    int i, j;
+    pair_of(b, i, j);
+    A = g.A + i * g.sAi + j * g.sAj;
+  } else {
+    A = g.A + b * g.sAb;
+    if (g.tri == 1) M = min(g.M, b + 1);
+  }
+  B = g.B + b * g.sBb;
+  C = g.C + b * g.sCb;
+}
 
 template <class T>
 __device__ inline T gemm_b(const GemmArgs<T> &g, const T *B, int k, int n) {
@@ -462,12 +491,14 @@ __device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int
   constexpr int WM = 32 * TM, WN = 32 * TN;
   __shared__ T As[2][16][WM + 1];
   __shared__ T Bs[2][16][WN + 1];
-  const int b = bz;
   const int m0 = by * WM, n0 = bx * WN;
+  const T *A, *B;
+  T *C;
+  int M;
+  gemm_batch(g, g.b0 + bz, A, B, C, M);
+  if (m0 >= M) return;  // whole workgroup: rows of a tri 1 entry beyond its pair
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
-  const T *A = g.A + b * g.sAb;
-  const T *B = g.B + b * g.sBb;
   typename MF::acc acc[TM][TN];
 #pragma unroll
   for (int x = 0; x < TM; ++x)
@@ -481,7 +512,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int y = 0; y < TN; ++y) live[x][y] = m0 + wm + 16 * x < g.M && n0 + wn + 16 * y < g.N;
+    for (int y = 0; y < TN; ++y) live[x][y] = m0 + wm + 16 * x < M && n0 + wn + 16 * y < g.N;
   T ra[2 * TM], rb[2 * TN];
   gemm_load<MF, TM, TN>(g, A, B, m0, n0, 0, a_kfast, b_nfast, ra, rb);
   gemm_stage<T, TM, TN>(As[0], Bs[0], ra, rb, a_kfast, b_nfast);
@@ -512,7 +543,6 @@ __device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int
     __syncthreads();
     cur ^= 1;
   }
-  T *C = g.C + b * g.sCb;
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
@@ -521,7 +551,7 @@ __device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int m = m0 + wm + x * 16 + MF::row(lane, j);
-        if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = T(MF::val(acc[x][y], j));
+        if (m < M && n < g.N) C[m * g.sCm + n * g.sCn] = T(MF::val(acc[x][y], j));
       }
     }
 }
@@ -537,12 +567,14 @@ __global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
   __shared__ __attribute__((aligned(16))) bf16 Al[64][40];
   __shared__ __attribute__((aligned(16))) bf16 Bh[64][40];
   __shared__ __attribute__((aligned(16))) bf16 Bl[64][40];
-  const int b = blockIdx.z;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const float *A, *B;
+  float *C;
+  int M;
+  gemm_batch(g, g.b0 + int(blockIdx.z), A, B, C, M);
+  if (m0 >= M) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const float *A = g.A + b * g.sAb;
-  const float *B = g.B + b * g.sBb;
   f4 big[2][2], small[2][2];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
@@ -595,7 +627,6 @@ __global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
         small[x][y] = mma_bf16(al[x], bh[y], small[x][y]);
       }
   }
-  float *C = g.C + b * g.sCb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -604,7 +635,7 @@ __global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int m = m0 + wm + x * 16 + bf16_row(lane, j);
-        if (m < g.M && n < g.N) C[m * g.sCm + n * g.sCn] = big[x][y][j] + small[x][y][j];
+        if (m < M && n < g.N) C[m * g.sCm + n * g.sCn] = big[x][y][j] + small[x][y][j];
       }
     }
 }
@@ -614,13 +645,20 @@ __global__ __launch_bounds__(256) void zq_gemm_bf16x3(GemmArgs<float> g) {
 // block is taken in 32 x 32 tile pairs (A <= B): tiles T[A][B] and T[B][A] are staged through LDS along
 // their rows, and each (a,b), a < b, contributes with its mirror 2 (x^2 - x y + y^2) / d (x = T[a][b],
 // y = T[b][a]), a diagonal element x^2 / d.  Every thread accumulates its positions in a fixed order and
-// the threads are summed in a fixed tree: partial[i*o + j], bitwise reproducible.
+// the threads are summed in a fixed tree: partial[ij], bitwise reproducible.
+// packed: ij is the pair index of i >= j (blocks of the symmetric transform's pairs only) and an i > j
+// block counts twice, for its mirror (j,i), whose sum is the same (its block is this one transposed and
+// its denominators are those of (b,a)); otherwise ij = i o + j.
 template <class T>
 __device__ __forceinline__ void mp2_pairs_body(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
-                                               double *__restrict__ partial, int ij) {
+                                               double *__restrict__ partial, int ij, bool packed) {
   __shared__ double X[32][33], Yt[32][33];
   __shared__ double red[256];
-  const int i = ij / o, j = ij % o;
+  int i, j;
+  if (packed)
+    pair_of(ij, i, j);
+  else
+    i = ij / o, j = ij % o;
   const T *t = t4 + size_t(ij) * v * v;
   const double eij = e[i] + e[j];
   const double *ev = e + o;
@@ -657,13 +695,29 @@ __device__ __forceinline__ void mp2_pairs_body(const T *__restrict__ t4, const d
     if (tid < w) red[tid] += red[tid + w];
     __syncthreads();
   }
-  if (tid == 0) partial[ij] = red[0];
+  if (tid == 0) partial[ij] = (packed && i != j) ? 2.0 * red[0] : red[0];
 }
 
 template <class T>
 __global__ __launch_bounds__(256) void zq_mp2_pairs(const T *__restrict__ t4, const double *__restrict__ e, int o, int v,
-                                                    double *__restrict__ partial) {
-  mp2_pairs_body<T>(t4, e, o, v, partial, blockIdx.x);
+                                                    double *__restrict__ partial, int packed) {
+  mp2_pairs_body<T>(t4, e, o, v, partial, blockIdx.x, packed != 0);
+}
+
+// The full out[i][j][a][b] (m1 x m1 x m3 x m3) of a symmetric transform from its pair blocks O[p][a][b],
+// i >= j: out[i][j] = O[p(i,j)], out[j][i] = O[p(i,j)] transposed.  One workgroup per (i,j).
+template <class T>
+__global__ __launch_bounds__(256) void zq_expand_pairs(const T *__restrict__ O, int m1, int m3, T *__restrict__ out) {
+  const int ij = blockIdx.x, i = ij / m1, j = ij - i * m1;
+  const bool lower = i >= j;
+  const int p = lower ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+  const T *src = O + size_t(p) * m3 * m3;
+  T *dst = out + size_t(ij) * m3 * m3;
+  const int mm = m3 * m3;
+  for (int x = threadIdx.x; x < mm; x += 256) {
+    const int a = x / m3, b = x - a * m3;
+    dst[x] = lower ? src[x] : src[size_t(b) * m3 + a];
+  }
 }
 
 // *out = sum of x[0..n): lane l adds x[l], x[l + 64], ... in order, then a fixed butterfly.
@@ -687,7 +741,7 @@ __global__ void zq_widen(const float *__restrict__ x, size_t n, double *__restri
 // FULL transform's n^2 (i,j) pairs from n = 256 on) run as several launches over consecutive slices.
 template <class T>
 GemmArgs<T> batch_slice(GemmArgs<T> g, long long b0) {
-  g.A += b0 * g.sAb, g.B += b0 * g.sBb, g.C += b0 * g.sCb;
+  g.b0 = int(b0);
   return g;
 }
 
@@ -790,22 +844,27 @@ struct Coefs {
   const double *q[4];
 };
 
-// Bytes run_chain carves from a static arena for these dimensions (each piece 256-byte aligned), the
+// Bytes run_chain carves from a static arena for these dimensions and one coefficient matrix (a
+// symmetric transform: Q3 and Q4 over the pairs i >= j), each piece 256-byte aligned: the
 // no-allocation entry point's size check.
 size_t chain_bytes(const Dims &d, int precision) {
   const size_t t = precision == ZQ_PREC_FP64 ? 8 : 4, np = d.np, m1 = d.m1, m2 = d.m2, m3 = d.m3, m4 = d.m4;
-  const size_t ldt = ((d.nb + 3) / 4) * 32;
+  const size_t ldt = ((d.nb + 3) / 4) * 32, pairs = m1 * (m1 + 1) / 2;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   size_t b = al(size_t(d.n) * d.n * 8) + al(np * m1 * t) + al(m2 * np * t) + al(np * m3 * t);
   if (precision == ZQ_PREC_BF16X3) b += 2 * al(m1 * ldt * sizeof(bf16));
-  b += al(size_t(d.Nb) * 64 * m1 * np * t) + al(m1 * m2 * np * np * t) + al(m1 * m2 * np * m3 * t) + al(m1 * m2 * m3 * m4 * t);
+  b += al(size_t(d.Nb) * 64 * m1 * np * t) + al(m1 * m2 * np * np * t) + al(pairs * np * m3 * t) + al(pairs * m3 * m4 * t);
   return b;
 }
 
 // Q1..Q4.  With a static arena (ar.base set) nothing is allocated: C goes to the arena and is sliced
-// there, FP32 forms need the resident FP32 ERI copy already made, and FP64 Q4 runs on the tile kernel.
+// there, and FP32 forms need the resident FP32 ERI copy already made.
+// A symmetric transform (the same coefficients for i and j, and for a and b) has out[j][i][b][a] =
+// out[i][j][a][b], (jb|ia) = (ia|jb): Q2 then forms the rows j <= i only, Q3 and Q4 run over the pairs
+// i >= j, and *d_out holds the pair blocks O[p][a][b], p = i (i + 1) / 2 + j (*packed = true).
+// Otherwise *d_out is out[i][j][a][b] for all i, j.
 template <class T, class MF>
-int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_out, Arena &ar) {
+int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_out, Arena &ar, bool *packed) {
   const int n = d.n, np = d.np, nb = d.nb, Nb = d.Nb, m1 = d.m1, m2 = d.m2, m3 = d.m3, m4 = d.m4;
   const bool fixed = ar.base != nullptr;
   const double *C = cf.q[0];
@@ -873,8 +932,10 @@ int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_
   }
 
   if (!d4) d4 = d3;
-  const size_t nX = size_t(Nb) * 64 * m1 * np, nY = size_t(m1) * m2 * np * np, nW = size_t(m1) * m2 * np * m3,
-               nO = size_t(m1) * m2 * m3 * m4;
+  const bool sym = cf.q[1] == cf.q[0] && cf.q[3] == cf.q[2] && m1 == m2 && m3 == m4;
+  const int nij = sym ? m1 * (m1 + 1) / 2 : m1 * m2;  // batch of Q3 and Q4
+  const size_t nX = size_t(Nb) * 64 * m1 * np, nY = size_t(m1) * m2 * np * np, nW = size_t(nij) * np * m3,
+               nO = size_t(nij) * m3 * m4;
   T *X = (T *)ar.get("X", nX * sizeof(T));
   T *Y = (T *)ar.get("Y", nY * sizeof(T));
   T *W = (T *)ar.get("W", nW * sizeof(T));
@@ -924,34 +985,37 @@ int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_
     ZQ_LAUNCH_CHECK(ctx, "ao2mo_q1");
   }
   GemmArgs<T> g2{}, g3{}, g4{};
-  // Q2: batch over i; A = C2T (m2 x np), B gathered from X, out Y[i][j][sig][nu]
+  // Q2: batch over i; A = C2T (m2 x np), B gathered from X, out Y[i][j][sig][nu] (j <= i if symmetric)
   g2.A = d2t, g2.sAb = 0, g2.sAm = np, g2.sAk = 1;
   g2.B = X, g2.sBb = np, g2.mode = 1, g2.npad = np, g2.sXs = (long long)m1 * np;
   g2.C = Y, g2.sCb = (long long)m2 * np * np, g2.sCm = (long long)np * np, g2.sCn = 1;
   g2.M = m2, g2.N = np * np, g2.K = np;
-  // Q3: batch over (i,j); A = Y_ij [sig][nu], B = C3 [nu][a], out W[i][j][sig][a]
+  g2.tri = sym ? 1 : 0;
+  // Q3: batch over (i,j) (the pairs i >= j if symmetric); A = Y_ij [sig][nu], B = C3 [nu][a], out W[ij][sig][a]
   g3.A = Y, g3.sAb = (long long)np * np, g3.sAm = np, g3.sAk = 1;
   g3.B = d3, g3.sBb = 0, g3.sBk = m3, g3.sBn = 1;
   g3.C = W, g3.sCb = (long long)np * m3, g3.sCm = m3, g3.sCn = 1;
   g3.M = np, g3.N = m3, g3.K = np;
-  // Q4: batch over (i,j); A[a][sig] = W[sig][a], B = C4 [sig][b], out[i][j][a][b]
+  if (sym) g3.tri = 2, g3.sAi = (long long)m2 * np * np, g3.sAj = (long long)np * np;
+  // Q4: batch over the same (i,j); A[a][sig] = W[sig][a], B = C4 [sig][b], out[ij][a][b]
   g4.A = W, g4.sAb = (long long)np * m3, g4.sAm = 1, g4.sAk = m3;
   g4.B = d4, g4.sBb = 0, g4.sBk = m4, g4.sBn = 1;
   g4.C = O, g4.sCb = (long long)m3 * m4, g4.sCm = m4, g4.sCn = 1;
   g4.M = m3, g4.N = m4, g4.K = np;
+  *packed = sym;
   if constexpr (sizeof(T) == 4) {
     if (precision == ZQ_PREC_BF16X3) {
       if (int rc = launch_gemm_bf16x3(ctx, g2, m1, "ao2mo_q2")) return rc;
-      if (int rc = launch_gemm_bf16x3(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
-      if (int rc = launch_gemm_bf16x3(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
+      if (int rc = launch_gemm_bf16x3(ctx, g3, nij, "ao2mo_q3")) return rc;
+      if (int rc = launch_gemm_bf16x3(ctx, g4, nij, "ao2mo_q4")) return rc;
       *d_out = O;
       return ZQ_OK;
     }
   }
   if (int rc = launch_gemm<MF>(ctx, g2, m1, "ao2mo_q2")) return rc;
-  if (int rc = launch_gemm<MF>(ctx, g3, m1 * m2, "ao2mo_q3")) return rc;
+  if (int rc = launch_gemm<MF>(ctx, g3, nij, "ao2mo_q3")) return rc;
   // Q4 on the tile kernel in every precision (no library GEMM in the transform)
-  if (int rc = launch_gemm<MF>(ctx, g4, m1 * m2, "ao2mo_q4")) return rc;
+  if (int rc = launch_gemm<MF>(ctx, g4, nij, "ao2mo_q4")) return rc;
   *d_out = O;
   return ZQ_OK;
 }
@@ -985,22 +1049,38 @@ int make_dims(zq_ctx *ctx, int nocc, int block, int precision, Dims &d) {
   return ZQ_OK;
 }
 
+// packed == nullptr: *d_out is the full out[i][j][a][b] (a symmetric chain's pair blocks are expanded);
+// else *d_out may hold the pair blocks, as *packed says.
+template <class T>
+int expand_if(zq_ctx *ctx, const Dims &d, Arena &ar, T *o, bool pk, bool *packed, void **d_out) {
+  if (pk && !packed) {
+    T *full = (T *)ar.get("OUTF", size_t(d.m1) * d.m2 * d.m3 * d.m4 * sizeof(T));
+    if (!full) return zq_fail(ctx, ar.base ? ZQ_ERR_INVALID : ZQ_ERR_OOM, "transform output");
+    hipLaunchKernelGGL(zq_expand_pairs<T>, dim3(unsigned(d.m1 * d.m2)), dim3(256), 0, ctx->stream, o, d.m1, d.m3, full);
+    ZQ_LAUNCH_CHECK(ctx, "zq_expand_pairs");
+    o = full;
+  }
+  if (packed) *packed = pk;
+  *d_out = o;
+  return ZQ_OK;
+}
+
 int transform_in(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, Arena &ar, void **d_out, int dims[4],
-                 bool *is_f32) {
+                 bool *is_f32, bool *packed = nullptr) {
   dims[0] = d.m1, dims[1] = d.m2, dims[2] = d.m3, dims[3] = d.m4;
+  bool pk = false;
   if (precision == ZQ_PREC_FP64) {
     *is_f32 = false;
     double *o = nullptr;
-    int rc = run_chain<double, MF64>(ctx, cf, d, precision, &o, ar);
-    *d_out = o;
-    return rc;
+    if (int rc = run_chain<double, MF64>(ctx, cf, d, precision, &o, ar, &pk)) return rc;
+    return expand_if(ctx, d, ar, o, pk, packed, d_out);
   }
   *is_f32 = true;
   float *o = nullptr;
-  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, cf, d, precision, &o, ar)
-                                      : run_chain<float, MF32>(ctx, cf, d, precision, &o, ar);
-  *d_out = o;
-  return rc;
+  int rc = precision == ZQ_PREC_FP32C ? run_chain<float, MF32C>(ctx, cf, d, precision, &o, ar, &pk)
+                                      : run_chain<float, MF32>(ctx, cf, d, precision, &o, ar, &pk);
+  if (rc) return rc;
+  return expand_if(ctx, d, ar, o, pk, packed, d_out);
 }
 }  // namespace
 
@@ -1042,23 +1122,24 @@ int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precisio
     return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2_noalloc: this molecule needs more than the 64 MiB static scratch");
   void *T = nullptr;
   int dims[4];
-  bool f32 = false;
-  if (int rc = transform_in(ctx, Coefs{{C, C, C, C}}, dm, precision, ar, &T, dims, &f32)) return rc;
-  const int o = dims[0], v = dims[2];
+  bool f32 = false, packed = false;
+  if (int rc = transform_in(ctx, Coefs{{C, C, C, C}}, dm, precision, ar, &T, dims, &f32, &packed)) return rc;
+  const int o = dims[0], v = dims[2], nij = packed ? o * (o + 1) / 2 : o * o;
   double *de = (double *)ar.get("mp2_e", size_t(ctx->n) * 8);
   double *part = (double *)ar.get("mp2_part", size_t(o) * o * 8 + 8);
   if (!de || !part) return zq_fail(ctx, ar.base ? ZQ_ERR_INVALID : ZQ_ERR_OOM, "mp2 scratch");
   ZQ_TRY(ctx, hipMemcpyAsync(de, e, size_t(ctx->n) * 8, hipMemcpyHostToDevice, ctx->stream));
   zq_prof_begin(ctx, "mp2_energy");
   if (f32)
-    hipLaunchKernelGGL(zq_mp2_pairs<float>, dim3(unsigned(o * o)), dim3(256), 0, ctx->stream, (const float *)T, de, o, v, part);
+    hipLaunchKernelGGL(zq_mp2_pairs<float>, dim3(unsigned(nij)), dim3(256), 0, ctx->stream, (const float *)T, de, o, v,
+                       part, int(packed));
   else
-    hipLaunchKernelGGL(zq_mp2_pairs<double>, dim3(unsigned(o * o)), dim3(256), 0, ctx->stream, (const double *)T, de, o, v,
-                       part);
-  hipLaunchKernelGGL(zq_sum_wave, dim3(1), dim3(64), 0, ctx->stream, part, o * o, part + size_t(o) * o);
+    hipLaunchKernelGGL(zq_mp2_pairs<double>, dim3(unsigned(nij)), dim3(256), 0, ctx->stream, (const double *)T, de, o, v,
+                       part, int(packed));
+  hipLaunchKernelGGL(zq_sum_wave, dim3(1), dim3(64), 0, ctx->stream, part, nij, part + nij);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "zq_mp2_pairs");
-  ZQ_TRY(ctx, hipMemcpyAsync(e_corr, part + size_t(o) * o, 8, hipMemcpyDeviceToHost, ctx->stream));
+  ZQ_TRY(ctx, hipMemcpyAsync(e_corr, part + nij, 8, hipMemcpyDeviceToHost, ctx->stream));
   ZQ_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return ZQ_OK;
 }
@@ -1086,8 +1167,9 @@ int zq_mp2_device(zq_ctx *ctx, const double *C, const double *e, int nocc, int p
 // One launch per stage (C slices, Q1, Q2, Q3, Q4, pair sums, total) over every molecule of a group.
 // Workgroup g of a stage belongs to the molecule m with start[m] <= g < start[m + 1] and runs the
 // single-molecule body at its local coordinates, so each molecule's arithmetic is exactly zq_mp2's: the
-// same kernels on the same shapes (Q1 with one 16-row tile of occupied orbitals per workgroup, which
-// changes which workgroup computes a tile, not how it is summed).
+// same kernels on the same shapes (the symmetric chain: Q2 rows j <= i, Q3 and Q4 over the pairs
+// i >= j; Q1 with one 16-row tile of occupied orbitals per workgroup, which changes which workgroup
+// computes a tile, not how it is summed).
 namespace {
 
 template <class T, class E>
@@ -1096,7 +1178,7 @@ struct GJob {
   const double *C, *e;
   T *c1, *c2t, *c3, *X, *O;
   double *partial, *out;
-  int n, np, nb, Nb, o, v, groups;
+  int n, np, nb, Nb, o, v, groups, pairs;
   GemmArgs<T> g[3];  // Q2, Q3, Q4
 };
 
@@ -1147,21 +1229,22 @@ __global__ __launch_bounds__(256) void zq_group_pairs(const GJob<T, E> *__restri
                                                       int nmol) {
   const int m = group_find(start, nmol, blockIdx.x);
   const GJob<T, E> J = jobs[m];
-  mp2_pairs_body<T>(J.O, J.e, J.o, J.v, J.partial, int(blockIdx.x) - start[m]);
+  mp2_pairs_body<T>(J.O, J.e, J.o, J.v, J.partial, int(blockIdx.x) - start[m], true);
 }
 
 template <class T, class E>
 __global__ __launch_bounds__(64) void zq_group_sum(const GJob<T, E> *__restrict__ jobs) {
   const GJob<T, E> J = jobs[blockIdx.x];
-  sum_wave_body(J.partial, J.o * J.o, J.out);
+  sum_wave_body(J.partial, J.pairs, J.out);
 }
 
 // Device bytes of one molecule's pieces (each 256-byte aligned) for element type T.
 size_t group_bytes(int n, int o, size_t t) {
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t nb = (n + ZQ_BLK - 1) / ZQ_BLK, np = nb * ZQ_BLK, Nb = nb * (nb + 1) / 2, v = n - o, oo = size_t(o) * o;
+  const size_t nb = (n + ZQ_BLK - 1) / ZQ_BLK, np = nb * ZQ_BLK, Nb = nb * (nb + 1) / 2, v = n - o, oo = size_t(o) * o,
+               pr = size_t(o) * (o + 1) / 2;
   return al(np * o * t) + al(o * np * t) + al(np * v * t) + al(Nb * 64 * o * np * t) + al(oo * np * np * t) +
-         al(oo * np * v * t) + al(oo * v * v * t) + al(oo * 8 + 8);
+         al(pr * np * v * t) + al(pr * v * v * t) + al(pr * 8 + 8);
 }
 
 constexpr size_t kGroupCap = size_t(8) << 30;  // device bytes of one group's intermediates
@@ -1211,8 +1294,8 @@ int group_run(hipStream_t stream, const std::vector<ZqGroupMol> &mols, void **ws
       const ZqGroupMol &mo = mols[c0 + k];
       Job &J = jobs[k];
       J.n = mo.n, J.nb = (mo.n + ZQ_BLK - 1) / ZQ_BLK, J.np = J.nb * ZQ_BLK, J.Nb = J.nb * (J.nb + 1) / 2;
-      J.o = mo.nocc, J.v = mo.n - mo.nocc, J.groups = (mo.nocc + 15) / 16;
-      const size_t np = J.np, o = J.o, v = J.v;
+      J.o = mo.nocc, J.v = mo.n - mo.nocc, J.groups = (mo.nocc + 15) / 16, J.pairs = mo.nocc * (mo.nocc + 1) / 2;
+      const size_t np = J.np, o = J.o, v = J.v, pr = J.pairs;
       if constexpr (std::is_same<E, float>::value)
         J.eri = mo.eri32;
       else
@@ -1223,25 +1306,26 @@ int group_run(hipStream_t stream, const std::vector<ZqGroupMol> &mols, void **ws
       J.c3 = (T *)take(np * v * sizeof(T));
       J.X = (T *)take(size_t(J.Nb) * 64 * o * np * sizeof(T));
       T *Y = (T *)take(o * o * np * np * sizeof(T));
-      T *W = (T *)take(o * o * np * v * sizeof(T));
-      J.O = (T *)take(o * o * v * v * sizeof(T));
-      J.partial = (double *)take(o * o * 8 + 8);
+      T *W = (T *)take(pr * np * v * sizeof(T));
+      J.O = (T *)take(pr * v * v * sizeof(T));
+      J.partial = (double *)take(pr * 8 + 8);
       J.out = d_out + k;
-      // Q2: batch over i; A = C2T (o x np), B gathered from X, out Y[i][j][sig][nu]
+      // Q2: batch over i; A = C2T (o x np), B gathered from X, out Y[i][j][sig][nu], j <= i
       GemmArgs<T> &g2 = J.g[0];
       g2 = GemmArgs<T>{};
       g2.A = J.c2t, g2.sAb = 0, g2.sAm = J.np, g2.sAk = 1;
       g2.B = J.X, g2.sBb = J.np, g2.mode = 1, g2.npad = J.np, g2.sXs = (long long)o * np;
       g2.C = Y, g2.sCb = (long long)o * np * np, g2.sCm = (long long)np * np, g2.sCn = 1;
-      g2.M = J.o, g2.N = J.np * J.np, g2.K = J.np;
-      // Q3: batch over (i,j); A = Y_ij [sig][nu], B = C3 [nu][a], out W[i][j][sig][a]
+      g2.M = J.o, g2.N = J.np * J.np, g2.K = J.np, g2.tri = 1;
+      // Q3: batch over the pairs i >= j; A = Y_ij [sig][nu], B = C3 [nu][a], out W[p][sig][a]
       GemmArgs<T> &g3 = J.g[1];
       g3 = GemmArgs<T>{};
       g3.A = Y, g3.sAb = (long long)np * np, g3.sAm = J.np, g3.sAk = 1;
       g3.B = J.c3, g3.sBb = 0, g3.sBk = J.v, g3.sBn = 1;
       g3.C = W, g3.sCb = (long long)np * v, g3.sCm = J.v, g3.sCn = 1;
       g3.M = J.np, g3.N = J.v, g3.K = J.np;
-      // Q4: batch over (i,j); A[a][sig] = W[sig][a], B = C3 [sig][b], out[i][j][a][b]
+      g3.tri = 2, g3.sAi = (long long)o * np * np, g3.sAj = (long long)np * np;
+      // Q4: batch over the pairs; A[a][sig] = W[sig][a], B = C3 [sig][b], out[p][a][b]
       GemmArgs<T> &g4 = J.g[2];
       g4 = GemmArgs<T>{};
       g4.A = W, g4.sAb = (long long)np * v, g4.sAm = 1, g4.sAk = J.v;
@@ -1252,9 +1336,9 @@ int group_run(hipStream_t stream, const std::vector<ZqGroupMol> &mols, void **ws
           J.np,
           (long long)J.Nb * J.nb * J.groups * 2,
           (long long)((g2.N + 63) / 64) * ((g2.M + 63) / 64) * J.o,
-          (long long)((g3.N + 63) / 64) * ((g3.M + 63) / 64) * J.o * J.o,
-          (long long)((g4.N + 63) / 64) * ((g4.M + 63) / 64) * J.o * J.o,
-          (long long)J.o * J.o};
+          (long long)((g3.N + 63) / 64) * ((g3.M + 63) / 64) * J.pairs,
+          (long long)((g4.N + 63) / 64) * ((g4.M + 63) / 64) * J.pairs,
+          (long long)J.pairs};
       for (int s = 0; s < GS_COUNT; ++s) {
         const long long next = start[size_t(s) * (nm + 1) + k] + cnt[s];
         if (next > (1LL << 30)) {

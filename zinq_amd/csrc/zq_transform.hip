@@ -1146,7 +1146,7 @@ int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precisio
 
 // ---- the static no-allocation scratch (zq_mp2_noalloc)
 // kNoAllocBytes hold every intermediate of the MP2 transform for n <= 72 in any precision (the
-// molecules of the batched workload: 47.5 MB for (H2O)_3 cc-pVDZ in FP64).  Each device holds its own
+// molecules of the batched workload: 41.4 MB for (H2O)_3 cc-pVDZ in FP64).  Each device holds its own
 // copy for the life of the process.  In host code the array's name is not a device address: its address
 // on each device comes from hipGetSymbolAddress, taken once per device with that device current.
 constexpr size_t kNoAllocBytes = size_t(64) << 20;

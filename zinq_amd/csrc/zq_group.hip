@@ -101,6 +101,17 @@ __device__ void wg_gemm_nt(const double *A, const double *B, double *C, int mp, 
   __syncthreads();
 }
 
+// The 16 x 16 tile (i0, j0) of X X^T for the symmetric mp x mp X held in LDS (leading dimension ld):
+// lane l (r = l % 16, q = l / 16) supplies X[i0 + r][k + q] and X[j0 + r][k + q] to each
+// v_mfma_f64_16x16x4f64; accumulator element j is row i0 + 4 j + q, column j0 + r.
+__device__ inline d4 lds_tile_sq(const double *Xl, int ld, int mp, int i0, int j0) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  const double *a = Xl + (i0 + r) * ld + q, *b = Xl + (j0 + r) * ld + q;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < mp; k += 4) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k], b[k], acc, 0, 0, 0);
+  return acc;
+}
+
 // Bordered DIIS system (m + 1) x (m + 1) by LU with partial pivoting, the steps of zq::getrf/getrs.
 __device__ bool lu_solve(int n, double *A, double *b) {
   int piv[kHist + 1];
@@ -137,11 +148,15 @@ __device__ bool lu_solve(int n, double *A, double *b) {
 }
 
 // One SCF step (MODE_ITER) or the core-Hamiltonian guess (MODE_GUESS) of molecule blockIdx.x.
+// The SP2 iterate X lives in dynamic LDS (mp x (mp + 1) doubles, sized for the batch's largest mp) for
+// the whole step loop: each wave computes its 16 x 16 tiles of X^2 from LDS into registers, and after a
+// barrier writes its tiles' next X back in place.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, const BMol *__restrict__ mols,
                                                    BState *__restrict__ st, int *__restrict__ done, int mode, int diis,
                                                    double thr, int maxit) {
   __shared__ double red[2 * (NT / 64)];
+  extern __shared__ double Xl[];
   __shared__ double Am[(kHist + 1) * (kHist + 1)], c[kHist + 1];  // the bordered DIIS system (thread 0)
   const int mi = blockIdx.x, tid = threadIdx.x;
   if (mode == MODE_ITER && done[mi]) return;
@@ -153,7 +168,7 @@ __global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, cons
   double *H = base + M_H * mm, *F = base + M_F * mm, *P = base + M_P * mm, *Sm = base + M_S * mm;
   double *X = base + M_X * mm, *XT = base + M_XT * mm, *W = base + M_W * mm, *T = base + M_T * mm;
   double *FP = base + M_FP * mm, *FPS = base + M_FPS * mm, *Fpr = base + M_FPR * mm, *Xs = base + M_XS * mm;
-  double *X2 = base + M_X2 * mm, *Fh = base + kMats * mm, *Eh = Fh + kHist * mm;
+  double *Fh = base + kMats * mm, *Eh = Fh + kHist * mm;
   const double *A = H;
 
   if (mode == MODE_GUESS) {
@@ -266,40 +281,63 @@ __global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, cons
   const double pad = 1e-12 * fmax(fabs(lo), fabs(hi));
   lo -= pad, hi += pad;
   const double inv = 1.0 / fmax(hi - lo, 1e-10 * fmax(1.0, fabs(hi)));
+  const int ld = mp + 1;  // odd: the 16 rows of a tile fall in distinct LDS banks
   double t1 = 0.0, t2 = 0.0;
   for (size_t x = tid; x < mm; x += NT) {
     const int i = int(x / mp), j = int(x - size_t(i) * mp);
     double v = 0.0;
     if (i < n && j < n) v = ((i == j ? hi : 0.0) - 0.5 * (Fpr[x] + Fpr[size_t(j) * mp + i])) * inv;
-    Xs[x] = v;
+    Xl[i * ld + j] = v;
     t1 += (i == j) ? v : 0.0;
     t2 += v * v;
   }
   t1 = block_sum<NT>(t1, red);
   t2 = block_sum<NT>(t2, red);
   const double N = double(M.nocc);
+  constexpr int NW = NT / 64, kTiles = (kMaxN / 16) * (kMaxN / 16), kPer = (kTiles + NW - 1) / NW;
+  const int Tt = mp >> 4, wave = tid >> 6, lane = tid & 63, r16 = lane & 15, q16 = lane >> 4;
   int count = -1, steps = 0;
   for (; steps < 100; ++steps) {
     if (count >= 2) break;
     if (count < 0 && t1 - t2 < 1e-9) count = 0;
     if (count >= 0) ++count;
     const bool sq = fabs(t2 - N) <= fabs(2.0 * t1 - t2 - N);
-    wg_gemm_nt<NT>(Xs, Xs, X2, mp, 1.0);
-    double a1 = 0.0, a2 = 0.0;
-    for (size_t x = tid; x < mm; x += NT) {
-      const int i = int(x / mp), j = int(x - size_t(i) * mp);
-      const double v = sq ? X2[x] : 2.0 * Xs[x] - X2[x];
-      Xs[x] = v;
-      a1 += (i == j) ? v : 0.0;
-      a2 += v * v;
+    d4 acc[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int t = wave + u * NW;
+      if (t < Tt * Tt) acc[u] = lds_tile_sq(Xl, ld, mp, (t / Tt) * 16, (t % Tt) * 16);
     }
-    t1 = block_sum<NT>(a1, red);
+    __syncthreads();  // every tile of X^2 is formed before X is overwritten
+    double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int t = wave + u * NW;
+      if (t < Tt * Tt) {
+        const int ti = t / Tt, tj = t % Tt;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int i = ti * 16 + 4 * j + q16, c = tj * 16 + r16;
+          const double x2 = acc[u][j];
+          const double v = sq ? x2 : 2.0 * Xl[i * ld + c] - x2;
+          Xl[i * ld + c] = v;
+          a1 += (i == c) ? v : 0.0;
+          a2 += v * v;
+        }
+      }
+    }
+    t1 = block_sum<NT>(a1, red);  // its barriers also order the writes before the next step's reads
     t2 = block_sum<NT>(a2, red);
   }
   if (count < 2) {  // no gap at the Fermi level: this molecule stops here
     if (tid == 0) S.status = 3, S.done = 1, done[mi] = 1;
     return;
   }
+  for (size_t x = tid; x < mm; x += NT) {
+    const int i = int(x / mp), j = int(x - size_t(i) * mp);
+    Xs[x] = Xl[i * ld + j];
+  }
+  __syncthreads();
   // P = 2 X D X^T, rms(dP)
   wg_gemm_nt<NT>(X, Xs, T, mp, 1.0);
   wg_gemm_nt<NT>(T, X, FP, mp, 2.0);
@@ -411,6 +449,7 @@ struct zq_batch {
   int nitems = 0, nblk = 0;
   std::vector<long long> eri_off, ws_off;
   size_t eri_count = 0;  // doubles in eri
+  int mp_max = 16;       // largest padded size of the batch (the SP2 iterate's LDS)
   int *h_count = nullptr;
   hipGraphExec_t graph = nullptr;
   std::string graph_key;
@@ -525,6 +564,7 @@ int upload(zq_batch *b) {
     ws_total += (kMats + 2 * kHist) * mm;
     canon_max = std::max(canon_max, h.eri.size());
     bm[m] = BMol{(long long)b->ws_off[m], h.n, mp, h.nocc, 0, h.VN};
+    b->mp_max = std::max(b->mp_max, mp);
     jm[m] = JKMol{b->eri_off[m], b->ws_off[m] + M_P * (long long)mm, b->ws_off[m] + M_H * (long long)mm,
                   b->ws_off[m] + M_F * (long long)mm, mp, h.n};
   }
@@ -586,11 +626,12 @@ int upload(zq_batch *b) {
 
 void launch_step(zq_batch *b, int mode, int diis, double thr, int maxit) {
   const dim3 grid(unsigned(b->mols.size()));
+  const size_t lds = size_t(b->mp_max) * (b->mp_max + 1) * sizeof(double);  // the SP2 iterate
   if (b->nt == 512)
-    hipLaunchKernelGGL(k_batch_step<512>, grid, dim3(512), 0, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
+    hipLaunchKernelGGL(k_batch_step<512>, grid, dim3(512), lds, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
                        diis, thr, maxit);
   else
-    hipLaunchKernelGGL(k_batch_step<256>, grid, dim3(256), 0, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
+    hipLaunchKernelGGL(k_batch_step<256>, grid, dim3(256), lds, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
                        diis, thr, maxit);
 }
 

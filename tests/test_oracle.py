@@ -183,6 +183,151 @@ def test_single_primitive_ssss_closed_form(tmp_path):
     assert g[0, 0, 0, 0] == pytest.approx(N4 * 2 * pi ** 2.5 / ((2 * alpha) ** 2 * sqrt(4 * alpha)), rel=1e-13)
 
 
+def _ssss(mp, a, b, c, d):
+    """(ss|ss) of unnormalised primitive s Gaussians exp(-x|r - X|^2) as an mpmath function of the 12
+    centre coordinates (Szabo & Ostlund, Modern Quantum Chemistry, eq. A.41):
+    2 pi^(5/2) / (p q sqrt(p + q)) exp(-ab/p |AB|^2 - cd/q |CD|^2) F0(pq/(p+q) |PQ|^2)."""
+    p, q = a + b, c + d
+
+    def G(*X):
+        A, B, C, D = X[0:3], X[3:6], X[6:9], X[9:12]
+        P = [(a * A[i] + b * B[i]) / p for i in range(3)]
+        Q = [(c * C[i] + d * D[i]) / q for i in range(3)]
+        ab2 = sum((A[i] - B[i]) ** 2 for i in range(3))
+        cd2 = sum((C[i] - D[i]) ** 2 for i in range(3))
+        T = p * q / (p + q) * sum((P[i] - Q[i]) ** 2 for i in range(3))
+        F0 = mp.mpf(1) if T == 0 else mp.sqrt(mp.pi / T) / 2 * mp.erf(mp.sqrt(T))
+        return 2 * mp.pi ** 2.5 / (p * q * mp.sqrt(p + q)) * mp.exp(-a * b / p * ab2 - c * d / q * cd2) * F0
+    return G
+
+
+CART_D = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]  # xx yy zz xy xz yz
+
+
+def _mixed(mp, f, X, orders, h):
+    """The mixed derivative prod_i d^{o_i} / dX[i]^{o_i} f (orders = {i: o_i}, o_i in {1, 2}) by the
+    product of central-difference stencils (first: (f(+h) - f(-h)) / 2h, second: (f(+h) - 2f + f(-h)) / h^2;
+    error O(h^2)); at 80 digits h = 1e-8 leaves round-off near 1e-80 / h^(sum o_i)."""
+    import itertools
+    taps = {1: ((1, 1), (-1, -1)), 2: ((1, 1), (0, -2), (-1, 1))}  # (offset / h, weight)
+    idx = list(orders)
+    tot = mp.mpf(0)
+    for combo in itertools.product(*(taps[orders[i]] for i in idx)):
+        Y = list(X)
+        w = 1
+        for i, (off, wt) in zip(idx, combo):
+            Y[i] += off * h
+            w *= wt
+        tot += w * f(*Y)
+    den = 1
+    for i in idx:
+        den *= (2 * h) if orders[i] == 1 else h * h
+    return tot / den
+
+
+def _d_basis(tmp_path, shells):
+    """A Gaussian94 file with one single-primitive shell per element: shells = [(symbol, L, exponent)]."""
+    text = "".join("%s     0\n%s    1   1.00\n      %.12f       1.0000000\n****\n" % (sym, L, e) for sym, L, e in shells)
+    (tmp_path / "d.g94").write_text(text)
+    return str(tmp_path / "d.g94")
+
+
+def test_d_shell_integrals_from_derivatives_of_the_ss_closed_form(tmp_path):
+    """d-shell ERIs of the integral engine against derivatives of the (ss|ss) closed form, at 30 digits
+    (mpmath): a Cartesian d primitive (x_i - A_i)(x_j - A_j) exp(-a|r - A|^2) is
+    (d^2/dA_i dA_j + 2a delta_ij) / (4a^2) applied to the s primitive on A.
+    1. (d s|s s) at 8 seeded geometries: the engine's five pure d functions are fitted as linear
+       combinations of the six Cartesian ones; the fit must be exact (a wrong Hermite recursion is not a
+       fixed linear map of the true integrals), each combination traceless (a solid harmonic), the five
+       mutually orthogonal with equal norms (tr Q_m Q_n, the pure-d inner product), and d_xy carries the
+       analytic normalisation 4a (2a/pi)^(3/4).
+    2. (d d|d d), L = 8, against 8th mixed derivatives (the stencils of _mixed): the xy functions on
+       four centres, and the x^2 - y^2 functions (Hermite indices up to 8 in x)."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 30
+    rng = np.random.default_rng(2024)
+    ex = dict(He=0.9, H=0.6, Li=0.75, Be=0.45)
+    Ns = {k: (2 * v / np.pi) ** 0.75 for k, v in ex.items()}
+    # 1. d on He; s on H, Li, Be
+    path = _d_basis(tmp_path, [("He", "D", ex["He"]), ("H", "S", ex["H"]), ("Li", "S", ex["Li"]), ("Be", "S", ex["Be"])])
+    G = _ssss(mpmath, *(mpmath.mpf(ex[k]) for k in ("He", "H", "Li", "Be")))
+    a = ex["He"]
+    eng, cart = [], []
+    for _ in range(8):
+        xyz = rng.uniform(-1.2, 1.2, (4, 3))
+        s = MolecularSystem.from_atoms([2, 1, 3, 4], xyz, path)
+        assert s.nbf == 8
+        g = s.coulomb()
+        eng.append([g[m, 6, 5, 7] for m in range(5)])  # (d_m s_H | s_Li s_Be) = <d_m s_Li | s_H s_Be>
+        X = [mpmath.mpf(float(v)) for v in xyz.reshape(-1)]
+        row = []
+        for i, j in CART_D:
+            order = [0] * 12
+            order[i] += 1
+            order[j] += 1
+            v = mpmath.diff(G, X, order)
+            if i == j:
+                v += 2 * a * G(*X)
+            row.append(float(v / (4 * a * a)) * Ns["H"] * Ns["Li"] * Ns["Be"])
+        cart.append(row)
+    eng, cart = np.array(eng), np.array(cart)
+    T, *_ = np.linalg.lstsq(cart, eng, rcond=None)  # eng = cart @ T, T: 6 x 5
+    resid = np.abs(cart @ T - eng).max()
+    print(f"(d s|s s): max |fit residual| {resid:.1e} of max |integral| {np.abs(eng).max():.1e}")
+    assert resid < 1e-10 * np.abs(eng).max()
+    Q = np.zeros((5, 3, 3))
+    for c, (i, j) in enumerate(CART_D):
+        for m in range(5):
+            if i == j:
+                Q[m, i, i] = T[c, m]
+            else:
+                Q[m, i, j] = Q[m, j, i] = T[c, m] / 2
+    norm = np.einsum("mij,nij->mn", Q, Q)
+    assert np.abs(np.trace(Q, axis1=1, axis2=2)).max() < 1e-9 * np.sqrt(norm[0, 0])
+    np.testing.assert_allclose(norm, norm[0, 0] * np.eye(5), atol=1e-9 * norm[0, 0])
+    m_xy = int(np.argmax(np.abs(T[3])))
+    assert np.abs(np.delete(T[:, m_xy], 3)).max() < 1e-9 * abs(T[3, m_xy])  # d_xy is the pure xy function
+    assert abs(T[3, m_xy]) == pytest.approx(4 * a * (2 * a / np.pi) ** 0.75, rel=1e-10)
+    # 2. (d d|d d) of the xy functions: d on He, Li, Be, B
+    ex4 = dict(He=0.9, Li=0.7, Be=0.55, B=0.8)
+    path4 = _d_basis(tmp_path, [(k, "D", v) for k, v in ex4.items()])
+    al = [ex4[k] for k in ("He", "Li", "Be", "B")]
+    mpmath.mp.dps = 80
+    G4 = _ssss(mpmath, *(mpmath.mpf(v) for v in al))
+    mpmath.mp.dps = 30
+    Nxy = np.prod([4 * x * (2 * x / np.pi) ** 0.75 for x in al])
+    # the x^2 - y^2 function: c (xx - yy), c scaling with the primitive normalisation (alpha / a)^(7/4)
+    m_d = [m for m in range(5) if abs(T[0, m]) > 1e-3 and abs(T[0, m] + T[1, m]) < 1e-9 * abs(T[0, m])
+           and np.abs(T[2:, m]).max() < 1e-9 * abs(T[0, m])]
+    assert len(m_d) == 1
+    m_d = m_d[0]
+    cd = np.prod([T[0, m_d] * (x / a) ** 1.75 for x in al])
+    den = np.prod([4 * x * x for x in al])
+    for _ in range(2):
+        xyz = rng.uniform(-1.0, 1.0, (4, 3))
+        s = MolecularSystem.from_atoms([2, 3, 4, 5], xyz, path4)
+        assert s.nbf == 20
+        g = s.coulomb()
+        mpmath.mp.dps = 80
+        X = [mpmath.mpf(float(v)) for v in xyz.reshape(-1)]
+        h = mpmath.mpf("1e-8")
+        # (d_A d_B | d_C d_D) = <d_A d_C | d_B d_D>, functions 5k + m
+        val = g[m_xy, 10 + m_xy, 5 + m_xy, 15 + m_xy]
+        ref = float(_mixed(mpmath, G4, X, {i: 1 for i in (0, 1, 3, 4, 6, 7, 9, 10)}, h)) / den * Nxy
+        print(f"(dxy dxy|dxy dxy): engine {val:.15e} reference {ref:.15e}")
+        assert val == pytest.approx(ref, rel=1e-10, abs=1e-14)
+        # (xx - yy) on every centre: 16 terms of second derivatives, x or y on each centre (the 2a terms cancel)
+        val = g[m_d, 10 + m_d, 5 + m_d, 15 + m_d]
+        tot = mpmath.mpf(0)
+        for axes in np.ndindex(2, 2, 2, 2):
+            sign = (-1) ** sum(axes)
+            tot += sign * _mixed(mpmath, G4, X, {3 * k + ax: 2 for k, ax in enumerate(axes)}, h)
+        ref = float(tot) / den * cd
+        mpmath.mp.dps = 30
+        print(f"(dx2-y2 ...|...): engine {val:.15e} reference {ref:.15e}")
+        assert val == pytest.approx(ref, rel=1e-10, abs=1e-14)
+
+
 def test_pure_d_shells_orthonormal_and_eri_symmetric():
     s, H, S, g, VN = system_ints("ccpvdz")
     assert s.nbf == 24  # pure d: 14 on O + 2 x 5 on H

@@ -43,6 +43,7 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 struct MF64 {
   typedef double T;
   typedef d4 acc;
+  static constexpr int kAccRegs = 8;  // 32-bit registers of one accumulator tile
   static __device__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
   static __device__ acc mma(T a, T b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
   static __device__ int row(int lane, int j) { return 4 * j + (lane >> 4); }
@@ -54,6 +55,7 @@ struct MF64 {
 struct MF32 {
   typedef float T;
   typedef f4 acc;
+  static constexpr int kAccRegs = 4;
   static __device__ acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
   static __device__ acc mma(T a, T b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   static __device__ int row(int lane, int j) { return 4 * (lane >> 4) + j; }
@@ -70,6 +72,7 @@ struct MF32C {
     f4 p;
     d4 s;
   };
+  static constexpr int kAccRegs = 12;
   static __device__ acc zero() { return acc{f4{0.f, 0.f, 0.f, 0.f}, d4{0.0, 0.0, 0.0, 0.0}}; }
   static __device__ acc mma(T a, T b, acc c) {
     c.p = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c.p, 0, 0, 0);
@@ -140,155 +143,202 @@ __device__ inline E q1_val(const E *__restrict__ eri, const Q1Src &s, int K, int
   return eri[s.base + idx];
 }
 
-// Off the diagonal (bmu != bnu, br != bs, Ib != Jb) the element (mu = bmu*8+u, nu = bnu*8+w | K) of a
-// quartet sits at base + K sK + u su + w sw, with strides fixed per block; the diagonal blocks take
-// q1_val's general path.
-struct Q1Lin {
-  int sK, su, sw;
-  bool general;
-};
-
-__device__ inline Q1Lin q1_lin(const Q1Src &s) {
-  Q1Lin l;
-  l.general = s.diagK || s.diagB || s.Ib == s.Jb;
-  const bool bra_fast = s.Ib > s.Jb;  // [ket][bra] storage: the bra pair (pl, ql) is the fast index
-  const int sp = bra_fast ? 8 : 512, sq = bra_fast ? 1 : 64;
-  l.sK = bra_fast ? 64 : 1;
-  l.su = s.swp ? sq : sp;
-  l.sw = s.swp ? sp : sq;
-  return l;
+// Element index (within its quartet) of (mu = bmu*8+u, nu = bnu*8+w | K), branch-free.  Unlike q1_val the
+// non-canonical ket slots of a diagonal ket block are not zeroed: Q1 zeroes their outputs when it stores X.
+__device__ inline int q1_idx(const Q1Src &s, int K, int u, int w) {
+  int pl = s.swp ? w : u, ql = s.swp ? u : w;
+  const int hi = max(pl, ql), lo = min(pl, ql);
+  if (s.diagB) pl = hi, ql = lo;
+  const int bl = pl * 8 + ql;
+  const int kbl = s.Ib > s.Jb ? K * 64 + bl : bl * 64 + K;
+  return s.Ib == s.Jb ? min(K, bl) * 64 + max(K, bl) : kbl;
 }
 
 // ---------------------------------------------------------------- Q1, FP64 / FP32
 // Workgroup (256 threads) owns half kh of ket block pair Jb's 64 slots (K = 32 kh .. 32 kh + 31) x AO
 // block bnu (8 nu) x a group of 16 MT occupied orbitals.  Loop over mu blocks: stage
-// T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then X[K][i][nu] += C1[mu,i] T via MFMA.
-// Output tiles: MT tiles of 16 i by 16 tiles of 16 (K,nu) columns, split evenly over the 4 waves.
-// Two such workgroups fit a CU, so one's staging and barriers overlap the other's MFMAs.  The staging
-// loads follow the quartet's storage order: bra-fastest quartets (Ib > Jb) are read along (u,w),
-// ket-fastest ones (Ib < Jb) along K.
+// T[K][u*8+w] = (mu=bmu*8+u, nu=bnu*8+w | K) into LDS, then X[K][i][nu] += C1[mu,i] T via MFMA.  Wave w
+// owns the 16-column tiles 4w .. 4w+3 (K = 32 kh + 8w .. 8w+7, all 8 nu) for all MT row tiles, so each
+// LDS operand feeds MT MFMAs.
+//
+// On gfx950 the FP64 MFMAs do not co-issue with other vector ALU instructions, so the loop is written to
+// issue few of them:
+// - a thread's eight staging elements of a quartet block lie at one per-thread offset plus a uniform
+//   stride, for each storage order (bra-fastest [K][bra] or ket-fastest [bra][K]) and bra orientation;
+//   the offsets are computed once, and only the diagonal forms (Ib == Jb, or a diagonal bra block stored
+//   ket-fastest) compute indices element by element;
+// - LDS stores and operand reads use immediate offsets from fixed per-thread bases;
+// - the non-canonical ket slots of a diagonal ket block are zeroed in the epilogue (X there is exactly
+//   +0, as the zeroed inputs gave).
+// Quartet blocks are fetched two mu blocks ahead and staged one ahead (double-buffered LDS, one barrier
+// per block); the C1 operands of a block are loaded one block ahead, before the quartet fetch.  Loads,
+// fetches and stages are unconditional (indices past the last block are clamped: their copies land in
+// the buffer no later step reads, and the A-operand rows past m1 load row m1 - 1, which only reaches output
+// rows that are never stored), so every path through the loop leaves the same loads in flight and the
+// compiler's memory-counter waits are exact: the MFMAs wait only for their C1 operands, a stage only for
+// its own quartet block.
 // The body takes the workgroup's coordinates (bx, by, bz) as arguments, so that a grouped launch over
 // many molecules can run it for each molecule's share of the grid.
+struct Q1Slots {
+  int kA, uA, wA, kB, wB;  // bra-fastest slot (kA + 4j, uA, wA); ket-fastest slot (kB, j, wB), j = 0..7
+  int offA0, offA1, offAd, offB0, offB1;
+};
+
+// Loads this thread's eight elements of block (bm, bnu | Jb); returns true for ket-fastest slots.  Only the
+// element indices depend on the block's form, the loads are one sequence (separate load paths would make
+// the compiler merge their memory-counter states conservatively).  The slots are taken by value: a select
+// between the addresses of their fields would keep them in scratch memory.
+template <class T, class E>
+__device__ __forceinline__ bool q1_fetch(const E *__restrict__ eri, const Q1Src src, const Q1Slots sl, int K0,
+                                         T (&v)[8]) {
+  const E *q = eri + src.base;
+  const bool bra = src.Ib > src.Jb, ket = src.Ib < src.Jb;
+  int idx[8];
+  if (bra || (ket && !src.diagB)) {
+    const int offA = sl.offA0 + (src.swp ? sl.offA1 - sl.offA0 : 0) + (src.diagB ? sl.offAd - sl.offA0 : 0);
+    const int offK = src.swp ? sl.offB1 + 0 : sl.offB0 + 0;
+    const int off = bra ? offA : offK;
+    const int stride = bra ? 256 : (src.swp ? 64 : 512);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) idx[j] = off + stride * j;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kl = ket ? sl.kB + 0 : sl.kA + 4 * j, u = ket ? j : sl.uA + 0, w = ket ? sl.wB + 0 : sl.wA + 0;
+      idx[j] = q1_idx(src, K0 + kl, u, w);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = T(q[idx[j]]);
+  return ket;
+}
+
+// T[kl][u*8+w] of the staged block: bra-fastest slots at lA + 4 rows per j, ket-fastest at lB + 8 per j.
+template <class T>
+__device__ __forceinline__ void q1_stage(T *lA, T *lB, const T (&v)[8], bool ket) {
+  if (ket) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lB[8 * j] = v[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lA[4 * 65 * j] = v[j];
+  }
+}
+
+// A operands of one mu block: C1[8 bm + 4 ks + (lane >> 4)][i0 + 16 h + (lane & 15)], cb at the block's row
+// and coff[h] the (clamped) column of row tile h.
+template <class T, int MT>
+__device__ __forceinline__ void q1_load_c(const T *__restrict__ cb, size_t ldc1, const int (&coff)[MT], T (&cv)[2][MT]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int h = 0; h < MT; ++h) cv[ks][h] = cb[ks * 4 * ldc1 + coff[h]];
+}
+
+// The block's MFMAs: B operand of column tile c is T[8w + 2c + ((lane >> 3) & 1)][(4 ks + (lane >> 4)) * 8
+// + (lane & 7)], read from lR + 130 c + 32 ks.
+template <class MF, int MT>
+__device__ __forceinline__ void q1_mfmas(const typename MF::T *lR, const typename MF::T (&cv)[2][MT],
+                                         typename MF::acc (&acc)[MT][4]) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const typename MF::T b = lR[2 * c * 65 + 32 * ks];
+#pragma unroll
+      for (int h = 0; h < MT; ++h) acc[h][c] = MF::mma(cv[ks][h], b, acc[h][c]);
+    }
+#pragma unroll
+  for (int h = 0; h < MT; ++h)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) MF::flush(acc[h][c]);
+}
+
 template <class MF, int MT, class E>
 __device__ __forceinline__ void q1_body(const E *__restrict__ eri, const typename MF::T *__restrict__ C1, int ldc1,
                                         int m1, int nb, int npad, typename MF::T *__restrict__ X, int bx, int by,
                                         int bz) {
   typedef typename MF::T T;
-  constexpr int TPW = MT * 4;  // tiles per wave
   __shared__ T Ts[2][32][65];
   const int Jb = int(unsigned(bx) / unsigned(nb)), bnu = int(unsigned(bx) % unsigned(nb));
-  const int i0 = by * (MT * 16);  // MT tiles of 16 occupied orbitals per workgroup
+  const int i0 = by * (MT * 16);
   const int K0 = 32 * bz;
   int br, bs;
   zq_unpair(Jb, br, bs);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  typename MF::acc acc[TPW];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  typename MF::acc acc[MT][4];
 #pragma unroll
-  for (int t = 0; t < TPW; ++t) acc[t] = MF::zero();
+  for (int h = 0; h < MT; ++h)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[h][c] = MF::zero();
+  // staging slots: bra-fastest (K0 + kA + 4j, uA, wA) at (K0+kl)*64 + pl*8 + ql, (pl, ql) = (u, w), (w, u)
+  // when the bra block is swapped, (max, min) on a diagonal bra block; ket-fastest (K0 + kB, j, wB) at
+  // (pl*8 + ql)*64 + K
+  Q1Slots sl;
+  sl.kA = tid >> 6, sl.uA = (tid >> 3) & 7, sl.wA = tid & 7;
+  sl.kB = tid & 31, sl.wB = tid >> 5;
+  sl.offA0 = (K0 + sl.kA) * 64 + sl.uA * 8 + sl.wA;
+  sl.offA1 = (K0 + sl.kA) * 64 + sl.wA * 8 + sl.uA;
+  sl.offAd = (K0 + sl.kA) * 64 + max(sl.uA, sl.wA) * 8 + min(sl.uA, sl.wA);
+  sl.offB0 = sl.wB * 64 + K0 + sl.kB;
+  sl.offB1 = sl.wB * 512 + K0 + sl.kB;
+  T *const lA0 = &Ts[0][sl.kA][tid & 63], *const lA1 = &Ts[1][sl.kA][tid & 63];
+  T *const lB0 = &Ts[0][sl.kB][sl.wB], *const lB1 = &Ts[1][sl.kB][sl.wB];
+  const T *const lR0 = &Ts[0][8 * wave + ((lane >> 3) & 1)][(lane >> 4) * 8 + (lane & 7)];
+  const T *const lR1 = &Ts[1][8 * wave + ((lane >> 3) & 1)][(lane >> 4) * 8 + (lane & 7)];
+  const T *const cbase = C1 + size_t(lane >> 4) * ldc1;
+  const size_t ldc = size_t(ldc1);
+  int coff[MT];
+#pragma unroll
+  for (int h = 0; h < MT; ++h) coff[h] = min(i0 + 16 * h + (lane & 15), m1 - 1);
 
-  // A wave's tiles span at most two 16-row i tiles (it_lo, it_lo + 1): their C1 operands for a mu block
-  // are four values per lane, loaded with the block instead of once per MFMA.
-  const int it_lo = (wave * TPW) >> 4;
-  auto load_a = [&](int bm, T (&a)[2][2]) {
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const int mu = bm * 8 + ks * 4 + (lane >> 4);
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int i = i0 + (it_lo + h) * 16 + (lane & 15);
-        a[ks][h] = (i < m1) ? C1[size_t(mu) * ldc1 + i] : T(0);
-      }
-    }
-  };
-  // element j of this thread's share of a staged block: (K local, u*8+w)
-  auto slot = [&](int j, bool kfast, int &kl, int &uw) {
-    const int e = tid + 256 * j;
-    if (kfast)
-      kl = e & 31, uw = e >> 5;
-    else
-      kl = e >> 6, uw = e & 63;
-  };
-  auto fetch = [&](int bm, T (&v)[8], bool &kfast) {
-    const Q1Src src = q1_src(bm, bnu, br, bs, Jb);
-    const Q1Lin lin = q1_lin(src);
-    kfast = src.Ib < src.Jb;
-    if (!lin.general) {
-      const E *q = eri + src.base;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int kl, uw;
-        slot(j, kfast, kl, uw);
-        v[j] = T(q[(K0 + kl) * lin.sK + (uw >> 3) * lin.su + (uw & 7) * lin.sw]);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int kl, uw;
-        slot(j, kfast, kl, uw);
-        v[j] = T(q1_val(eri, src, K0 + kl, uw >> 3, uw & 7));
-      }
-    }
-  };
-  auto stage = [&](const T (&v)[8], bool kfast, T (*dst)[65]) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      int kl, uw;
-      slot(j, kfast, kl, uw);
-      dst[kl][uw] = v[j];
-    }
-  };
-  T a_cur[2][2], a_nxt[2][2];
-  load_a(0, a_cur);
-  // double-buffered LDS: block bmu + 1 is fetched before block bmu is multiplied out of Ts[cur] and
-  // staged into Ts[cur ^ 1] after it, so each mu block waits on one barrier
-  T rv[8];
-  bool kf;
-  fetch(0, rv, kf);
-  stage(rv, kf, Ts[0]);
+  T ca[2][MT], cb[2][MT], ra[8], rb[8];
+  const int last = nb - 1;
+  q1_load_c<T, MT>(cbase, ldc, coff, ca);
+  q1_stage(lA0, lB0, ra, q1_fetch<T, E>(eri, q1_src(0, bnu, br, bs, Jb), sl, K0, ra));
+  bool kb = q1_fetch<T, E>(eri, q1_src(min(1, last), bnu, br, bs, Jb), sl, K0, rb);
   __syncthreads();
-  int cur = 0;
-  for (int bmu = 0; bmu < nb; ++bmu) {
-    const bool more = bmu + 1 < nb;
-    if (more) {
-      fetch(bmu + 1, rv, kf);
-      load_a(bmu + 1, a_nxt);
-    }
+  for (int bm = 0; bm < nb; bm += 2) {
+    // block bm from buffer 0 with ca; fetch bm + 2 into ra; stage bm + 1 (rb) into buffer 1
+    q1_load_c<T, MT>(cbase + size_t(min(bm + 1, last)) * 8 * ldc, ldc, coff, cb);
+    const bool ka = q1_fetch<T, E>(eri, q1_src(min(bm + 2, last), bnu, br, bs, Jb), sl, K0, ra);
+    q1_mfmas<MF, MT>(lR0, ca, acc);
+    q1_stage(lA1, lB1, rb, kb);
+    __syncthreads();
+    // block bm + 1 from buffer 1 with cb; fetch bm + 3 into rb; stage bm + 2 (ra) into buffer 0
+    q1_load_c<T, MT>(cbase + size_t(min(bm + 2, last)) * 8 * ldc, ldc, coff, ca);
+    kb = q1_fetch<T, E>(eri, q1_src(min(bm + 3, last), bnu, br, bs, Jb), sl, K0, rb);
+    if (bm + 1 < nb) q1_mfmas<MF, MT>(lR1, cb, acc);
+    q1_stage(lA0, lB0, ra, ka);
+    __syncthreads();
+  }
+  const bool diagK = br == bs;
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const int uu = ks * 4 + (lane >> 4);
+  for (int h = 0; h < MT; ++h)
 #pragma unroll
-      for (int t = 0; t < TPW; ++t) {
-        const int tile = wave * TPW + t, it = tile >> 4, nt = tile & 15;
-        const T a = (it == it_lo) ? a_cur[ks][0] : a_cur[ks][1];
-        const int col = nt * 16 + (lane & 15), kl = col >> 3, wl = col & 7;
-        acc[t] = MF::mma(a, Ts[cur][kl][uu * 8 + wl], acc[t]);
+    for (int c = 0; c < 4; ++c) {
+      const int col = (4 * wave + c) * 16 + (lane & 15), K = K0 + (col >> 3), wl = col & 7;
+      const bool zero = diagK && (K >> 3) < (K & 7);  // non-canonical slot of a diagonal ket block
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = i0 + h * 16 + MF::row(lane, j);
+        if (i < m1)
+          X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = zero ? T(0) : T(MF::val(acc[h][c], j));
       }
     }
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) a_cur[ks][h] = a_nxt[ks][h];
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) MF::flush(acc[t]);
-    if (more) stage(rv, kf, Ts[cur ^ 1]);
-    __syncthreads();
-    cur ^= 1;
-  }
-#pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    const int tile = wave * TPW + t, it = tile >> 4, nt = tile & 15;
-    const int col = nt * 16 + (lane & 15), K = K0 + (col >> 3), wl = col & 7;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = i0 + it * 16 + MF::row(lane, j);
-      if (i < m1) X[(size_t(Jb) * 64 + K) * size_t(m1) * npad + size_t(i) * npad + bnu * 8 + wl] = T(MF::val(acc[t], j));
-    }
-  }
+}
+
+// Two workgroups per CU (the accumulators then sit in the architectural VGPRs), unless the accumulators
+// alone take more than 144 of a wave's registers (FP32 products with FP64 sums at MT = 4), which would spill.
+template <class MF, int MT>
+constexpr int q1_min_waves() {
+  return MT * 4 * MF::kAccRegs <= 144 ? 2 : 1;
 }
 
 template <class MF, int MT, class E>
-__global__ __launch_bounds__(256) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
-                                             int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
+__global__ __launch_bounds__(256, (q1_min_waves<MF, MT>())) void zq_q1(const E *__restrict__ eri, const typename MF::T *__restrict__ C1,
+                                                int ldc1, int m1, int nb, int npad, typename MF::T *__restrict__ X) {
   q1_body<MF, MT, E>(eri, C1, ldc1, m1, nb, npad, X, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -1207,7 +1257,7 @@ __global__ __launch_bounds__(256) void zq_group_slices(const GJob<T, E> *__restr
 }
 
 template <class MF, class E>
-__global__ __launch_bounds__(256) void zq_group_q1(const GJob<typename MF::T, E> *__restrict__ jobs,
+__global__ __launch_bounds__(256, (q1_min_waves<MF, 1>())) void zq_group_q1(const GJob<typename MF::T, E> *__restrict__ jobs,
                                                    const int *__restrict__ start, int nmol) {
   const int m = group_find(start, nmol, blockIdx.x);
   const GJob<typename MF::T, E> J = jobs[m];

@@ -476,127 +476,213 @@ __device__ inline T gemm_b(const GemmArgs<T> &g, const T *B, int k, int n) {
   return B[k * g.sBk + n * g.sBn];
 }
 
-// FP64 / FP32 / FP32C: (32 TM) x (32 TN) tiles, BK = 16, 4 waves of (16 TM) x (16 TN) (TM x TN MFMA
-// tiles each, so each LDS operand read feeds TN or TM MFMAs).  The next K tile is loaded into registers
-// while the current one is multiplied out of a double-buffered LDS tile, so each K step waits on one
-// barrier and no global-load round trip.
-template <class MF, int TM, int TN>
-__device__ inline void gemm_load(const GemmArgs<typename MF::T> &g, const typename MF::T *A, const typename MF::T *B,
-                                 int m0, int n0, int k0, bool a_kfast, bool b_nfast, typename MF::T (&ra)[2 * TM],
-                                 typename MF::T (&rb)[2 * TN]) {
+// FP64 / FP32 / FP32C: 64 x 64 tiles, BK = 16, four waves of 32 x 32 (2 x 2 MFMA tiles, so each LDS
+// operand read feeds two MFMAs).  Written, like Q1, to issue few vector ALU instructions (the FP64 MFMAs
+// do not co-issue with them):
+// - each thread loads 4 elements of A and 4 of B per K step at per-thread offsets fixed for the whole
+//   loop, from uniform K-step bases; rows past M and columns past N are clamped to the last one (they
+//   only reach outputs that are not stored), so no load is masked;
+// - the gathered B of Q2 (mode 1, element (lambda, (sigma, nu)) at X[slot(lambda, sigma)][i][nu]) picks,
+//   per element, one of three per-thread offsets by whether lambda's AO block lies above, below or at
+//   sigma's, plus a uniform part per lambda block;
+// - K steps are loaded two ahead and staged one ahead, unconditionally (step indices past the last are
+//   clamped and their copies land in the LDS buffer no later step reads), so every path leaves the same
+//   loads in flight and the compiler's memory-counter waits are exact;
+// - a K that is not a multiple of 16 ends with one masked step (k >= K loads k = K - 1 and stages 0).
+struct GemmSlots {
+  unsigned offA[4], offB[4];           // mode 0: element offsets from the K-step bases (within one matrix)
+  unsigned long long p1[4], p2[4], pe[4];  // mode 1: slot offsets, lambda's block above / below / at sigma's
+  int kA[4], kB[4];                    // k of element j within its step
+  int mA[4], nB[4];                    // row of A element j, column of B element j (clamped)
+};
+
+template <class MF>
+__device__ __forceinline__ void gemm_load(const GemmArgs<typename MF::T> &g, const typename MF::T *A,
+                                          const typename MF::T *B, const GemmSlots sl, int k0, int sb,
+                                          typename MF::T (&ra)[4], typename MF::T (&rb)[4]) {
+  const typename MF::T *Ak = A + (long long)k0 * g.sAk;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) ra[j] = Ak[sl.offA[j]];
+  if (g.mode == 1) {
+    const unsigned long long sX = (unsigned long long)g.sXs;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int lb = (k0 >> 3) + h;  // lambda block of elements 2h, 2h + 1 (uniform)
+      const unsigned long long u1 = (unsigned long long)(lb * (lb + 1) / 2) * 64 * sX;  // block pair (lb, sb)
+      const unsigned long long u2 = (unsigned long long)lb * 64 * sX;                    // block pair (sb, lb)
+      const int rel = lb - sb;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int j = 2 * h + q;
+        const unsigned long long off = rel > 0 ? u1 + sl.p1[j] : (rel < 0 ? u2 + sl.p2[j] : sl.pe[j]);
+        rb[j] = B[off];
+      }
+    }
+  } else {
+    const typename MF::T *Bk = B + (long long)k0 * g.sBk;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) rb[j] = Bk[sl.offB[j]];
+  }
+}
+
+// The last, partial K step: elements with k >= K are zero (their loads read k = K - 1).
+template <class MF>
+__device__ __forceinline__ void gemm_load_tail(const GemmArgs<typename MF::T> &g, const typename MF::T *A,
+                                               const typename MF::T *B, const GemmSlots sl, int k0,
+                                               typename MF::T (&ra)[4], typename MF::T (&rb)[4]) {
   typedef typename MF::T T;
-  constexpr int WM = 32 * TM, WN = 32 * TN;
-  const int tid = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < 2 * TM; ++j) {
-    const int e = tid + 256 * j;
-    int mm, kk;
-    if (a_kfast)
-      mm = e >> 4, kk = e & 15;
-    else
-      kk = e / WM, mm = e % WM;
-    const int m = m0 + mm, k = k0 + kk;
-    ra[j] = (m < g.M && k < g.K) ? A[m * g.sAm + k * g.sAk] : T(0);
+  for (int j = 0; j < 4; ++j) {
+    const int k = k0 + sl.kA[j], kc = min(k, g.K - 1);
+    const T x = A[(long long)sl.mA[j] * g.sAm + (long long)kc * g.sAk];
+    ra[j] = k < g.K ? x : T(0);
   }
 #pragma unroll
-  for (int j = 0; j < 2 * TN; ++j) {
-    const int e = tid + 256 * j;
-    int nn, kk;
-    if (b_nfast)
-      kk = e / WN, nn = e % WN;
-    else
-      nn = e >> 4, kk = e & 15;
-    const int n = n0 + nn, k = k0 + kk;
-    rb[j] = (n < g.N && k < g.K) ? gemm_b(g, B, k, n) : T(0);
+  for (int j = 0; j < 4; ++j) {
+    const int k = k0 + sl.kB[j], kc = min(k, g.K - 1);
+    const T x = gemm_b(g, B, kc, sl.nB[j]);
+    rb[j] = k < g.K ? x : T(0);
   }
 }
 
-template <class T, int TM, int TN>
-__device__ inline void gemm_stage(T (*As)[32 * TM + 1], T (*Bs)[32 * TN + 1], const T (&ra)[2 * TM],
-                                  const T (&rb)[2 * TN], bool a_kfast, bool b_nfast) {
-  constexpr int WM = 32 * TM, WN = 32 * TN;
-  const int tid = threadIdx.x;
+// a_kfast: A element j at As[k][m + 16 j], else As[k + 4 j][m]; b_nfast: B element j at Bs[k + 4 j][n],
+// else Bs[k][n + 16 j] (lA, lB: the thread's element 0 in the buffer)
+template <class T>
+__device__ __forceinline__ void gemm_stage(T *lA, T *lB, bool a_kfast, bool b_nfast, const T (&ra)[4],
+                                           const T (&rb)[4]) {
+  if (a_kfast) {
 #pragma unroll
-  for (int j = 0; j < 2 * TM; ++j) {
-    const int e = tid + 256 * j;
-    if (a_kfast)
-      As[e & 15][e >> 4] = ra[j];
-    else
-      As[e / WM][e % WM] = ra[j];
+    for (int j = 0; j < 4; ++j) lA[16 * j] = ra[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) lA[4 * 65 * j] = ra[j];
   }
+  if (b_nfast) {
 #pragma unroll
-  for (int j = 0; j < 2 * TN; ++j) {
-    const int e = tid + 256 * j;
-    if (b_nfast)
-      Bs[e / WN][e % WN] = rb[j];
-    else
-      Bs[e & 15][e >> 4] = rb[j];
+    for (int j = 0; j < 4; ++j) lB[4 * 65 * j] = rb[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) lB[16 * j] = rb[j];
   }
 }
 
-// Workgroup (bx, by, bz): the (32 TM) x (32 TN) output tile (by, bx) of batch entry bz.
-template <class MF, int TM, int TN>
+// One K step: operands As[4 ks + (lane >> 4)][wm + 16 x + (lane & 15)] and the same of Bs (lRA, lRB at
+// ks = x = 0).  16 x 16 sub-tiles wholly outside M or N take no MFMA (uniform over the wave).
+template <class MF>
+__device__ __forceinline__ void gemm_mfmas(const typename MF::T *lRA, const typename MF::T *lRB,
+                                           const bool (&live)[2][2], typename MF::acc (&acc)[2][2]) {
+  typedef typename MF::T T;
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    T a[2], b[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) a[x] = lRA[ks * 4 * 65 + 16 * x];
+#pragma unroll
+    for (int y = 0; y < 2; ++y) b[y] = lRB[ks * 4 * 65 + 16 * y];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+        if (live[x][y]) acc[x][y] = MF::mma(a[x], b[y], acc[x][y]);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) MF::flush(acc[x][y]);
+}
+
+// Workgroup (bx, by, bz): the 64 x 64 output tile (by, bx) of batch entry bz.
+template <class MF>
 __device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int bx, int by, int bz) {
   typedef typename MF::T T;
-  constexpr int WM = 32 * TM, WN = 32 * TN;
-  __shared__ T As[2][16][WM + 1];
-  __shared__ T Bs[2][16][WN + 1];
-  const int m0 = by * WM, n0 = bx * WN;
+  __shared__ T As[2][16][65];
+  __shared__ T Bs[2][16][65];
+  const int m0 = by * 64, n0 = bx * 64;
   const T *A, *B;
   T *C;
   int M;
   gemm_batch(g, g.b0 + bz, A, B, C, M);
   if (m0 >= M) return;  // whole workgroup: rows of a tri 1 entry beyond its pair
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
-  typename MF::acc acc[TM][TN];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  typename MF::acc acc[2][2];
 #pragma unroll
-  for (int x = 0; x < TM; ++x)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < TN; ++y) acc[x][y] = MF::zero();
+    for (int y = 0; y < 2; ++y) acc[x][y] = MF::zero();
   const bool a_kfast = (g.sAk == 1);
   const bool b_nfast = (g.mode == 1) || (g.sBn == 1);
-  // 16 x 16 sub-tiles wholly outside M or N (o = 40 rows in a 64-row tile, v = 152 columns in three
-  // 64-column tiles) take no MFMA: the test is uniform over the wave, and the SIMD time goes to others
-  bool live[TM][TN];
+  bool live[2][2];
 #pragma unroll
-  for (int x = 0; x < TM; ++x)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < TN; ++y) live[x][y] = m0 + wm + 16 * x < M && n0 + wn + 16 * y < g.N;
-  T ra[2 * TM], rb[2 * TN];
-  gemm_load<MF, TM, TN>(g, A, B, m0, n0, 0, a_kfast, b_nfast, ra, rb);
-  gemm_stage<T, TM, TN>(As[0], Bs[0], ra, rb, a_kfast, b_nfast);
-  __syncthreads();
-  int cur = 0;
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    const bool more = k0 + 16 < g.K;
-    if (more) gemm_load<MF, TM, TN>(g, A, B, m0, n0, k0 + 16, a_kfast, b_nfast, ra, rb);
+    for (int y = 0; y < 2; ++y) live[x][y] = m0 + wm + 16 * x < M && n0 + wn + 16 * y < g.N;
+
+  // the thread's element j: A (m, k) = (m0 + (tid >> 4) + 16 j, tid & 15) if a_kfast, else (m0 + (tid & 63),
+  // (tid >> 6) + 4 j); B (k, n) = ((tid >> 6) + 4 j, n0 + (tid & 63)) if b_nfast, else (tid & 15,
+  // n0 + (tid >> 4) + 16 j)
+  GemmSlots sl;
+  int sb = 0;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const int kk = ks * 4 + (lane >> 4);
-      T a[TM], bb[TN];
+  for (int j = 0; j < 4; ++j) {
+    const int mm = a_kfast ? (tid >> 4) + 16 * j : tid & 63, kk = a_kfast ? tid & 15 : (tid >> 6) + 4 * j;
+    const int m = min(m0 + mm, M - 1);
+    sl.mA[j] = m, sl.kA[j] = kk;
+    sl.offA[j] = unsigned((long long)m * g.sAm + (long long)kk * g.sAk);
+    const int nn = b_nfast ? tid & 63 : (tid >> 4) + 16 * j, kb = b_nfast ? (tid >> 6) + 4 * j : tid & 15;
+    const int n = min(n0 + nn, g.N - 1);
+    sl.nB[j] = n, sl.kB[j] = kb;
+    sl.offB[j] = unsigned((long long)kb * g.sBk + (long long)n * g.sBn);
+  }
+  if (g.mode == 1) {
+    // slot(lambda, sigma) = bpair(max block, min block) * 64 + (max & 7) * 8 + (min & 7), as slot_of
+    const int n = sl.nB[0], sig = n / g.npad, nu = n - sig * g.npad, sr = sig & 7;
+    sb = sig >> 3;
+    const unsigned long long sX = (unsigned long long)g.sXs;
 #pragma unroll
-      for (int x = 0; x < TM; ++x) a[x] = As[cur][kk][wm + x * 16 + (lane & 15)];
-#pragma unroll
-      for (int y = 0; y < TN; ++y) bb[y] = Bs[cur][kk][wn + y * 16 + (lane & 15)];
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y)
-          if (live[x][y]) acc[x][y] = MF::mma(a[x], bb[y], acc[x][y]);
+    for (int j = 0; j < 4; ++j) {
+      const int lr = ((tid >> 6) + 4 * j) & 7;
+      sl.p1[j] = (unsigned long long)(sb * 64 + lr * 8 + sr) * sX + nu;
+      sl.p2[j] = (unsigned long long)((sb * (sb + 1) / 2) * 64 + sr * 8 + lr) * sX + nu;
+      sl.pe[j] = (unsigned long long)((sb * (sb + 1) / 2 + sb) * 64 + max(lr, sr) * 8 + min(lr, sr)) * sX + nu;
     }
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TN; ++y) MF::flush(acc[x][y]);
-    if (more) gemm_stage<T, TM, TN>(As[cur ^ 1], Bs[cur ^ 1], ra, rb, a_kfast, b_nfast);
+  }
+  T *const lA0 = a_kfast ? &As[0][tid & 15][tid >> 4] : &As[0][tid >> 6][tid & 63], *const lA1 = lA0 + 16 * 65;
+  T *const lB0 = b_nfast ? &Bs[0][tid >> 6][tid & 63] : &Bs[0][tid & 15][tid >> 4], *const lB1 = lB0 + 16 * 65;
+  const T *const lRA0 = &As[0][lane >> 4][wm + (lane & 15)], *const lRA1 = lRA0 + 16 * 65;
+  const T *const lRB0 = &Bs[0][lane >> 4][wn + (lane & 15)], *const lRB1 = lRB0 + 16 * 65;
+
+  const int nfull = g.K >> 4, last = nfull - 1;
+  T ra[4], qa[4], rb[4], qb[4];  // (A, B) of the step in flight by parity: (ra, qa) even, (rb, qb) odd
+  if (nfull > 0) {
+    gemm_load<MF>(g, A, B, sl, 0, sb, ra, qa);
+    gemm_stage(lA0, lB0, a_kfast, b_nfast, ra, qa);
+    gemm_load<MF>(g, A, B, sl, 16 * min(1, last), sb, rb, qb);
     __syncthreads();
-    cur ^= 1;
+    for (int s = 0; s < nfull; s += 2) {
+      // step s from buffer 0; load s + 2 into (ra, qa); stage s + 1 (rb, qb) into buffer 1
+      gemm_load<MF>(g, A, B, sl, 16 * min(s + 2, last), sb, ra, qa);
+      gemm_mfmas<MF>(lRA0, lRB0, live, acc);
+      gemm_stage(lA1, lB1, a_kfast, b_nfast, rb, qb);
+      __syncthreads();
+      // step s + 1 from buffer 1; load s + 3 into (rb, qb); stage s + 2 (ra, qa) into buffer 0
+      gemm_load<MF>(g, A, B, sl, 16 * min(s + 3, last), sb, rb, qb);
+      if (s + 1 < nfull) gemm_mfmas<MF>(lRA1, lRB1, live, acc);
+      gemm_stage(lA0, lB0, a_kfast, b_nfast, ra, qa);
+      __syncthreads();
+    }
+  }
+  if (g.K & 15) {  // the partial last step, through buffer 0 (no step reads it any more)
+    gemm_load_tail<MF>(g, A, B, sl, 16 * nfull, ra, qa);
+    gemm_stage(lA0, lB0, a_kfast, b_nfast, ra, qa);
+    __syncthreads();
+    gemm_mfmas<MF>(lRA0, lRB0, live, acc);
   }
 #pragma unroll
-  for (int x = 0; x < TM; ++x)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < TN; ++y) {
+    for (int y = 0; y < 2; ++y) {
       const int n = n0 + wn + y * 16 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -606,9 +692,9 @@ __device__ __forceinline__ void gemm_body(const GemmArgs<typename MF::T> &g, int
     }
 }
 
-template <class MF, int TM = 2, int TN = 2>
-__global__ __launch_bounds__(256) void zq_gemm(GemmArgs<typename MF::T> g) {
-  gemm_body<MF, TM, TN>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+template <class MF>
+__global__ __launch_bounds__(256, 2) void zq_gemm(GemmArgs<typename MF::T> g) {
+  gemm_body<MF>(g, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // BF16X3: FP32 operands split into bf16 hi/lo while staged; 64 x 64 tiles, BK = 32 (one k-step).
@@ -795,13 +881,12 @@ GemmArgs<T> batch_slice(GemmArgs<T> g, long long b0) {
   return g;
 }
 
-template <class MF, int TM = 2, int TN = 2>
+template <class MF>
 int launch_gemm(zq_ctx *ctx, const GemmArgs<typename MF::T> &g, int batch, const char *name) {
   zq_prof_begin(ctx, name);
   for (int b0 = 0; b0 < batch; b0 += kMaxGridZ) {
-    const dim3 grid(unsigned((g.N + 32 * TN - 1) / (32 * TN)), unsigned((g.M + 32 * TM - 1) / (32 * TM)),
-                    unsigned(std::min(kMaxGridZ, batch - b0)));
-    hipLaunchKernelGGL((zq_gemm<MF, TM, TN>), grid, dim3(256), 0, ctx->stream, batch_slice(g, b0));
+    const dim3 grid(unsigned((g.N + 63) / 64), unsigned((g.M + 63) / 64), unsigned(std::min(kMaxGridZ, batch - b0)));
+    hipLaunchKernelGGL((zq_gemm<MF>), grid, dim3(256), 0, ctx->stream, batch_slice(g, b0));
   }
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, name);
@@ -1266,12 +1351,12 @@ __global__ __launch_bounds__(256, (q1_min_waves<MF, 1>())) void zq_group_q1(cons
 }
 
 template <class MF, class E>
-__global__ __launch_bounds__(256) void zq_group_gemm(const GJob<typename MF::T, E> *__restrict__ jobs,
+__global__ __launch_bounds__(256, 2) void zq_group_gemm(const GJob<typename MF::T, E> *__restrict__ jobs,
                                                      const int *__restrict__ start, int nmol, int q) {
   const int m = group_find(start, nmol, blockIdx.x);
   const GemmArgs<typename MF::T> g = jobs[m].g[q];
   const int l = int(blockIdx.x) - start[m], gx = (g.N + 63) / 64, gy = (g.M + 63) / 64, t = l / gx;
-  gemm_body<MF, 2, 2>(g, l - t * gx, t % gy, t / gy);
+  gemm_body<MF>(g, l - t * gx, t % gy, t / gy);
 }
 
 template <class T, class E>

@@ -440,6 +440,9 @@ __global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_step(const double *__r
 // result count as unsettled; a wait that sees *abort raised gives up too.
 constexpr unsigned kSpinLimit = 1u << 22;
 
+// Tiles per workgroup and step of the one-launch SP2 (the grid is ceil(tiles / kSp2TilesPerGroup)).
+constexpr int kSp2TilesPerGroup = 1;
+
 // With several waves a workgroup (NW > 1) takes its tiles through wave 0: lane 0 of wave 0 draws the
 // ticket and hands it over in LDS, wave 0 alone publishes (it alone writes the tile) and waits, and a
 // barrier hands its acquire on to the other waves.  The ticket slot alternates: slot s is written again
@@ -460,14 +463,16 @@ __device__ inline unsigned claim(unsigned *ticket, int &slot) {
   return t;
 }
 
-// Counts a finished tile.  Every wave has read X_in before the tile's reduction barrier, so only the
-// writer, wave 0, orders its stores before the count.
+// Counts a workgroup's finished tiles of a step (n of them).  Every wave has read X_in before each tile's
+// reduction barrier, so only the writer, wave 0, orders its stores before the count.  The release fence
+// writes back the XCD's L2 and is the costliest part of a step's hand-over, so it is issued once per
+// workgroup and step, not once per tile.
 template <int NW>
-__device__ inline void publish(unsigned *done) {
+__device__ inline void publish(unsigned *done, unsigned n) {
   if (NW == 1) __syncthreads();
   if (threadIdx.x < 64) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(done, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -540,15 +545,17 @@ __global__ __launch_bounds__(sp2_threads(PRE)) void k_sp2_run(double *__restrict
     steps = k + 1;
     if (count == 2) fin = (k + 1) & 1;
     double2 *pout = part + size_t(k + 1) * ntiles;
+    unsigned mine = 0;
     while (t < unsigned(ntiles)) {
       sp2_finish<PRE>(o, Xin, Xout, m, ti, tj, mp, pout, int(t));
-      publish<NW>(&done[k]);
+      ++mine;
       t = claim<NW>(&ticket[k], slot);
       if (t < unsigned(ntiles)) {
         tile_of(int(t), T, ti, tj);
         sp2_load<PRE>(o, Xin, m, ti * 16, tj * 16);
       }
     }
+    if (mine) publish<NW>(&done[k], mine);
     if (count >= 2 || k + 1 == nsteps) break;  // the launch's end publishes the last step
     t = claim<NW>(&ticket[k + 1], slot);
     if (!wait_done<NW>(&done[k], unsigned(ntiles), abort)) break;
@@ -645,12 +652,14 @@ int zq_sp2_enqueue(zq_ctx *ctx, const double *dF, int n, int nocc, double tol, i
   zq_prof_begin(ctx, "scf_sp2");
   sp2_start(ctx, b, dF, n, nocc, state6, skip, track);
   double *est = track ? b.est : nullptr;
-  // one workgroup per tile: each normally takes one tile a step
+  // about kSp2TilesPerGroup tiles per workgroup and step: fewer workgroups hand each step over with fewer
+  // release fences, at the cost of more tiles per workgroup on the step's critical path
+  const int groups = std::max(1, (b.ntiles + kSp2TilesPerGroup - 1) / kSp2TilesPerGroup);
   if (b.m <= 256)
-    hipLaunchKernelGGL(k_sp2_run<true>, dim3(b.ntiles), dim3(sp2_threads(true)), 0, ctx->stream, b.X, b.mm, b.m, nocc, b.part, b.sync,
+    hipLaunchKernelGGL(k_sp2_run<true>, dim3(groups), dim3(sp2_threads(true)), 0, ctx->stream, b.X, b.mm, b.m, nocc, b.part, b.sync,
                        state6, nsteps, tol, skip, est);
   else
-    hipLaunchKernelGGL(k_sp2_run<false>, dim3(std::min(b.ntiles, 1024)), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc,
+    hipLaunchKernelGGL(k_sp2_run<false>, dim3(std::min(groups, 1024)), dim3(64), 0, ctx->stream, b.X, b.mm, b.m, nocc,
                        b.part, b.sync, state6, nsteps, tol, skip, est);
   zq_prof_end(ctx);
   ZQ_LAUNCH_CHECK(ctx, "k_sp2_run");

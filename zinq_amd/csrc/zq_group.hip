@@ -458,8 +458,21 @@ struct zq_batch {
   float *eri32 = nullptr;
   void *mp2_ws = nullptr;
   size_t mp2_ws_bytes = 0;
-  // results of the last run, and its stage times (host wall clock, ms): SCF, eigenpairs, MP2, total
-  std::vector<std::vector<double>> C, e;
+  // the eigenpair stage's device buffers (grow-only, kept across runs) and pinned host copies of C, e and
+  // the solver's info codes, which a second stream brings back while the MP2 runs
+  double *d_eig = nullptr, *d_c = nullptr, *d_w = nullptr, *d_e = nullptr;
+  long long *d_eoff = nullptr;
+  int *d_info = nullptr;
+  size_t eig_cap = 0, w_cap = 0, m_cap = 0;  // doubles of d_eig / d_c, of d_w / d_e; molecules
+  double *h_c = nullptr, *h_w = nullptr;
+  int *h_info = nullptr;
+  size_t hc_cap = 0, hw_cap = 0, hi_cap = 0;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t coef_done = nullptr;
+  // results of the last run (C and e of molecule m at c_off[m], w_off[m] of h_c, h_w), and its stage
+  // times (host wall clock, ms): SCF, eigenpairs, MP2, total
+  std::vector<long long> c_off, w_off;
+  std::vector<int> res_n;
   double stage_ms[4] = {0, 0, 0, 0};
 };
 
@@ -484,8 +497,63 @@ void free_plan(zq_batch *b) {
   b->graph = nullptr;
 }
 
+void free_eig(zq_batch *b) {
+  if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);  // copies into the pinned buffers
+  for (void *p : {(void *)b->d_eig, (void *)b->d_c, (void *)b->d_w, (void *)b->d_e, (void *)b->d_eoff, (void *)b->d_info})
+    if (p) (void)hipFree(p);
+  b->d_eig = b->d_c = b->d_w = b->d_e = nullptr;
+  b->d_eoff = nullptr, b->d_info = nullptr;
+  b->eig_cap = b->w_cap = b->m_cap = 0;
+  for (void *p : {(void *)b->h_c, (void *)b->h_w, (void *)b->h_info})
+    if (p) (void)hipHostFree(p);
+  b->h_c = b->h_w = nullptr, b->h_info = nullptr;
+  b->hc_cap = b->hw_cap = b->hi_cap = 0;
+}
+
+// Grow-only device and pinned host buffers of the eigenpair stage.
+int eig_buffers(zq_batch *b, size_t etot, size_t wtot, size_t M) {
+  if (etot > b->eig_cap) {
+    for (double **p : {&b->d_eig, &b->d_c}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    if (b->h_c) (void)hipHostFree(b->h_c);
+    b->h_c = nullptr, b->eig_cap = b->hc_cap = 0;
+    BTRY(b, hipMalloc(&b->d_eig, etot * sizeof(double)));
+    BTRY(b, hipMalloc(&b->d_c, etot * sizeof(double)));
+    BTRY(b, hipHostMalloc((void **)&b->h_c, etot * sizeof(double), hipHostMallocDefault));
+    b->eig_cap = b->hc_cap = etot;
+  }
+  if (wtot > b->w_cap) {
+    for (double **p : {&b->d_w, &b->d_e}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+    if (b->h_w) (void)hipHostFree(b->h_w);
+    b->h_w = nullptr, b->w_cap = b->hw_cap = 0;
+    BTRY(b, hipMalloc(&b->d_w, wtot * sizeof(double)));
+    BTRY(b, hipMalloc(&b->d_e, wtot * sizeof(double)));
+    BTRY(b, hipHostMalloc((void **)&b->h_w, wtot * sizeof(double), hipHostMallocDefault));
+    b->w_cap = b->hw_cap = wtot;
+  }
+  if (M > b->m_cap) {
+    if (b->d_eoff) (void)hipFree(b->d_eoff);
+    if (b->d_info) (void)hipFree(b->d_info);
+    if (b->h_info) (void)hipHostFree(b->h_info);
+    b->d_eoff = nullptr, b->d_info = nullptr, b->h_info = nullptr, b->m_cap = b->hi_cap = 0;
+    BTRY(b, hipMalloc(&b->d_eoff, M * sizeof(long long)));
+    BTRY(b, hipMalloc(&b->d_info, M * sizeof(int)));
+    BTRY(b, hipHostMalloc((void **)&b->h_info, M * sizeof(int), hipHostMallocDefault));
+    b->m_cap = b->hi_cap = M;
+  }
+  if (!b->copy_stream) BTRY(b, hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking));
+  if (!b->coef_done) BTRY(b, hipEventCreateWithFlags(&b->coef_done, hipEventDisableTiming));
+  return ZQ_OK;
+}
+
 void free_all(zq_batch *b) {
   free_plan(b);
+  free_eig(b);
   for (void *p : {(void *)b->eri, (void *)b->ws, (void *)b->d_mols, (void *)b->d_state, (void *)b->d_done,
                   (void *)b->d_count, (void *)b->d_jkm, (void *)b->eri32, b->mp2_ws})
     if (p) (void)hipFree(p);
@@ -682,9 +750,12 @@ void zq_batch_destroy(zq_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
+  if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
   free_all(b);
   if (b->h_count) (void)hipHostFree(b->h_count);
   if (b->solver) rocblas_destroy_handle(b->solver);
+  if (b->coef_done) (void)hipEventDestroy(b->coef_done);
+  if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -830,21 +901,12 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
     wtot += b->mols[m].n;
   }
   for (int m = 0; m < M; ++m) coff[m] = eoff[m];
-  double *d_eig = nullptr, *d_w = nullptr, *d_e = nullptr, *d_c = nullptr;
-  long long *d_eoff = nullptr;
-  int *d_info = nullptr;
-  BTRY(b, hipMalloc(&d_eig, etot * sizeof(double)));
-  BTRY(b, hipMalloc(&d_c, etot * sizeof(double)));
-  BTRY(b, hipMalloc(&d_w, wtot * sizeof(double)));
-  BTRY(b, hipMalloc(&d_e, wtot * sizeof(double)));
-  BTRY(b, hipMalloc(&d_eoff, M * sizeof(long long)));
-  BTRY(b, hipMalloc(&d_info, M * sizeof(int)));
-  struct Free {
-    std::vector<void *> p;
-    ~Free() {
-      for (void *x : p) (void)hipFree(x);
-    }
-  } guard{{d_eig, d_c, d_w, d_e, d_eoff, d_info}};
+  // the previous run's copies of C and e must have landed before their buffers are reused
+  if (b->copy_stream) BTRY(b, hipStreamSynchronize(b->copy_stream));
+  if (int rc = eig_buffers(b, size_t(etot), size_t(wtot), size_t(M))) return rc;
+  double *d_eig = b->d_eig, *d_w = b->d_w, *d_e = b->d_e, *d_c = b->d_c;
+  long long *d_eoff = b->d_eoff;
+  int *d_info = b->d_info;
   BTRY(b, hipMemcpyAsync(d_eoff, eoff.data(), M * sizeof(long long), hipMemcpyHostToDevice, b->stream));
   if (b->nt == 512)
     hipLaunchKernelGGL(k_batch_fprime<512>, dim3(M), dim3(512), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff);
@@ -871,25 +933,16 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
   else
     hipLaunchKernelGGL(k_batch_coef<256>, dim3(M), dim3(256), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff, d_c, d_eoff);
   BTRY(b, hipGetLastError());
-  std::vector<double> hc(etot), hw(wtot);
-  std::vector<int> info(M);
-  BTRY(b, hipMemcpyAsync(hc.data(), d_c, etot * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  BTRY(b, hipMemcpyAsync(hw.data(), d_w, wtot * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  BTRY(b, hipMemcpyAsync(info.data(), d_info, M * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  // C, e and the info codes go to pinned host memory on the copy stream, overlapped with the MP2
+  BTRY(b, hipEventRecord(b->coef_done, b->stream));
+  BTRY(b, hipStreamWaitEvent(b->copy_stream, b->coef_done, 0));
+  BTRY(b, hipMemcpyAsync(b->h_c, d_c, etot * sizeof(double), hipMemcpyDeviceToHost, b->copy_stream));
+  BTRY(b, hipMemcpyAsync(b->h_w, d_w, wtot * sizeof(double), hipMemcpyDeviceToHost, b->copy_stream));
+  BTRY(b, hipMemcpyAsync(b->h_info, d_info, M * sizeof(int), hipMemcpyDeviceToHost, b->copy_stream));
+  b->c_off = coff, b->w_off = woff;
+  b->res_n.resize(M);
+  for (int m = 0; m < M; ++m) b->res_n[m] = b->mols[m].n;
   BTRY(b, hipStreamSynchronize(b->stream));
-  b->C.assign(M, {});
-  b->e.assign(M, {});
-  for (int m = 0; m < M; ++m) {
-    const int n = b->mols[m].n;
-    b->C[m].assign(hc.begin() + coff[m], hc.begin() + coff[m] + (long long)n * n);
-    b->e[m].assign(hw.begin() + woff[m], hw.begin() + woff[m] + n);
-  }
-  {
-    std::vector<int> by_mol(M);  // rocSOLVER wrote info by batch position; reorder by molecule
-    for (int s = 0; s < M; ++s) by_mol[order[s]] = info[s] ? 1 : 0;
-    info.swap(by_mol);
-  }
-
   b->stage_ms[1] = since(t1);
   const auto t2 = std::chrono::steady_clock::now();
 
@@ -918,7 +971,8 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
       return bfail(b, rc, err);
     for (size_t k = 0; k < which.size(); ++k) E_corr[which[k]] = ec[k];
   } else {
-    // BF16X3: several view contexts at once, each with its own stream
+    // BF16X3: several view contexts at once, each with its own stream, on the host copies of C and e
+    BTRY(b, hipStreamSynchronize(b->copy_stream));
     const int nworkers = std::min(M, 8);
     std::vector<zq_ctx *> views(nworkers, nullptr);
     for (int w = 0; w < nworkers; ++w) {
@@ -938,7 +992,8 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
         for (int m; (m = next.fetch_add(1)) < M;) {
           if (hs[m].status != 1 && hs[m].status != 2) continue;
           bind_view(views[w], b, m);
-          rc_m[m] = zq_mp2_device(views[w], b->C[m].data(), b->e[m].data(), b->mols[m].nocc, mp2_precision, &E_corr[m]);
+          rc_m[m] = zq_mp2_device(views[w], b->h_c + coff[m], b->h_w + woff[m], b->mols[m].nocc, mp2_precision,
+                                  &E_corr[m]);
           views[w]->eri = nullptr;
         }
       });
@@ -949,6 +1004,9 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
     }
   }
   b->stage_ms[2] = since(t2);
+  BTRY(b, hipStreamSynchronize(b->copy_stream));  // C, e and the info codes are on the host
+  std::vector<int> info(M);  // rocSOLVER wrote info by batch position; by molecule here
+  for (int s = 0; s < M; ++s) info[order[s]] = b->h_info[s] ? 1 : 0;
   b->stage_ms[3] = since(t0);
   for (int m = 0; m < M; ++m) {
     E_rhf[m] = hs[m].e_new;
@@ -967,9 +1025,10 @@ int zq_batch_timings(const zq_batch *b, double ms[4]) {
 }
 
 int zq_batch_orbitals(const zq_batch *b, int index, double *e, double *C) {
-  if (!b || index < 0 || size_t(index) >= b->C.size() || !e || !C) return ZQ_ERR_INVALID;
-  std::copy(b->e[index].begin(), b->e[index].end(), e);
-  std::copy(b->C[index].begin(), b->C[index].end(), C);
+  if (!b || index < 0 || size_t(index) >= b->res_n.size() || !e || !C || !b->h_c || !b->h_w) return ZQ_ERR_INVALID;
+  const long long n = b->res_n[index];
+  std::copy(b->h_w + b->w_off[index], b->h_w + b->w_off[index] + n, e);
+  std::copy(b->h_c + b->c_off[index], b->h_c + b->c_off[index] + n * n, C);
   return ZQ_OK;
 }
 

@@ -64,6 +64,18 @@ int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps);
  * with no counterpart in the reference. */
 int zq_scf_set_sp2_guess_budget(zq_ctx *ctx, int steps);
 
+/* A test hook with no counterpart in the reference: zq_scf_run_device stores its DIIS error vectors
+ * FPS - SPF (src/hartree_fock.zig:658-683) multiplied by scale, in [0, 1] (default 1, which changes no
+ * bit).  At 0 every DIIS system from the second iteration on is singular, so each iteration takes the
+ * reset of src/hartree_fock.zig:821-845 (history cut to the current entry, F left unextrapolated) and
+ * the SCF is the one without DIIS.  The convergence test (|dE|, rms(dP)) does not use the errors. */
+int zq_scf_set_diis_error_scale(zq_ctx *ctx, double scale);
+
+/* Diagnostics, with no counterpart in the reference: how many singular DIIS systems the last
+ * zq_scf_run_device on ctx met (each one resets the history to the current entry), from the driver
+ * that finished the run. */
+int zq_scf_last_diis_resets(const zq_ctx *ctx);
+
 /* Diagnostics, with no counterpart in the reference: the iterations the device-decided driver of the
  * last zq_scf_run_device on ctx enqueued (graph replays; the host reads the state one iteration late,
  * so a run that ends after iteration k enqueued k + 1 or k + 2 of them), 0 if it did not run. */

@@ -311,6 +311,47 @@ def test_device_scf_unsettled_guess_stops_the_replays(ctx):
             ctx.set_sp2_guess_budget(bad)
 
 
+def test_device_scf_singular_diis_resets(ctx):
+    """The device DIIS reset on a singular system (src/hartree_fock.zig:821-845), in both drivers.  With
+    the stored error vectors scaled by 0, every system [B -1; -1 0] from the second iteration on has a
+    zero B block and is singular: each iteration cuts the history to its own entry and keeps F
+    unextrapolated, so the SCF is the oracle's SCF without DIIS (48 iterations for cc-pVDZ water,
+    against 16 with DIIS), with one reset per iteration after the first.  Scale 1 gives the DIIS SCF
+    back with no reset."""
+    from zinq_amd import hip
+    s, H, S, g, VN = _load(ctx, "ccpvdz")
+    o = s.nel // 2
+    plain = O.scf(H, S, g, VN, o, diis=None, threshold=1e-10)
+    with_diis = O.scf(H, S, g, VN, o, threshold=1e-10)
+    ctx.set_diis_error_scale(0.0)
+    try:
+        r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+        runs = [("device", r, ctx.last_scf_driver(), ctx.last_diis_resets())]
+        ctx.set_sp2_guess_budget(2)  # an unsettled guess: the host-synchronised driver finishes the run
+        try:
+            r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+            runs.append(("synced", r, ctx.last_scf_driver(), ctx.last_diis_resets()))
+        finally:
+            ctx.set_sp2_guess_budget(128)
+            ctx.set_sp2_budget(100)
+    finally:
+        ctx.set_diis_error_scale(1.0)
+    for want, r, driver, resets in runs:
+        print(f"zeroed DIIS errors, {driver} driver: {r['iterations']} iterations (oracle without DIIS "
+              f"{plain['iterations']}), {resets} resets, dE {r['E'] - plain['E']:+.1e}", flush=True)
+        assert driver == want
+        assert abs(r["iterations"] - plain["iterations"]) <= 1
+        assert resets == r["iterations"] - 1
+        assert r["E"] == pytest.approx(plain["E"], abs=1e-10)
+    r = ctx.scf(H, S, VN, o, threshold=1e-10, device=True)
+    assert ctx.last_scf_driver() == "device" and ctx.last_diis_resets() == 0
+    assert abs(r["iterations"] - with_diis["iterations"]) <= 1
+    assert r["E"] == pytest.approx(with_diis["E"], abs=1e-10)
+    for bad in (-1.0, 2.0, float("nan")):
+        with pytest.raises(hip.InvalidInput):
+            ctx.set_diis_error_scale(bad)
+
+
 @pytest.mark.parametrize("kind", ["ccpvdz", "w2"])
 def test_mixed_path_within_bar(ctx, kind):
     """Mixed path: incremental Fock builds from FP32 ERIs to 1e-6, one full FP64 build, FP32C transform.

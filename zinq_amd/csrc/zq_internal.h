@@ -122,6 +122,8 @@ struct zq_ctx {
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
   int sp2_guess_budget = 128;  // SP2 steps for the core-Hamiltonian guess density of that SCF
+  double diis_err_scale = 1.0;  // factor on the device SCF's stored DIIS error vectors (a test hook)
+  int scf_diis_resets = 0;  // singular DIIS systems the last device SCF reset its history on
   int scf_replays = 0;   // iterations the asynchronous driver enqueued in the last device SCF
   int scf_driver = 0;    // driver that finished the last device SCF: 1 device-decided, 2 host-synchronised
   int scf_sp2[3] = {0, 0, 0};  // its SP2 densities, folded ones, steps (device-decided driver only)

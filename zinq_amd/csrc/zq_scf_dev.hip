@@ -71,11 +71,11 @@ __global__ void k_finish(const double *__restrict__ part, int nslots, double *__
   }
 }
 
-// err = FPS - (FPS)^T
-__global__ void k_antisym(const double *__restrict__ FPS, int n, double *__restrict__ err) {
+// err = scale (FPS - (FPS)^T); scale is 1 outside the singular-system test hook (exact: x * 1 = x)
+__global__ void k_antisym(const double *__restrict__ FPS, int n, double scale, double *__restrict__ err) {
   for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < size_t(n) * n; e += size_t(gridDim.x) * blockDim.x) {
     const int i = int(e / n), j = int(e % n);
-    err[e] = FPS[e] - FPS[size_t(j) * n + i];
+    err[e] = scale * (FPS[e] - FPS[size_t(j) * n + i]);
   }
 }
 
@@ -259,7 +259,8 @@ struct ScfState {
   int it;         // iterations completed
   int sp2_steps;  // SP2 steps of the last density
   int sp2_max;    // most SP2 steps of any density so far
-  int hist_n, cur, combine, pad;
+  int hist_n, cur, combine;
+  int resets;          // singular DIIS systems met (the history reset to the current entry)
   int hist[kMaxHist];  // DIIS slots, oldest first
   int sp2[6];          // SP2 {count, steps} pairs, the settled buffer's parity, the abort flag
   double e_old, e_new, rms, dE;
@@ -290,19 +291,19 @@ __device__ void diis_next_slot(ScfState *st, int diis) {
 __global__ void k_state_init(ScfState *st, double VN, int diis) {
   if (threadIdx.x != 0) return;
   st->done = 0, st->it = 0, st->sp2_steps = 0, st->sp2_max = 0;
-  st->hist_n = 0, st->cur = 0, st->combine = 0;
+  st->hist_n = 0, st->cur = 0, st->combine = 0, st->resets = 0;
   st->e_old = VN, st->e_new = VN, st->rms = 0.0, st->dE = 0.0;
   if (diis > 0) diis_next_slot(st, diis);
 }
 
-// err = FPS - (FPS)^T into Eh[cur], F into Fh[cur].
-__global__ void k_antisym_store(const double *__restrict__ FPS, const double *__restrict__ F, int n,
+// err = scale (FPS - (FPS)^T) into Eh[cur], F into Fh[cur].
+__global__ void k_antisym_store(const double *__restrict__ FPS, const double *__restrict__ F, int n, double scale,
                                 double *__restrict__ Eh, double *__restrict__ Fh, const ScfState *__restrict__ st) {
   if (st->done) return;
   const size_t nn = size_t(n) * n, off = size_t(st->cur) * nn;
   for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < nn; e += size_t(gridDim.x) * blockDim.x) {
     const int i = int(e / n), j = int(e % n);
-    Eh[off + e] = FPS[e] - FPS[size_t(j) * n + i];
+    Eh[off + e] = scale * (FPS[e] - FPS[size_t(j) * n + i]);
     Fh[off + e] = F[e];
   }
 }
@@ -496,6 +497,7 @@ __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ pa
   } else if (lane == 0) {
     st->hist[0] = slot;
     st->hist_n = 1;
+    ++st->resets;
   }
 }
 
@@ -644,7 +646,8 @@ int enqueue_iteration(Dev &d, const Run &r, int budget, ScfState *st, double *pa
     // F P (P symmetric) and (F P) S (S symmetric) as A B^T products
     if (int rc = zq_gemm_nt(ctx, ctx->d_F, n, ctx->d_P, r.np, d.FP, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
     if (int rc = zq_gemm_nt(ctx, d.FP, n, d.S, n, d.FPS, n, n, n, n, 1.0, "scf_gemm", skip)) return rc;
-    hipLaunchKernelGGL(k_antisym_store, dim3(gblocks), dim3(256), 0, ctx->stream, d.FPS, ctx->d_F, n, d.Eh, d.Fh, st);
+    hipLaunchKernelGGL(k_antisym_store, dim3(gblocks), dim3(256), 0, ctx->stream, d.FPS, ctx->d_F, n, ctx->diis_err_scale,
+                       d.Eh, d.Fh, st);
     hipLaunchKernelGGL(k_dots_state, dim3(kRedBlocks, maxh), dim3(256), 0, ctx->stream, d.Eh, nn, d.part + kRedBlocks, st);
   }
   hipLaunchKernelGGL(k_diis_solve, dim3(1), dim3(64), 0, ctx->stream, d.part, d.epart, ctx->Nb, r.opt->diis, r.VN, st);
@@ -696,8 +699,8 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   // one iteration as a hipGraph, captured once per context and key, replayed with one launch
   const bool use_graph = !ctx->profile;
   char key[256];
-  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%c%llu/%p/%p/%d", n, nocc, budget, r.opt->diis,
-           r.opt->iterations, r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->set_id ? 's' : 'g',
+  snprintf(key, sizeof key, "%d/%d/%d/%d/%d/%a/%a/%a/%a/%a/%c%llu/%p/%p/%d", n, nocc, budget, r.opt->diis,
+           r.opt->iterations, r.x_exch, r.factor, r.VN, r.opt->threshold, ctx->diis_err_scale, ctx->set_id ? 's' : 'g',
            ctx->set_id ? ctx->set_id : (unsigned long long)ctx->eri_gen, (void *)st, (void *)ddone, int(r.mixed));
   if (ctx->scf_graphs_gen != ctx->scratch_gen || ctx->scf_graphs.size() >= 64) {
     drop_graphs(ctx);
@@ -751,6 +754,7 @@ int scf_async(Dev &d, const Run &r, int *status, double *energy, int *iterations
   *status = hs.done == 0 ? 2 : hs.done;
   *energy = hs.e_new;
   *iterations = hs.it;
+  ctx->scf_diis_resets = hs.resets;
   return ZQ_OK;
 }
 
@@ -771,6 +775,7 @@ int scf_synced(Dev &d, const Run &r, int *status, double *energy, int *iteration
   double e_old = VN, e_new = VN;
   int it;
   bool converged = false;
+  ctx->scf_diis_resets = 0;
   for (it = 0; it < opt->iterations; ++it) {
     if (int rc = zq_fock_jk_device(ctx, ctx->d_P, ctx->d_H, r.x_exch, ctx->d_F)) return rc;
     hipLaunchKernelGGL(k_energy_part, dim3(kRedBlocks), dim3(256), 0, ctx->stream, ctx->d_P, np, ctx->d_H, ctx->d_F, n,
@@ -788,7 +793,8 @@ int scf_synced(Dev &d, const Run &r, int *status, double *energy, int *iteration
         slot = 0;
         while (std::find(hist.begin(), hist.end(), slot) != hist.end()) ++slot;
       }
-      hipLaunchKernelGGL(k_antisym, dim3(gblocks), dim3(256), 0, ctx->stream, d.FPS, n, d.Eh + size_t(slot) * nn);
+      hipLaunchKernelGGL(k_antisym, dim3(gblocks), dim3(256), 0, ctx->stream, d.FPS, n, ctx->diis_err_scale,
+                         d.Eh + size_t(slot) * nn);
       ZQ_TRY(ctx, hipMemcpyAsync(d.Fh + size_t(slot) * nn, ctx->d_F, nn * 8, hipMemcpyDeviceToDevice, ctx->stream));
       hist.push_back(slot);
       const int m = int(hist.size());
@@ -818,6 +824,7 @@ int scf_synced(Dev &d, const Run &r, int *status, double *energy, int *iteration
           // singular DIIS matrix: keep only the current entry, src/hartree_fock.zig:821-845
           hist.clear();
           hist.push_back(slot);
+          ++ctx->scf_diis_resets;
         }
       }
     } else {
@@ -882,6 +889,7 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   double energy = VN;
   if (int rc = setup(d, H, S)) return rc;
   ctx->scf_replays = 0;
+  ctx->scf_diis_resets = 0;
   if (opt->diis <= kMaxHist) {
     gate.reset();
     const int rc = scf_async(d, r, &status, &energy, &iterations);
@@ -941,6 +949,15 @@ extern "C" int zq_scf_set_sp2_guess_budget(zq_ctx *ctx, int steps) {
   ctx->sp2_guess_budget = steps;
   return ZQ_OK;
 }
+
+extern "C" int zq_scf_set_diis_error_scale(zq_ctx *ctx, double scale) {
+  if (!ctx || !(scale >= 0.0 && scale <= 1.0))
+    return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_set_diis_error_scale: scale outside [0, 1]");
+  ctx->diis_err_scale = scale;
+  return ZQ_OK;
+}
+
+extern "C" int zq_scf_last_diis_resets(const zq_ctx *ctx) { return ctx ? ctx->scf_diis_resets : 0; }
 
 extern "C" int zq_scf_last_replays(const zq_ctx *ctx) { return ctx ? ctx->scf_replays : 0; }
 

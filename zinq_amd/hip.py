@@ -18,7 +18,9 @@ EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_l
             "zq_mp2", "zq_profile_enable", "zq_profile_reset", "zq_profile_get", "zq_device_bytes", "zq_scf_run",
             "zq_host_sygv", "zq_scf_run_device", "zq_scf_set_sp2_budget", "zq_eri_set_create", "zq_eri_set_destroy",
             "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
-            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_scf_set_sp2_guess_budget", "zq_scf_last_replays", "zq_mp2_noalloc", "zq_ao2mo_pp", "zq_ci", "zq_mp", "zq_grad_eri", "zq_ao2mo_dual", "zq_jk_record_count"]
+            "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_scf_set_sp2_guess_budget",
+            "zq_scf_set_diis_error_scale", "zq_scf_last_diis_resets", "zq_scf_last_replays", "zq_mp2_noalloc",
+            "zq_ao2mo_pp", "zq_ci", "zq_mp", "zq_grad_eri", "zq_ao2mo_dual", "zq_jk_record_count"]
 
 
 class GpuError(RuntimeError):
@@ -95,6 +97,8 @@ def lib():
         L.zq_scf_last_driver.argtypes = [vp]
         L.zq_scf_captures.argtypes = [vp]
         L.zq_scf_set_sp2_guess_budget.argtypes = [vp, ci]
+        L.zq_scf_set_diis_error_scale.argtypes = [vp, cd]
+        L.zq_scf_last_diis_resets.argtypes = [vp]
         L.zq_scf_last_replays.argtypes = [vp]
         L.zq_scf_last_sp2.argtypes = [vp, vp, vp, vp]
         L.zq_eri_set_create.restype = vp
@@ -336,6 +340,15 @@ class Context:
     def set_sp2_guess_budget(self, steps):
         """SP2 steps for the device SCF's guess density (zq_scf_set_sp2_guess_budget)."""
         self._check(self._L.zq_scf_set_sp2_guess_budget(self._h, int(steps)))
+
+    def set_diis_error_scale(self, scale):
+        """Factor on the device SCF's stored DIIS error vectors, in [0, 1] (zq_scf_set_diis_error_scale);
+        0 makes every DIIS system singular, so each iteration takes the history reset."""
+        self._check(self._L.zq_scf_set_diis_error_scale(self._h, float(scale)))
+
+    def last_diis_resets(self):
+        """Singular DIIS systems the last device SCF met (zq_scf_last_diis_resets)."""
+        return int(self._L.zq_scf_last_diis_resets(self._h))
 
     def set_sp2_budget(self, steps):
         """SP2 steps per density in the captured device-SCF iteration (zq_scf_set_sp2_budget)."""

@@ -11,16 +11,20 @@
 //     row k -> LDS; alpha = A[k+1][k], beta = -sign(alpha) |A[k+1:][k]|, tau = (beta - alpha) / beta,
 //     v = (1, A[k+2:][k] / (alpha - beta));  p = tau A v;  w = p - tau/2 (p.v) v;  A -= v w^T + w v^T,
 //   two barriers a step; every wave computes the step's scalars itself, in the same order, so all agree
-//   bitwise; column blocks wholly left of k are skipped.  The reflectors stay in LDS, and Q^T =
-//   H_(n-3) ... H_0 is then accumulated in the same registers by right multiplications M <- M H_k,
-//   whose row sums (quad sums) need no barrier.
+//   bitwise; column blocks wholly left of k are skipped.  The reflectors stay in LDS until the end,
+//   when they go to global memory.
+// k_tridiag_q: Q^T = H_(n-3) ... H_0 accumulated in registers (same layout) by right multiplications
+//   M <- M H_k, whose row sums (quad sums) need no barrier.  It runs on a second stream while the
+//   tridiagonal problem is solved, which needs only d and e.
 // The tridiagonal problem T = Z diag(w) Z^T is rocSOLVER dstedc's divide and conquer, and the
-// eigenvectors Q Z are one MFMA product.
+// eigenvectors Q Z are one MFMA product once both streams are done.
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/zinq_scf.h"
@@ -50,7 +54,8 @@ __device__ inline double row_sum_last(double v) {
 // Offset of reflector k in the packed store: entries k+1 .. n-1.
 __device__ inline int vofs(int k, int n) { return k * (n - 1) - k * (k - 1) / 2; }
 
-// A (n x n symmetric, ld lda) -> d (n), e (n - 1), Q (n x n row-major, ld n: A = Q T Q^T), n <= 16 RB.
+// A (n x n symmetric, ld lda) -> d (n), e (n - 1) and the reflectors R (refl_words(n): tau, scal, the
+// packed row tails) of A = Q T Q^T, n <= 16 RB.
 // One workgroup of 16 TR threads per matrix of a strided batch.  Thread (tr, tc) = (tid / 16, tid % 16)
 // owns rows tr + TR a (a < RA) and columns tc + 16 b (b < RB), at most 72 doubles (TR = 32 keeps two
 // waves per SIMD, so they fit the 256 registers a lane then has); a DPP row of 16 lanes holds one
@@ -60,11 +65,12 @@ __device__ inline int vofs(int k, int n) { return k * (n - 1) - k * (k - 1) / 2;
 // step waits on a 64-lane shuffle reduction.  Row vectors in LDS are stored column-permuted,
 // [c % 16][c / 16] with a padded stride, so a lane reads its own columns contiguously and the 16
 // column groups fall in different banks.  The reflectors (raw row tails and 1 / (alpha - beta)), d, e
-// and tau stay in LDS (at most 156 KB for n = 192), so no barrier waits for a global store.
+// and tau stay in LDS (at most 156 KB for n = 192) until the loop has ended, so no barrier waits for a
+// global store.
 template <int RB, int TR>
 __global__ __launch_bounds__(16 * TR) void k_tridiag(const double *__restrict__ A, int lda, long long sA, int n,
                                                   double *__restrict__ d, double *__restrict__ e, long long sDE,
-                                                  double *__restrict__ Q, long long sQ) {
+                                                  double *__restrict__ R, long long sR) {
   constexpr int NT = 16 * TR, NM = 16 * RB, RA = (NM + TR - 1) / TR, CS = RB + 1;
   __shared__ double cbp[2][16 * CS];  // row k, permuted
   __shared__ double pvp[16 * CS];     // p = tau A v, permuted
@@ -74,7 +80,7 @@ __global__ __launch_bounds__(16 * TR) void k_tridiag(const double *__restrict__ 
   A += blockIdx.x * sA;
   d += blockIdx.x * sDE;
   e += blockIdx.x * sDE;
-  Q += blockIdx.x * sQ;
+  R += blockIdx.x * sR;
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15, wv = tid >> 6;
   auto P = [](int j) { return (j & 15) * CS + (j >> 4); };
   double a[RA][RB];
@@ -198,8 +204,30 @@ __global__ __launch_bounds__(16 * TR) void k_tridiag(const double *__restrict__ 
   __syncthreads();  // reflectors, taus, d and e complete
   for (int t = tid; t < n; t += NT) d[t] = dl[t];
   for (int t = tid; t + 1 < n; t += NT) e[t] = el[t];
+  // the reflectors for k_tridiag_q: tau_k and scal_k (k < n - 2), then the packed row tails
+  const int nk = n > 2 ? n - 2 : 0, nv = n > 2 ? vofs(n - 2, n) : 0;
+  for (int t = tid; t < nk; t += NT) R[t] = taus[t], R[n + t] = scals[t];
+  for (int t = tid; t < nv; t += NT) R[2 * n + t] = vl[t];
+}
 
-  // M = Q^T = H_(n-3) ... H_0, from M = I by M <- M H_k = M - tau (M v) v^T for k = n-3 .. 0
+// Q from the reflectors k_tridiag left in R (one workgroup per matrix of the batch, the same thread
+// layout): M = Q^T = H_(n-3) ... H_0, from M = I by M <- M H_k = M - tau (M v) v^T for k = n-3 .. 0,
+// whose row sums (quad sums) need no barrier; then Q[c][r] = M[r][c].  It runs beside the tridiagonal
+// eigensolver, which needs only d and e.
+template <int RB, int TR>
+__global__ __launch_bounds__(16 * TR) void k_tridiag_q(const double *__restrict__ R, long long sR, int n,
+                                                    double *__restrict__ Q, long long sQ) {
+  constexpr int NT = 16 * TR, NM = 16 * RB, RA = (NM + TR - 1) / TR;
+  __shared__ double taus[NM], scals[NM];
+  __shared__ double vl[(NM - 2) * (NM + 1) / 2];
+  R += blockIdx.x * sR;
+  Q += blockIdx.x * sQ;
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const int nk = n > 2 ? n - 2 : 0, nv = n > 2 ? vofs(n - 2, n) : 0;
+  for (int t = tid; t < nk; t += NT) taus[t] = R[t], scals[t] = R[n + t];
+  for (int t = tid; t < nv; t += NT) vl[t] = R[2 * n + t];
+  __syncthreads();
+  double a[RA][RB];
 #pragma unroll
   for (int ai = 0; ai < RA; ++ai)
 #pragma unroll
@@ -246,10 +274,20 @@ __global__ __launch_bounds__(16 * TR) void k_tridiag(const double *__restrict__ 
     }
 }
 
+// Words of the reflector store of one n x n matrix: tau and scal per step, the packed row tails.
+inline size_t refl_words(int n) { return 2 * size_t(n) + (n > 2 ? size_t(n - 2) * (n + 1) / 2 : 0); }
+
+// The reduction on s; Q on s2, which waits for it, while s goes on (the caller joins s2 back).
 template <int RB, int TR>
-void launch_tridiag(hipStream_t s, int count, const double *A, int lda, long long sA, int n, double *d, double *e,
-                    long long sDE, double *Q, long long sQ) {
-  hipLaunchKernelGGL((k_tridiag<RB, TR>), dim3(count), dim3(16 * TR), 0, s, A, lda, sA, n, d, e, sDE, Q, sQ);
+hipError_t launch_tridiag(hipStream_t s, hipStream_t s2, hipEvent_t fork, int count, const double *A, int lda,
+                          long long sA, int n, double *d, double *e, long long sDE, double *R, double *Q,
+                          long long sQ) {
+  const long long sR = (long long)refl_words(n);
+  hipLaunchKernelGGL((k_tridiag<RB, TR>), dim3(count), dim3(16 * TR), 0, s, A, lda, sA, n, d, e, sDE, R, sR);
+  if (hipError_t err = hipEventRecord(fork, s)) return err;
+  if (hipError_t err = hipStreamWaitEvent(s2, fork, 0)) return err;
+  hipLaunchKernelGGL((k_tridiag_q<RB, TR>), dim3(count), dim3(16 * TR), 0, s2, R, sR, n, Q, sQ);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -270,6 +308,24 @@ namespace {
 
 rocblas_handle eig_handle(zq_ctx *ctx) { return (rocblas_handle)zq_blas(ctx); }
 
+// The stream the Q accumulations of every context on a device share, created on first use and kept
+// until the process ends: a stream per context would add a hardware queue per context, and streams
+// beyond the hardware queues share them, ordering unrelated work.  Two contexts' accumulations may
+// queue behind each other here; each still waits only for its own reduction.
+hipError_t device_eig_stream(int device, hipStream_t *out) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  std::lock_guard<std::mutex> lock(mu);
+  hipStream_t &st = streams[device];
+  if (!st)
+    if (hipError_t err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) {
+      st = nullptr;
+      return err;
+    }
+  *out = st;
+  return hipSuccess;
+}
+
 }  // namespace
 
 int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *info) {
@@ -286,21 +342,37 @@ int zq_syev_rows(zq_ctx *ctx, double *A, int n, double *w, double *offd, int *in
   const size_t nn = size_t(n) * n;
   auto *Qm = (double *)zq_scratch(ctx, "eig_Q", nn * sizeof(double));
   auto *Z = (double *)zq_scratch(ctx, "eig_Z", nn * sizeof(double));
-  if (!Qm || !Z) {
+  // no captured SCF iteration holds the reflector store, so growing it keeps the capture cache
+  auto *R = (double *)zq_scratch(ctx, "eig_refl", refl_words(n) * sizeof(double), nullptr, false);
+  if (!Qm || !Z || !R) {
     zq_prof_end(ctx);
     return zq_fail(ctx, ZQ_ERR_OOM, "eigen-solver buffers");
   }
+  hipStream_t s2 = nullptr;
+  ZQ_TRY(ctx, device_eig_stream(ctx->device, &s2));
+  for (hipEvent_t &ev : ctx->eig_events)
+    if (!ev) ZQ_TRY(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   hipStream_t s = ctx->stream;
+  hipEvent_t fork = ctx->eig_events[0], join = ctx->eig_events[1];
+  hipError_t le;
   if (n <= 64)
-    launch_tridiag<4, 64>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0);
+    le = launch_tridiag<4, 64>(s, s2, fork, 1, A, n, 0, n, w, offd, 0, R, Qm, 0);
   else if (n <= 128)
-    launch_tridiag<8, 64>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0);
+    le = launch_tridiag<8, 64>(s, s2, fork, 1, A, n, 0, n, w, offd, 0, R, Qm, 0);
   else
-    launch_tridiag<12, 32>(s, 1, A, n, 0, n, w, offd, 0, Qm, 0);
-  ZQ_LAUNCH_CHECK(ctx, "k_tridiag");
-  // T = Z diag(w) Z^T: column k of the column-major Z (= row k of Z as stored) is eigenvector k
+    le = launch_tridiag<12, 32>(s, s2, fork, 1, A, n, 0, n, w, offd, 0, R, Qm, 0);
+  if (le != hipSuccess) {
+    zq_prof_end(ctx);
+    return zq_check(ctx, le, "k_tridiag");
+  }
+  // T = Z diag(w) Z^T: column k of the column-major Z (= row k of Z as stored) is eigenvector k; Q is
+  // accumulated on s2 meanwhile
   const rocblas_status st = rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, offd, Z, n, info);
+  // s waits for Q even when the solver failed, so no later work on s overtakes it
+  const hipError_t je = hipEventRecord(join, s2);
+  const hipError_t we = je == hipSuccess ? hipStreamWaitEvent(s, join, 0) : je;
   zq_prof_end(ctx);  // before the product, which times itself (profile regions do not nest)
+  if (we != hipSuccess) return zq_check(ctx, we, "eigen-solver join");
   if (st != rocblas_status_success) return zq_fail(ctx, ZQ_ERR_LAPACK, "rocsolver_dstedc");
   // row k of A = (Q z_k)^T: A[k][p] = sum_q Z[k][q] Q[p][q]
   return zq_gemm_nt(ctx, Z, n, Qm, n, A, n, n, n, n, 1.0, "scf_eig");

@@ -111,6 +111,9 @@ struct zq_ctx {
 
   // rocBLAS handle used by the device SCF's eigensolver (created on first use)
   void *solver = nullptr;
+  // fork/join events of the eigensolver's Q accumulation, which runs on a stream shared by the device's
+  // contexts beside the tridiagonal solve
+  hipEvent_t eig_events[2] = {nullptr, nullptr};
 
   // captured SCF iterations (hipGraphExec_t) by the key they were captured for: sizes, options, and
   // the residency (the attached set's id, else the ERI load's generation), so that a context serving

@@ -862,6 +862,8 @@ void zq_release_solver(zq_ctx *ctx) {
   drop_graphs(ctx);
   for (hipEvent_t &e : ctx->scf_events)
     if (e) (void)hipEventDestroy(e), e = nullptr;
+  for (hipEvent_t &e : ctx->eig_events)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
 }
 
 extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const double *H, const double *S, double VN,

@@ -330,13 +330,16 @@ __global__ __launch_bounds__(256) void k_dots_state(const double *__restrict__ h
   if (threadIdx.x == 0) part[blockIdx.y * kRedBlocks + blockIdx.x] = red[0];
 }
 
-// LU with partial pivoting and the triangular solves of the host zq::getrf/getrs, on one wave: lane i
-// holds row i of A (n <= kLuMax) and b[i] in registers.  Every element sees the same operations in the
-// same order as in the serial steps: the same pivot (the first row with the largest |A[i][k]|), the same
-// multipliers and updates, the forward sums in ascending j, the backward sums in ascending j.  The rows
-// of b are swapped as the pivots are found, which permutes b exactly as applying the swaps afterwards
-// does.  Returns false, on every lane, for a zero pivot; otherwise lane i < n holds x[i] in b.
+// LU with partial pivoting and the triangular solves of the host zq::getrf/getrs, on one wave: row i
+// of A (n <= kLuMax) in LDS at A[i * kLuLd], b[i] in lane i's register.  Every element sees the same
+// operations in the same order as in the serial steps: the same pivot (the first row with the largest
+// |A[i][k]|), the same multipliers and updates, the forward sums in ascending j, the backward sums in
+// ascending j.  The rows of b are swapped as the pivots are found, which permutes b exactly as applying
+// the swaps afterwards does.  The loops run to n, not to kLuMax: the code stays small, which matters for
+// a kernel that runs once per iteration with its instructions not in the cache.  Returns false, on every
+// lane, for a zero pivot; otherwise lane i < n holds x[i] in b.
 constexpr int kLuMax = kMaxHist + 1;
+constexpr int kLuLd = kLuMax + 1;  // row stride of A in LDS
 
 // lane src's v on every lane; src is wave-uniform, so this is two v_readlane, not an LDS permute
 __device__ inline double lane_bcast(double v, int src) {
@@ -344,71 +347,46 @@ __device__ inline double lane_bcast(double v, int src) {
                           __builtin_amdgcn_readlane(__double2loint(v), src));
 }
 
-__device__ __forceinline__ bool lu_solve_wave(int n, double (&a)[kLuMax], double &b) {
-  // every loop has a constant trip count and guards its body, so that it unrolls fully and a[] stays in
-  // registers; every row or value that moves between lanes comes from a wave-uniform lane (readlane)
+__device__ bool lu_solve_lds(int n, double *A, double &b) {
   const int lane = threadIdx.x & 63;
-  bool ok = true;
-#pragma unroll
-  for (int k = 0; k < kLuMax; ++k) {
-    if (k < n && ok) {
-      // pivot: the first row i >= k with the largest |A[i][k]|, scanned in row order as the serial steps do
-      int p = k;
-      double best = fabs(lane_bcast(a[k], k));
-#pragma unroll
-      for (int i = k + 1; i < kLuMax; ++i)
-        if (i < n) {
-          const double v = fabs(lane_bcast(a[k], i));
-          if (v > best) best = v, p = i;
-        }
-      ok = best != 0.0;
-      if (ok && p != k) {
-#pragma unroll
-        for (int j = 0; j < kLuMax; ++j)
-          if (j < n) {
-            const double rk = lane_bcast(a[j], k), rp = lane_bcast(a[j], p);
-            a[j] = lane == k ? rp : (lane == p ? rk : a[j]);
-          }
-        const double bk = lane_bcast(b, k), bp = lane_bcast(b, p);
-        b = lane == k ? bp : (lane == p ? bk : b);
-      }
-      const double akk = lane_bcast(a[k], k);
-      const bool below = ok && lane > k && lane < n;
-      double f = 0.0;
-      if (below) {
-        f = a[k] / akk;
-        a[k] = f;
-      }
-#pragma unroll
-      for (int j = k + 1; j < kLuMax; ++j) {
-        if (j < n) {
-          const double akj = lane_bcast(a[j], k);
-          if (below) a[j] -= f * akj;
-        }
-      }
+  for (int k = 0; k < n; ++k) {
+    // pivot: every lane scans column k in row order, so all find the same row
+    int p = k;
+    double best = fabs(A[k * kLuLd + k]);
+    for (int i = k + 1; i < n; ++i) {
+      const double v = fabs(A[i * kLuLd + k]);
+      if (v > best) best = v, p = i;
     }
+    if (best == 0.0) return false;
+    p = __builtin_amdgcn_readfirstlane(p);
+    if (p != k) {
+      if (lane < n) {  // lane j swaps column j of rows k and p
+        const double rk = A[k * kLuLd + lane], rp = A[p * kLuLd + lane];
+        A[k * kLuLd + lane] = rp;
+        A[p * kLuLd + lane] = rk;
+      }
+      const double bk = lane_bcast(b, k), bp = lane_bcast(b, p);
+      b = lane == k ? bp : (lane == p ? bk : b);
+      __syncthreads();
+    }
+    if (lane > k && lane < n) {
+      const double f = A[lane * kLuLd + k] / A[k * kLuLd + k];
+      A[lane * kLuLd + k] = f;
+      for (int j = k + 1; j < n; ++j) A[lane * kLuLd + j] -= f * A[k * kLuLd + j];
+    }
+    __syncthreads();
   }
-  if (!ok) return false;
-#pragma unroll
-  for (int j = 0; j < kLuMax; ++j) {
-    if (j < n) {
+  for (int j = 0; j < n; ++j) {
+    const double bj = lane_bcast(b, j);
+    if (lane > j && lane < n) b -= A[lane * kLuLd + j] * bj;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double acc = b;
+    for (int j = i + 1; j < n; ++j) {
       const double bj = lane_bcast(b, j);
-      if (lane > j && lane < n) b -= a[j] * bj;
+      if (lane < n) acc -= A[lane * kLuLd + j] * bj;
     }
-  }
-#pragma unroll
-  for (int i = kLuMax - 1; i >= 0; --i) {
-    if (i < n) {
-      double acc = b;
-#pragma unroll
-      for (int j = i + 1; j < kLuMax; ++j) {
-        if (j < n) {
-          const double bj = lane_bcast(b, j);
-          acc -= a[j] * bj;
-        }
-      }
-      if (lane == i) b = acc / a[i];
-    }
+    if (lane == i) b = acc / A[i * kLuLd + i];
   }
   return true;
 }
@@ -430,68 +408,49 @@ __global__ __launch_bounds__(64) void k_diis_solve(const double *__restrict__ pa
   double dp[kMaxHist];
 #pragma unroll
   for (int h = 0; h < kMaxHist; ++h) hist[h] = st->hist[h];
+  const int hist_lane = st->hist[min(lane, kMaxHist - 1)];
 #pragma unroll
   for (int k = 0; k < kMaxHist; ++k) dp[k] = part[(1 + k) * kRedBlocks + lane];
   double e = 0.0;
   for (int b = lane; b < nep; b += 64) e += epart[b];
+  // the live slots as a mask, the history and the dot products in LDS (indexed at run time below)
+  __shared__ int hs[kMaxHist];
+  __shared__ double dots[kMaxHist];
+  unsigned live = 0;
 #pragma unroll
-  for (int h = 0; h < kMaxHist; ++h) hist[h] = h < m ? hist[h] : -1;
+  for (int h = 0; h < kMaxHist; ++h) live |= h < m ? 1u << hist[h] : 0u;
+  if (lane < kMaxHist) hs[lane] = lane < m ? hist_lane : -1;
   e = zqt::wave_sum(e);
-  double dots[kMaxHist];
 #pragma unroll
   for (int k = 0; k < kMaxHist; ++k) {
-    bool live = false;
-#pragma unroll
-    for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
-    const double d = zqt::wave_sum(dp[k]);
-    dots[k] = (live && diis > 0) ? d : 0.0;
+    double d = 0.0;
+    if (diis > 0 && (live >> k & 1)) d = zqt::wave_sum(dp[k]);  // uniform branch
+    if (lane == 0) dots[k] = d;
   }
+  __syncthreads();
   // the m x m block of B over the history, one entry per lane, with the new row and column from dots
   for (int idx = lane; diis > 0 && m >= 2 && idx < m * m; idx += 64) {
     const int i = idx / m, j = idx % m;
-    int si = 0, sj = 0;
-#pragma unroll
-    for (int h = 0; h < kMaxHist; ++h) {
-      if (h == i) si = hist[h];
-      if (h == j) sj = hist[h];
-    }
-    double b;
-    if (si == slot || sj == slot) {
-      const int other = si == slot ? sj : si;
-      b = 0.0;
-#pragma unroll
-      for (int k = 0; k < kMaxHist; ++k)
-        if (k == other) b = dots[k];
-    } else {
-      b = st->B[si * kMaxHist + sj];
-    }
-    Bs[i * kMaxHist + j] = b;
+    const int si = hs[i], sj = hs[j];
+    Bs[i * kMaxHist + j] = (si == slot || sj == slot) ? dots[si == slot ? sj : si] : st->B[si * kMaxHist + sj];
   }
   __syncthreads();
   if (lane == 0) {
     st->e_new = e + VN;
     st->combine = 0;
-    if (diis > 0) {
-#pragma unroll
-      for (int k = 0; k < kMaxHist; ++k) {
-        bool live = false;
-#pragma unroll
-        for (int h = 0; h < kMaxHist; ++h) live |= hist[h] == k;
-        if (live) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
-      }
-    }
+    if (diis > 0)
+      for (int k = 0; k < kMaxHist; ++k)
+        if (live >> k & 1) st->B[slot * kMaxHist + k] = st->B[k * kMaxHist + slot] = dots[k];
   }
   if (diis <= 0 || m < 2) return;
-  // the bordered system [B -1; -1 0] c = [0; -1], row i on lane i
-  double a[kLuMax], c = lane == m ? -1.0 : 0.0;
-#pragma unroll
-  for (int j = 0; j < kLuMax; ++j) {
-    double x = 0.0;
-    if (lane < m) x = j < m ? Bs[lane * kMaxHist + j] : (j == m ? -1.0 : 0.0);
-    else if (lane == m) x = j < m ? -1.0 : 0.0;
-    a[j] = x;
-  }
-  if (lu_solve_wave(m + 1, a, c)) {
+  // the bordered system [B -1; -1 0] c = [0; -1], row i written by lane i
+  __shared__ double Alu[kLuMax * kLuLd];
+  if (lane <= m)
+    for (int j = 0; j <= m; ++j)
+      Alu[lane * kLuLd + j] = lane < m ? (j < m ? Bs[lane * kMaxHist + j] : -1.0) : (j < m ? -1.0 : 0.0);
+  double c = lane == m ? -1.0 : 0.0;
+  __syncthreads();
+  if (lu_solve_lds(m + 1, Alu, c)) {
     if (lane < m) st->coef[lane] = c;
     if (lane == 0) st->combine = 1;
   } else if (lane == 0) {

@@ -147,18 +147,18 @@ __device__ bool lu_solve(int n, double *A, double *b) {
   return true;
 }
 
-// One SCF step (MODE_ITER) or the core-Hamiltonian guess (MODE_GUESS) of molecule blockIdx.x.
+// One SCF step (MODE_ITER) or the core-Hamiltonian guess (MODE_GUESS) of molecule order[blockIdx.x].
 // The SP2 iterate X lives in dynamic LDS (mp x (mp + 1) doubles, sized for the batch's largest mp) for
 // the whole step loop: each wave computes its 16 x 16 tiles of X^2 from LDS into registers, and after a
 // barrier writes its tiles' next X back in place.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_batch_step(double *__restrict__ ws, const BMol *__restrict__ mols,
-                                                   BState *__restrict__ st, int *__restrict__ done, int mode, int diis,
-                                                   double thr, int maxit) {
+                                                   const int *__restrict__ order, BState *__restrict__ st,
+                                                   int *__restrict__ done, int mode, int diis, double thr, int maxit) {
   __shared__ double red[2 * (NT / 64)];
   extern __shared__ double Xl[];
   __shared__ double Am[(kHist + 1) * (kHist + 1)], c[kHist + 1];  // the bordered DIIS system (thread 0)
-  const int mi = blockIdx.x, tid = threadIdx.x;
+  const int mi = order[blockIdx.x], tid = threadIdx.x;
   if (mode == MODE_ITER && done[mi]) return;
   const BMol M = mols[mi];
   BState &S = st[mi];
@@ -441,6 +441,7 @@ struct zq_batch {
   // device residency
   double *eri = nullptr, *ws = nullptr, *rec = nullptr;
   BMol *d_mols = nullptr;
+  int *d_order = nullptr;  // molecules by descending padded size: the step kernel's workgroup order
   BState *d_state = nullptr;
   int *d_done = nullptr, *d_count = nullptr, *d_ptr = nullptr, *d_idx = nullptr;
   JKItem *d_items = nullptr;
@@ -554,10 +555,11 @@ int eig_buffers(zq_batch *b, size_t etot, size_t wtot, size_t M) {
 void free_all(zq_batch *b) {
   free_plan(b);
   free_eig(b);
-  for (void *p : {(void *)b->eri, (void *)b->ws, (void *)b->d_mols, (void *)b->d_state, (void *)b->d_done,
-                  (void *)b->d_count, (void *)b->d_jkm, (void *)b->eri32, b->mp2_ws})
+  for (void *p : {(void *)b->eri, (void *)b->ws, (void *)b->d_mols, (void *)b->d_order, (void *)b->d_state,
+                  (void *)b->d_done, (void *)b->d_count, (void *)b->d_jkm, (void *)b->eri32, b->mp2_ws})
     if (p) (void)hipFree(p);
-  b->eri = nullptr, b->ws = nullptr, b->d_mols = nullptr, b->d_state = nullptr, b->d_done = nullptr;
+  b->eri = nullptr, b->ws = nullptr, b->d_mols = nullptr, b->d_order = nullptr, b->d_state = nullptr;
+  b->d_done = nullptr;
   b->d_count = nullptr, b->d_jkm = nullptr, b->eri32 = nullptr, b->mp2_ws = nullptr, b->mp2_ws_bytes = 0;
   b->uploaded = false;
 }
@@ -640,6 +642,7 @@ int upload(zq_batch *b) {
   b->eri_count = eri_total;
   BTRY(b, hipMalloc(&b->ws, ws_total * sizeof(double)));
   BTRY(b, hipMalloc(&b->d_mols, M * sizeof(BMol)));
+  BTRY(b, hipMalloc(&b->d_order, M * sizeof(int)));
   BTRY(b, hipMalloc(&b->d_state, M * sizeof(BState)));
   BTRY(b, hipMalloc(&b->d_done, M * sizeof(int)));
   BTRY(b, hipMalloc(&b->d_count, sizeof(int)));
@@ -648,6 +651,13 @@ int upload(zq_batch *b) {
   BTRY(b, hipMemsetAsync(b->d_done, 0, M * sizeof(int), b->stream));
   BTRY(b, hipMemcpyAsync(b->d_mols, bm.data(), M * sizeof(BMol), hipMemcpyHostToDevice, b->stream));
   BTRY(b, hipMemcpyAsync(b->d_jkm, jm.data(), M * sizeof(JKMol), hipMemcpyHostToDevice, b->stream));
+  // largest first: a step of a large molecule takes a compute unit for longest, so the small ones fill
+  // in behind it rather than leaving a large one for the end (one workgroup per compute unit at 512
+  // threads)
+  std::vector<int> order(M);
+  for (size_t m = 0; m < M; ++m) order[m] = int(m);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return bm[x].mp > bm[y].mp; });
+  BTRY(b, hipMemcpyAsync(b->d_order, order.data(), M * sizeof(int), hipMemcpyHostToDevice, b->stream));
   BTRY(b, hipStreamSynchronize(b->stream));
   double *canon = nullptr;
   BTRY(b, hipMalloc(&canon, canon_max * sizeof(double)));
@@ -696,11 +706,11 @@ void launch_step(zq_batch *b, int mode, int diis, double thr, int maxit) {
   const dim3 grid(unsigned(b->mols.size()));
   const size_t lds = size_t(b->mp_max) * (b->mp_max + 1) * sizeof(double);  // the SP2 iterate
   if (b->nt == 512)
-    hipLaunchKernelGGL(k_batch_step<512>, grid, dim3(512), lds, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
-                       diis, thr, maxit);
+    hipLaunchKernelGGL(k_batch_step<512>, grid, dim3(512), lds, b->stream, b->ws, b->d_mols, b->d_order, b->d_state,
+                       b->d_done, mode, diis, thr, maxit);
   else
-    hipLaunchKernelGGL(k_batch_step<256>, grid, dim3(256), lds, b->stream, b->ws, b->d_mols, b->d_state, b->d_done, mode,
-                       diis, thr, maxit);
+    hipLaunchKernelGGL(k_batch_step<256>, grid, dim3(256), lds, b->stream, b->ws, b->d_mols, b->d_order, b->d_state,
+                       b->d_done, mode, diis, thr, maxit);
 }
 
 int enqueue_iteration(zq_batch *b, int diis, double thr, int maxit) {

@@ -73,7 +73,9 @@ def run_one(m, mode, device_scf):
     r = m["ctx"].scf(m["H"], m["S"], m["VN"], m["o"], diis=8, iterations=100, threshold=mode["threshold"],
                      device=device_scf, fock_precision=mode["fock"] if device_scf else "fp64")
     driver = m["ctx"].last_scf_driver() if device_scf else "host"
-    e2 = m["ctx"].mp2(r["C"], r["e"], m["o"], mode["transform"])
+    # after the device SCF its orbitals are still on the device: the MP2 takes them there
+    e2 = (m["ctx"].mp2_resident(m["o"], mode["transform"]) if device_scf
+          else m["ctx"].mp2(r["C"], r["e"], m["o"], mode["transform"]))
     return dict(k=m["k"], E_rhf=r["E"], E_mp2=r["E"] + e2, iterations=r["iterations"], scf_driver=driver)
 
 

@@ -105,6 +105,14 @@ int zq_ao2mo_dual(zq_ctx *ctx, zq_eri_set *dset, const double *C, const double *
  * equal to the order-2 term of mp() (src/moller_plesset.zig:340-481) for an RHF reference. */
 int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision, double *e_corr);
 
+/* zq_mp2 on the orbitals and orbital energies of the last successful zq_scf_run_device on ctx
+ * (include/zinq_scf.h), which stay on the device: the same kernels and result as zq_mp2 with the C and
+ * e that call returned, without sending them back to the device.  ZQ_ERR_STATE if ctx holds no such
+ * result for its current ERIs (none yet, a failed run, or zq_eri_load* / zq_attach to another set
+ * since).  No counterpart in the reference, which hands the orbitals to mp() in host memory: it is for
+ * request loops that keep a molecule's ERIs and MO coefficients resident. */
+int zq_mp2_resident(zq_ctx *ctx, size_t nocc, int precision, double *e_corr);
+
 /* ---- determinant consumers of the FULL transform ----
  * Both run the FULL transform of the resident ERIs (FP64) on the GPU and read the spin-orbital
  * integrals from it on the fly: spin orbital p = 2P + s for a restricted reference (the column order of

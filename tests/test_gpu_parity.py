@@ -352,6 +352,27 @@ def test_device_scf_singular_diis_resets(ctx):
             ctx.set_diis_error_scale(bad)
 
 
+def test_mp2_resident_matches_mp2(ctx):
+    """zq_mp2_resident: the MP2 energy from the orbitals and energies the device SCF leaves on the device
+    is bitwise zq_mp2's on the C and e that SCF returned, in every precision, after the plain and the
+    mixed SCF; fresh ERIs and a failed SCF leave no result (ZQ_ERR_STATE)."""
+    from zinq_amd import hip
+    for kind in ("ccpvdz", "w2"):
+        s, H, S, g, VN = _load(ctx, kind)
+        o = s.nel // 2
+        with pytest.raises(hip.BadState):
+            ctx.mp2_resident(o)
+        for fock, thr in (("fp64", 1e-10), ("fp32", 1e-6)):
+            r = ctx.scf(H, S, VN, o, threshold=thr, device=True, fock_precision=fock)
+            for prec in ("fp64", "fp32", "fp32c", "bf16x3"):
+                a, b = ctx.mp2(r["C"], r["e"], o, prec), ctx.mp2_resident(o, prec)
+                assert a == b, (kind, fock, prec, a, b)
+    with pytest.raises(hip.ScfDidNotConverge):
+        ctx.scf(H, S, VN, o, iterations=1, device=True)
+    with pytest.raises(hip.BadState):
+        ctx.mp2_resident(o)
+
+
 @pytest.mark.parametrize("kind", ["ccpvdz", "w2"])
 def test_mixed_path_within_bar(ctx, kind):
     """Mixed path: incremental Fock builds from FP32 ERIs to 1e-6, one full FP64 build, FP32C transform.

@@ -63,6 +63,9 @@ def _fake_hip(budget_box, log):
                 raise RuntimeError("set freed while in use")
             return -1e-3 * self.set.n
 
+        def mp2_resident(self, nocc, precision):
+            return self.mp2(None, None, nocc, precision)
+
         def close(self):
             pass
 

@@ -125,6 +125,12 @@ struct zq_ctx {
   unsigned scratch_gen = 0, eri_gen = 0;
   int sp2_budget = 100;  // SP2 steps per density in the asynchronous SCF; grows after a fallback
   int sp2_guess_budget = 128;  // SP2 steps for the core-Hamiltonian guess density of that SCF
+  // the orbitals (n x n, C[mu][k]) and orbital energies of the last successful zq_scf_run_device, left on
+  // the device for zq_mp2_resident, with the residency they belong to (null: none)
+  const double *res_C = nullptr, *res_e = nullptr;
+  int res_n = 0;
+  unsigned long long res_set = 0;
+  unsigned res_gen = 0;
   double diis_err_scale = 1.0;  // factor on the device SCF's stored DIIS error vectors (a test hook)
   int scf_diis_resets = 0;  // singular DIIS systems the last device SCF reset its history on
   int scf_replays = 0;   // iterations the asynchronous driver enqueued in the last device SCF

@@ -838,6 +838,7 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   if (opt->fock_precision != ZQ_PREC_FP64 && opt->fock_precision != ZQ_PREC_FP32)
     return zq_fail(ctx, ZQ_ERR_INVALID, "zq_scf_run_device: fock_precision must be ZQ_PREC_FP64 or ZQ_PREC_FP32");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  ctx->res_C = ctx->res_e = nullptr;  // until this run succeeds
   const int n = int(n_), nocc = int(nocc_), np = ctx->npad;
   const bool generalized = opt->generalized != 0;
   Run r{n, nocc, np, generalized ? 1.0 : 0.5, generalized ? 1.0 : 2.0, VN, opt, opt->fock_precision == ZQ_PREC_FP32};
@@ -894,7 +895,9 @@ extern "C" int zq_scf_run_device(zq_ctx *ctx, size_t n_, size_t nocc_, const dou
   for (int i = 0; i < n; ++i) std::memcpy(P + size_t(i) * n, hP.data() + size_t(i) * np, n * sizeof(double));
   res->energy = energy;
   res->iterations = iterations;
-  return status == 1 ? ZQ_OK : zq_fail(ctx, ZQ_ERR_NOT_CONVERGED, "SCF did not converge");
+  if (status != 1) return zq_fail(ctx, ZQ_ERR_NOT_CONVERGED, "SCF did not converge");
+  ctx->res_C = d.C, ctx->res_e = d.eig, ctx->res_n = n, ctx->res_set = ctx->set_id, ctx->res_gen = ctx->eri_gen;
+  return ZQ_OK;
 }
 
 extern "C" int zq_scf_set_sp2_budget(zq_ctx *ctx, int steps) {

@@ -977,6 +977,7 @@ struct Dims {
 // terms of the dual transform.
 struct Coefs {
   const double *q[4];
+  bool dev = false;  // q[] are device pointers (one matrix: the context's resident SCF orbitals)
 };
 
 // Bytes run_chain carves from a static arena for these dimensions and one coefficient matrix (a
@@ -1004,7 +1005,8 @@ int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_
   const bool fixed = ar.base != nullptr;
   const double *C = cf.q[0];
   const bool own4 = cf.q[3] != cf.q[2];  // a fourth quarter with its own coefficients (dual transform)
-  if (fixed && (own4 || cf.q[1] != C || cf.q[2] != C)) return zq_fail(ctx, ZQ_ERR_INVALID, "one coefficient matrix only");
+  if ((fixed || cf.dev) && (own4 || cf.q[1] != C || cf.q[2] != C))
+    return zq_fail(ctx, ZQ_ERR_INVALID, "one coefficient matrix only");
   // BF16X3 Q1 takes C1 split and transposed: C1h/C1l[i][mu], mu padded to a multiple of 32
   const int ldt = ((nb + 3) / 4) * 32;
   T *d1, *d2t, *d3, *d4 = nullptr;
@@ -1033,6 +1035,23 @@ int run_chain(zq_ctx *ctx, const Coefs &cf, const Dims &d, int precision, T **d_
                        d2t, d3);
     if (precision == ZQ_PREC_BF16X3)
       hipLaunchKernelGGL(zq_c1_split, dim3(unsigned(m1)), dim3(256), 0, ctx->stream, dC, n, m1, ldt, d1h, d1l);
+    ZQ_LAUNCH_CHECK(ctx, "zq_c_slices");
+  } else if (cf.dev) {
+    // C already on the device: the slices are cut there, as above, into the buffers the host staging
+    // below fills
+    d1 = (T *)zq_scratch(ctx, "C1", size_t(np) * m1 * sizeof(T), nullptr, false);
+    d2t = (T *)zq_scratch(ctx, "C2T", size_t(m2) * np * sizeof(T), nullptr, false);
+    d3 = (T *)zq_scratch(ctx, "C3", size_t(np) * m3 * sizeof(T), nullptr, false);
+    if (precision == ZQ_PREC_BF16X3) {
+      d1h = (bf16 *)zq_scratch(ctx, "C1h", size_t(m1) * ldt * sizeof(bf16), nullptr, false);
+      d1l = (bf16 *)zq_scratch(ctx, "C1l", size_t(m1) * ldt * sizeof(bf16), nullptr, false);
+    }
+    if (!d1 || !d2t || !d3 || (precision == ZQ_PREC_BF16X3 && (!d1h || !d1l)))
+      return zq_fail(ctx, ZQ_ERR_OOM, "transform scratch C slices");
+    hipLaunchKernelGGL(zq_c_slices<T>, dim3(unsigned(np)), dim3(256), 0, ctx->stream, C, n, np, m1, m2, m3, d.off3, d1,
+                       d2t, d3);
+    if (precision == ZQ_PREC_BF16X3)
+      hipLaunchKernelGGL(zq_c1_split, dim3(unsigned(m1)), dim3(256), 0, ctx->stream, C, n, m1, ldt, d1h, d1l);
     ZQ_LAUNCH_CHECK(ctx, "zq_c_slices");
   } else {
     // coefficient slices, zero rows for padded AOs: C1 (np x m1), C2T (m2 x np), C3 = C4 (np x m3)
@@ -1250,7 +1269,9 @@ extern "C" int zq_ao2mo(zq_ctx *ctx, const double *C, size_t nocc, int block, in
 }
 
 namespace {
-int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr, Arena &ar) {
+// C and e on the host, or with dev on the device (the context's resident SCF orbitals and energies).
+int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precision, double *e_corr, Arena &ar,
+           bool dev = false) {
   Dims dm;
   if (int rc = make_dims(ctx, nocc, ZQ_BLOCK_OVOV, precision, dm)) return rc;
   if (ar.base && chain_bytes(dm, precision) + 2 * 256 + size_t(ctx->n) * 8 + size_t(nocc) * nocc * 8 + 8 > ar.cap)
@@ -1258,12 +1279,13 @@ int mp2_in(zq_ctx *ctx, const double *C, const double *e, int nocc, int precisio
   void *T = nullptr;
   int dims[4];
   bool f32 = false, packed = false;
-  if (int rc = transform_in(ctx, Coefs{{C, C, C, C}}, dm, precision, ar, &T, dims, &f32, &packed)) return rc;
+  if (int rc = transform_in(ctx, Coefs{{C, C, C, C}, dev}, dm, precision, ar, &T, dims, &f32, &packed)) return rc;
   const int o = dims[0], v = dims[2], nij = packed ? o * (o + 1) / 2 : o * o;
   double *de = (double *)ar.get("mp2_e", size_t(ctx->n) * 8);
   double *part = (double *)ar.get("mp2_part", size_t(o) * o * 8 + 8);
   if (!de || !part) return zq_fail(ctx, ar.base ? ZQ_ERR_INVALID : ZQ_ERR_OOM, "mp2 scratch");
-  ZQ_TRY(ctx, hipMemcpyAsync(de, e, size_t(ctx->n) * 8, hipMemcpyHostToDevice, ctx->stream));
+  ZQ_TRY(ctx, hipMemcpyAsync(de, e, size_t(ctx->n) * 8, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             ctx->stream));
   zq_prof_begin(ctx, "mp2_energy");
   if (f32)
     hipLaunchKernelGGL(zq_mp2_pairs<float>, dim3(unsigned(nij)), dim3(256), 0, ctx->stream, (const float *)T, de, o, v,
@@ -1546,6 +1568,16 @@ extern "C" int zq_mp2(zq_ctx *ctx, const double *C, const double *e, size_t nocc
   if (!ctx || !C || !e || !e_corr) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2: null argument");
   ZQ_TRY(ctx, hipSetDevice(ctx->device));
   return zq_mp2_device(ctx, C, e, int(nocc), precision, e_corr);
+}
+
+extern "C" int zq_mp2_resident(zq_ctx *ctx, size_t nocc, int precision, double *e_corr) {
+  ZqGate _gate;
+  if (!ctx || !e_corr) return zq_fail(ctx, ZQ_ERR_INVALID, "zq_mp2_resident: null argument");
+  if (!ctx->res_C || ctx->res_n != ctx->n || ctx->res_set != ctx->set_id || ctx->res_gen != ctx->eri_gen)
+    return zq_fail(ctx, ZQ_ERR_STATE, "zq_mp2_resident: no device SCF result for the ERIs this context holds");
+  ZQ_TRY(ctx, hipSetDevice(ctx->device));
+  Arena ar{ctx};
+  return mp2_in(ctx, ctx->res_C, ctx->res_e, int(nocc), precision, e_corr, ar, true);
 }
 
 extern "C" int zq_mp2_noalloc(zq_ctx *ctx, const double *C, const double *e, size_t nocc, int precision,

@@ -20,7 +20,7 @@ EXPORTED = ["zq_create", "zq_destroy", "zq_last_error", "zq_eri_load", "zq_eri_l
             "zq_eri_set_bytes", "zq_attach", "zq_mem_info", "zq_fock_jk_multi", "zq_eigh",
             "zq_scf_last_driver", "zq_scf_last_sp2", "zq_scf_captures", "zq_scf_set_sp2_guess_budget",
             "zq_scf_set_diis_error_scale", "zq_scf_last_diis_resets", "zq_scf_last_replays", "zq_mp2_noalloc",
-            "zq_ao2mo_pp", "zq_ci", "zq_mp", "zq_grad_eri", "zq_ao2mo_dual", "zq_jk_record_count"]
+            "zq_ao2mo_pp", "zq_ci", "zq_mp", "zq_grad_eri", "zq_ao2mo_dual", "zq_jk_record_count", "zq_mp2_resident"]
 
 
 class GpuError(RuntimeError):
@@ -78,6 +78,7 @@ def lib():
         L.zq_ao2mo.argtypes = [vp, vp, sz, ci, ci, vp]
         L.zq_mp2.argtypes = [vp, vp, vp, sz, ci, vp]
         L.zq_mp2_noalloc.argtypes = [vp, vp, vp, sz, ci, vp]
+        L.zq_mp2_resident.argtypes = [vp, sz, ci, vp]
         L.zq_ao2mo_pp.argtypes = [vp, sz, vp, vp, vp]
         L.zq_grad_eri.argtypes = [vp, vp, sz, sz, vp, cd, vp]
         L.zq_ao2mo_dual.argtypes = [vp, vp, vp, vp, sz, ci, vp, vp]
@@ -241,6 +242,13 @@ class Context:
         C, e = _f64(C), _f64(e)
         out = np.zeros(1)
         self._check(self._L.zq_mp2(self._h, ptr(C), ptr(e), nocc, PRECISION[precision], ptr(out)))
+        return float(out[0])
+
+    def mp2_resident(self, nocc, precision="fp64"):
+        """MP2 correlation energy from the orbitals of the last device SCF on this context, which stay
+        on the device (zq_mp2_resident); the same value as mp2() with the C and e it returned."""
+        out = np.zeros(1)
+        self._check(self._L.zq_mp2_resident(self._h, nocc, PRECISION[precision], ptr(out)))
         return float(out[0])
 
     def ao2mo_dual(self, dset, C, dC, nocc=0, block="full"):

@@ -107,7 +107,7 @@ class Server:
         c.attach(self.resident[key])
         r = c.scf(m.H, m.S, m.VN, m.o, diis=8, iterations=100, threshold=md["threshold"], device=True,
                   fock_precision=md["fock"])
-        e2 = c.mp2(r["C"], r["e"], m.o, md["transform"])
+        e2 = c.mp2_resident(m.o, md["transform"])  # the SCF's orbitals, still on the device
         return dict(E_rhf=r["E"], E_corr=e2, E_mp2=r["E"] + e2, iterations=r["iterations"])
 
     def serve(self, requests):

@@ -470,6 +470,12 @@ struct zq_batch {
   size_t hc_cap = 0, hw_cap = 0, hi_cap = 0;
   hipStream_t copy_stream = nullptr;
   hipEvent_t coef_done = nullptr;
+  // the eigenpair stage's extra lanes: size groups of the batched eigensolver run side by side, each
+  // lane a stream with its own rocSOLVER handle (its own workspace); fork/join events
+  static constexpr int kEigLanes = 2;
+  hipStream_t eig_lane[kEigLanes] = {};
+  rocblas_handle eig_solver[kEigLanes] = {};
+  hipEvent_t eig_ev[kEigLanes + 1] = {};
   // results of the last run (C and e of molecule m at c_off[m], w_off[m] of h_c, h_w), and its stage
   // times (host wall clock, ms): SCF, eigenpairs, MP2, total
   std::vector<long long> c_off, w_off;
@@ -764,6 +770,13 @@ void zq_batch_destroy(zq_batch *b) {
   free_all(b);
   if (b->h_count) (void)hipHostFree(b->h_count);
   if (b->solver) rocblas_destroy_handle(b->solver);
+  for (int l = 0; l < zq_batch::kEigLanes; ++l) {
+    if (b->eig_lane[l]) (void)hipStreamSynchronize(b->eig_lane[l]);
+    if (b->eig_solver[l]) rocblas_destroy_handle(b->eig_solver[l]);
+    if (b->eig_lane[l]) (void)hipStreamDestroy(b->eig_lane[l]);
+  }
+  for (hipEvent_t e : b->eig_ev)
+    if (e) (void)hipEventDestroy(e);
   if (b->coef_done) (void)hipEventDestroy(b->coef_done);
   if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   (void)hipStreamDestroy(b->stream);
@@ -927,16 +940,50 @@ int zq_batch_run(zq_batch *b, const zq_scf_options *opt, int mp2_precision, doub
     if (rocblas_create_handle(&b->solver) != rocblas_status_success) return bfail(b, ZQ_ERR_LAPACK, "rocblas handle");
   }
   rocblas_set_stream(b->solver, b->stream);
+  for (int l = 0; l < zq_batch::kEigLanes; ++l) {
+    if (!b->eig_lane[l]) BTRY(b, hipStreamCreateWithFlags(&b->eig_lane[l], hipStreamNonBlocking));
+    if (!b->eig_solver[l]) {
+      if (rocblas_create_handle(&b->eig_solver[l]) != rocblas_status_success)
+        return bfail(b, ZQ_ERR_LAPACK, "rocblas handle");
+      rocblas_set_stream(b->eig_solver[l], b->eig_lane[l]);
+    }
+  }
+  for (hipEvent_t &e : b->eig_ev)
+    if (!e) BTRY(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // one strided batch per distinct n; the groups go to the least loaded lane (the batch's stream is
+  // lane 0), largest first by n^3 x count
+  struct Group {
+    int s, cnt, n;
+    double cost;
+  };
+  std::vector<Group> groups;
   for (int s = 0; s < M;) {
     const int n = b->mols[order[s]].n;
     int t = s;
     while (t < M && b->mols[order[t]].n == n) ++t;
-    const int cnt = t - s, m0 = order[s];
-    const rocblas_status rs = rocsolver_dsyevd_strided_batched(
-        b->solver, rocblas_evect_original, rocblas_fill_upper, n, d_eig + eoff[m0], n, rocblas_stride(n) * n,
-        d_w + woff[m0], rocblas_stride(n), d_e + woff[m0], rocblas_stride(n), d_info + s, cnt);
-    if (rs != rocblas_status_success) return bfail(b, ZQ_ERR_LAPACK, "rocsolver_dsyevd_strided_batched");
+    groups.push_back(Group{s, t - s, n, double(n) * n * n * (t - s)});
     s = t;
+  }
+  std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.cost > y.cost; });
+  BTRY(b, hipEventRecord(b->eig_ev[0], b->stream));  // F' is ready
+  for (int l = 0; l < zq_batch::kEigLanes; ++l) BTRY(b, hipStreamWaitEvent(b->eig_lane[l], b->eig_ev[0], 0));
+  double load[zq_batch::kEigLanes + 1] = {};
+  for (const Group &g : groups) {
+    int lane = 0;
+    for (int l = 1; l <= zq_batch::kEigLanes; ++l)
+      if (load[l] < load[lane]) lane = l;
+    load[lane] += g.cost;
+    const int m0 = order[g.s];
+    const rocblas_status rs = rocsolver_dsyevd_strided_batched(
+        lane == 0 ? b->solver : b->eig_solver[lane - 1], rocblas_evect_original, rocblas_fill_upper, g.n,
+        d_eig + eoff[m0], g.n, rocblas_stride(g.n) * g.n, d_w + woff[m0], rocblas_stride(g.n), d_e + woff[m0],
+        rocblas_stride(g.n), d_info + g.s, g.cnt);
+    if (rs != rocblas_status_success) return bfail(b, ZQ_ERR_LAPACK, "rocsolver_dsyevd_strided_batched");
+  }
+  // the batch's stream waits for every lane before the eigenvectors are used
+  for (int l = 0; l < zq_batch::kEigLanes; ++l) {
+    BTRY(b, hipEventRecord(b->eig_ev[1 + l], b->eig_lane[l]));
+    BTRY(b, hipStreamWaitEvent(b->stream, b->eig_ev[1 + l], 0));
   }
   if (b->nt == 512)
     hipLaunchKernelGGL(k_batch_coef<512>, dim3(M), dim3(512), 0, b->stream, b->ws, b->d_mols, d_eig, d_eoff, d_c, d_eoff);

@@ -127,8 +127,9 @@ __device__ bool sp2_estimate(int steps, double lo, double hi, const double *h, i
 // partials in gpart and counts itself in *gcount after a release fence; the last one, after an acquire
 // fence, combines them in workgroup order and clears the count.  track != 0 (one-launch SP2 of the
 // device SCF; 1 starts an SCF) also decides whether this run folds: the last workgroup turns the last
-// plain run's history into bounds, takes ||F - ref||_F against the reference buffer kEstRefSel names,
-// and writes kEstFold / kEstA0 / kEstB0.  The same pass stores F into the other buffer; a plain run
+// plain run's history into bounds, bounds the shift ||F - ref||_2 by the smaller of ||F - ref||_F and
+// the largest row sum of |F - ref| (ref: the buffer kEstRefSel names), and writes kEstFold / kEstA0 /
+// kEstB0.  The same pass stores F into the other buffer; a plain run
 // makes that one the reference.
 constexpr int kGershBlocks = 16;
 
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F
                                                     const int *__restrict__ skip, int track, int nocc,
                                                     double *__restrict__ est, double *__restrict__ ref,
                                                     double *__restrict__ gpart, unsigned *__restrict__ gcount) {
-  __shared__ double lo_s[4], hi_s[4], dd_s[4], hist_s[4 * kSp2MaxSteps];
+  __shared__ double lo_s[4], hi_s[4], dd_s[4], dm_s[4], hist_s[4 * kSp2MaxSteps];
   __shared__ int last_s;
   if (skip && *skip) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -154,12 +155,12 @@ __global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F
   }
   const double *rold = ref + size_t(sel) * n * n;
   double *rnew = ref + size_t(sel ^ 1) * n * n;
-  double lo = 1e300, hi = -1e300, dd = 0.0;
+  double lo = 1e300, hi = -1e300, dd = 0.0, dm = 0.0;  // dm: the largest row sum of |F' - F'_ref|
   const int gw = blockIdx.x * 4 + w, ngw = gridDim.x * 4;
   for (int i = gw; i < n; i += ngw) {
     const double *row = F + size_t(i) * n;
     const double d = row[i];
-    double rsum = 0.0;
+    double rsum = 0.0, rdif = 0.0;
     for (int j0 = 0; j0 < n; j0 += 256) {
       double f[4], r[4];
 #pragma unroll
@@ -173,21 +174,23 @@ __global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F
         const int j = j0 + 64 * u + lane;
         rsum += fabs(f[u]);
         const double df = f[u] - r[u];
-        if (track == 2) dd += df * df;
+        if (track == 2) dd += df * df, rdif += fabs(df);
         if (track && j < n) rnew[size_t(i) * n + j] = f[u];
       }
     }
     rsum = wave_sum(rsum) - fabs(d);
     lo = fmin(lo, d - rsum);
     hi = fmax(hi, d + rsum);
+    if (track == 2) dm = fmax(dm, wave_sum(rdif));
   }
   dd = wave_sum(dd);
-  if (lane == 0) lo_s[w] = lo, hi_s[w] = hi, dd_s[w] = dd;
+  if (lane == 0) lo_s[w] = lo, hi_s[w] = hi, dd_s[w] = dd, dm_s[w] = dm;
   __syncthreads();
   if (threadIdx.x == 0) {
-    gpart[3 * blockIdx.x] = fmin(fmin(lo_s[0], lo_s[1]), fmin(lo_s[2], lo_s[3]));
-    gpart[3 * blockIdx.x + 1] = fmax(fmax(hi_s[0], hi_s[1]), fmax(hi_s[2], hi_s[3]));
-    gpart[3 * blockIdx.x + 2] = ((dd_s[0] + dd_s[1]) + dd_s[2]) + dd_s[3];
+    gpart[4 * blockIdx.x] = fmin(fmin(lo_s[0], lo_s[1]), fmin(lo_s[2], lo_s[3]));
+    gpart[4 * blockIdx.x + 1] = fmax(fmax(hi_s[0], hi_s[1]), fmax(hi_s[2], hi_s[3]));
+    gpart[4 * blockIdx.x + 2] = ((dd_s[0] + dd_s[1]) + dd_s[2]) + dd_s[3];
+    gpart[4 * blockIdx.x + 3] = fmax(fmax(dm_s[0], dm_s[1]), fmax(dm_s[2], dm_s[3]));
     __threadfence();
     last_s = atomicAdd(gcount, 1u) == gridDim.x - 1;
   }
@@ -200,9 +203,10 @@ __global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F
   __syncthreads();
   if (threadIdx.x == 0) {
     *gcount = 0u;
-    double glo = gpart[0], ghi = gpart[1], dsum = gpart[2];
+    double glo = gpart[0], ghi = gpart[1], dsum = gpart[2], dmax = gpart[3];
     for (int k = 1; k < int(gridDim.x); ++k)
-      glo = fmin(glo, gpart[3 * k]), ghi = fmax(ghi, gpart[3 * k + 1]), dsum += gpart[3 * k + 2];
+      glo = fmin(glo, gpart[4 * k]), ghi = fmax(ghi, gpart[4 * k + 1]), dsum += gpart[4 * k + 2],
+      dmax = fmax(dmax, gpart[4 * k + 3]);
     // widen by a relative margin: the row sums above carry rounding errors
     const double pad = 1e-12 * fmax(fabs(glo), fabs(ghi));
     const double emin = glo - pad, emax = ghi + pad;
@@ -221,7 +225,10 @@ __global__ __launch_bounds__(256) void k_gershgorin(const double *__restrict__ F
       }
       est[kEstValid] = valid;
       if (valid != 0.0) {
-        const double D = sqrt(dsum);
+        // ||F' - F'_ref||_2 is at most its Frobenius norm and at most its largest absolute row sum
+        // (F' symmetric); the row sum is often the smaller one early in an SCF.  A relative pad covers
+        // the rounding of both sums.
+        const double D = fmin(sqrt(dsum), dmax) * (1.0 + 1e-12);
         const double lumo = el - D, homo = eh + D;
         if (lumo - homo > 0.5 * (el - eh)) {
           const double inv = 1.0 / fmax(emax - emin, 1e-10 * fmax(1.0, fabs(emax)));
@@ -610,7 +617,7 @@ int sp2_bufs(zq_ctx *ctx, int n, Sp2Bufs &b) {
   b.sync = (unsigned *)zq_scratch(ctx, "sp2_sync", 2 * kSp2MaxSteps * sizeof(unsigned));
   b.est = (double *)zq_scratch(ctx, "sp2_est", kEstSize * sizeof(double));
   b.ref = (double *)zq_scratch(ctx, "sp2_ref", 2 * size_t(n) * n * sizeof(double));
-  b.gpart = (double *)zq_scratch(ctx, "sp2_gpart", 3 * kGershBlocks * sizeof(double));
+  b.gpart = (double *)zq_scratch(ctx, "sp2_gpart", 4 * kGershBlocks * sizeof(double));
   bool fresh = false;  // the Gershgorin arrival count starts at zero and its last workgroup clears it again
   b.gcount = (unsigned *)zq_scratch(ctx, "sp2_gcount", sizeof(unsigned), &fresh);
   if (!b.X || !b.part || !b.scal || !b.sync || !b.est || !b.ref || !b.gpart || !b.gcount)
